@@ -1,0 +1,277 @@
+"""Angles2Backbone fwd+bwd benchmark on one MI355X.
+
+A step is one forward (angles -> coords) plus one backward (grad coords ->
+grad angles) of ``Angles2Backbone`` over one batch of the mixed1000 workload:
+B=1024 chains, lengths log-uniform in [30,1000], padded to 1000, float32
+angles/gradients, inputs resident in HBM.  The step is captured once per
+buffer set in a HIP graph (torch.cuda.graph) and replayed round-robin over
+enough buffer sets to exceed the 256 MB Infinity Cache, so replays stream
+from HBM.
+
+Prints one JSON line.  ``value`` is residues/s (sum of chain lengths per
+second).  The line also carries:
+* ``roofline``: the slower of the two kernels, timed alone with HIP events on
+  the stream it runs on; algorithmic bytes per launch / average duration
+  against the MI355X 8 TB/s HBM peak;
+* ``plain``: the same step with the plain kernels (serial forward, O(L^2)
+  backward) and the speedup of the scan path over it;
+* ``cpu_baseline``: the reference CPU algorithm on a bounded sample of the same
+  workload, on the host, one thread.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "fwd+bwd residues/s, mixed-L B=1024; batched adjoint vs plain; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+MALL_BYTES = 256 << 20
+
+
+def alg_bytes(n, L, io_bytes=4):
+    """Compulsory HBM bytes of one forward and one backward launch.
+
+    forward:  read angles (3 per residue), write coords (9 per slot incl. padding),
+              read lengths;
+    backward: read grad coords (9 per residue) and angles (3 per residue), write
+              grad angles (3 per slot incl. padding), read lengths."""
+    import numpy as np
+    B = len(n)
+    S = int(np.sum(n))
+    fwd = io_bytes * (3 * S + 9 * B * L) + 4 * B
+    bwd = io_bytes * (9 * S + 3 * S + 3 * B * L) + 4 * B
+    return fwd, bwd
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=2000)
+    p.add_argument("--warmup", type=int, default=200)
+    p.add_argument("--workload", default="mixed1000")
+    p.add_argument("--sets", type=int, default=0, help="buffer sets (0: enough to exceed the MALL)")
+    p.add_argument("--plain-steps", type=int, default=20)
+    p.add_argument("--kernel-reps", type=int, default=200)
+    p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--config", default="", help="pin tile config 'waves,res_per_lane,max_wgs'")
+    return p.parse_args()
+
+
+def capture_step(m, A, N, G):
+    import torch
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for _ in range(2):
+            A.grad = None
+            m(A, N).backward(G)
+    torch.cuda.current_stream().wait_stream(s)
+    g = torch.cuda.CUDAGraph()
+    A.grad = None
+    with torch.cuda.graph(g):
+        x = m(A, N)
+        x.backward(G)
+    return g, x
+
+
+def time_graphs(graphs, steps, warmup):
+    import torch
+    for i in range(warmup):
+        graphs[i % len(graphs)].replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for i in range(steps):
+        graphs[i % len(graphs)].replay()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / 1e3, wall
+
+
+def time_kernels(sets, cfg, reps):
+    """Average duration of the forward and of the backward launch alone, HIP
+    events on torch's current stream (the stream the extension launches on)."""
+    import torch
+    from torchproteinlibrary_amd._native import ext
+    cur = torch.cuda.current_stream()
+    res = {}
+    for name in ("forward", "backward"):
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(reps)]
+        for i in range(reps + 10):
+            A, N, G, X, GA, st, sy = sets[i % len(sets)]
+            rec = i >= 10
+            if rec:
+                evs[i - 10][0].record(cur)
+            if name == "forward":
+                ext.Angles2BackboneGPU_forward(A, X, N, st, sy, None, list(cfg))
+            else:
+                ext.Angles2BackboneGPU_backward(GA, G, A, N, st, sy, None, list(cfg))
+            if rec:
+                evs[i - 10][1].record(cur)
+        torch.cuda.synchronize()
+        ms = sorted(a.elapsed_time(b) for a, b in evs)
+        res[name] = {"avg_ms": sum(ms) / len(ms), "median_ms": ms[len(ms) // 2]}
+    return res
+
+
+def cpu_baseline(a, n, g, seconds):
+    """The reference CPU algorithm (FP64, single thread): its own kernels
+    compiled from the reference sources (oracle/_ref) if present, else this
+    repository's C restatement with the same derivative-tensor behaviour.
+    Chains are taken in workload order, one call per chain at the workload's
+    padded length, until the time budget is used."""
+    import numpy as np
+    from oracle import oracle as O
+    kind = "reference" if O.ref_available() else "port"
+    L = a.shape[2]
+    done_res, t_total, k = 0, 0.0, 0
+    while k < len(n) and t_total < seconds:
+        a1 = a[k:k + 1].astype(np.float64)
+        n1 = n[k:k + 1]
+        g1 = g[k:k + 1].astype(np.float64)
+        t0 = time.perf_counter()
+        if kind == "reference":
+            O.ref_forward_backward(a1, n1, g1)
+        else:
+            O.forward(a1, n1)
+            O.backward_dense(a1, n1, g1)
+        t_total += time.perf_counter() - t0
+        done_res += int(n1[0])
+        k += 1
+    return {
+        "value": done_res / t_total,
+        "unit": "residues/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": f"first {k} chains of the workload ({done_res} residues, L_max={L}), one "
+                  f"call per chain, FP64, dr_dangle materialised; {t_total:.1f} s on "
+                  f"{os.cpu_count()} visible host cores",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); "
+              "chains are independent, so batches shard across GPUs without exchange, "
+              "but no multi-GPU path is built or measured.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from torchproteinlibrary_amd import _tuning, synthetic
+    from torchproteinlibrary_amd._native import ext
+    from torchproteinlibrary_amd.ReducedModel import Angles2Backbone
+
+    os.environ.setdefault("TPL_A2B_CHECKS", "0")
+    dev = torch.device("cuda:0")
+    a, n, g = synthetic.workload(args.workload)
+    B, _, L = a.shape
+    residues = int(n.sum())
+    fwd_b, bwd_b = alg_bytes(n, L)
+    set_bytes = a.nbytes + g.nbytes + 4 * B + B * 9 * L * 4 + B * 3 * L * 4
+    sets_n = args.sets or max(2, -(-2 * MALL_BYTES // set_bytes))
+
+    A0 = torch.tensor(a, device=dev)
+    N0 = torch.tensor(n, device=dev)
+    if args.config:
+        _tuning.set_config(A0, tuple(int(v) for v in args.config.split(",")))
+    cfg = _tuning.config_for(A0, N0)  # tunes on the real inputs (eager)
+
+    # Buffer sets for the graphs and for the per-kernel timing.
+    gsets, ksets = [], []
+    st_b, sy_b = ext.workspace_bytes(B, L, list(cfg))
+    for _ in range(sets_n):
+        A = A0.clone().requires_grad_(True)
+        N = N0.clone()
+        G = torch.tensor(g, device=dev)
+        gsets.append((A, N, G))
+        ksets.append((A.detach(), N, G, torch.empty(B, 9 * L, device=dev),
+                      torch.empty(B, 3, L, device=dev),
+                      torch.empty(max(st_b, 1), dtype=torch.uint8, device=dev),
+                      torch.empty(max(sy_b, 1), dtype=torch.uint8, device=dev)))
+
+    m = Angles2Backbone()
+    graphs = [capture_step(m, A, N, G)[0] for A, N, G in gsets]
+    t_ev, t_wall = time_graphs(graphs, args.steps, args.warmup)
+    value = residues * args.steps / t_ev
+    ms_step = 1e3 * t_ev / args.steps
+
+    kern = time_kernels(ksets, cfg, args.kernel_reps)
+    dom = max(("forward", "backward"), key=lambda k: kern[k]["avg_ms"])
+    dom_bytes = fwd_b if dom == "forward" else bwd_b
+    achieved = dom_bytes / (kern[dom]["avg_ms"] * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get(dom)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "residues/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": args.workload,
+            "batch": B,
+            "max_length": L,
+            "residues": residues,
+            "io_dtype": "f32",
+            "tile": list(cfg),
+            "graph": True,
+            "buffer_sets": sets_n,
+            "step_alg_bytes": fwd_b + bwd_b,
+            "step_alg_GBs": (fwd_b + bwd_b) / (ms_step * 1e-3) / 1e9,
+            "wall_s": t_wall,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "alg_bytes_per_launch": dom_bytes,
+            "avg_ms": kern[dom]["avg_ms"],
+            "kernels": kern,
+        },
+    }
+
+    if not args.no_plain:
+        pm = Angles2Backbone("plain")
+        pg = [capture_step(pm, A, N, G)[0] for A, N, G in gsets[:2]]
+        pt, _ = time_graphs(pg, args.plain_steps, 2)
+        pv = residues * args.plain_steps / pt
+        out["plain"] = {"value": pv, "ms_per_step": 1e3 * pt / args.plain_steps,
+                        "speedup_scan_over_plain": value / pv}
+
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(a, n, g, args.cpu_seconds)
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

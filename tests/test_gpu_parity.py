@@ -1,0 +1,254 @@
+"""GPU parity: the HIP path against the FP64 oracle on the same seeded inputs.
+
+Tolerances (stated per test):
+* coordinates: the scan composes in FP64 and rounds once to float32, so the
+  bound is a fraction of an ulp of the coordinate magnitude:
+  |x_gpu - x_64| <= 1.2e-7 * max|x_64| + 1e-6 A  (<= 1e-4 A up to |x| = 800 A);
+* gradients: per chain ||g_gpu - g_64||_inf / ||g_64||_inf <= 1e-5;
+* exact zeros: padding coordinates, padding gradients, psi[len-1], omega[len-1].
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from torchproteinlibrary_amd import _tuning, synthetic
+from torchproteinlibrary_amd.ReducedModel import Angles2Backbone
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def run(a, n, g, cfg=None, dtype=torch.float32, form="scan"):
+    A = torch.tensor(a, device=DEV, dtype=dtype).contiguous().requires_grad_(True)
+    N = torch.tensor(np.asarray(n, np.int32), device=DEV)
+    if cfg is not None:
+        _tuning.set_config(A, cfg)
+    x = Angles2Backbone(form)(A, N)
+    x.backward(torch.tensor(g, device=DEV, dtype=dtype))
+    torch.cuda.synchronize()
+    return x.detach().cpu().numpy().astype(np.float64), A.grad.cpu().numpy().astype(np.float64)
+
+
+def oracle(a, n, g):
+    a64, g64 = np.asarray(a, np.float64), np.asarray(g, np.float64)
+    return O.forward(a64, n), O.backward_adjoint(a64, n, g64)
+
+
+def coord_tol(x64):
+    return 1.2e-7 * max(1.0, float(np.abs(x64).max())) + 1e-6
+
+
+def grad_rel_err(got, want, n):
+    errs = []
+    for b in range(len(n)):
+        if n[b] < 2:
+            continue
+        den = np.abs(want[b]).max()
+        errs.append(np.abs(got[b] - want[b]).max() / den if den > 0 else np.abs(got[b]).max())
+    return max(errs) if errs else 0.0
+
+
+def assert_exact_zeros(x, ga, n, L):
+    for b in range(len(n)):
+        assert np.all(x[b, 9 * n[b]:] == 0.0), f"chain {b}: padding coords not zero"
+        assert np.all(ga[b, :, n[b]:] == 0.0), f"chain {b}: padding grads not zero"
+        if n[b] > 0:
+            assert ga[b, 1, n[b] - 1] == 0.0 and ga[b, 2, n[b] - 1] == 0.0
+
+
+RAGGED = [300, 0, 1, 2, 64, 65, 127, 128, 257]
+CONFIGS = [
+    (1, 1, 0),   # 64-residue tiles: up to 5 tiles per chain, look-back
+    (1, 4, 0),   # 256-residue tiles: 2 tiles
+    (4, 1, 0),
+    (2, 2, 0),
+    (4, 2, 0),   # 512: one tile per chain
+    (4, 4, 0),
+    (1, 8, 0),
+    (4, 8, 0),
+    (1, 1, 3),   # persistent: 3 workgroups walk 45 tiles with look-back
+    (2, 1, 7),
+]
+
+
+@pytest.mark.parametrize("cfg", CONFIGS, ids=lambda c: "w%d_r%d_g%d" % c)
+def test_ragged_batch_all_configs(cfg):
+    a, n, g = synthetic.custom(RAGGED, 300, seed=41)
+    x, ga = run(a, n, g, cfg)
+    x64, g64 = oracle(a, n, g)
+    assert np.abs(x - x64).max() <= coord_tol(x64)
+    assert grad_rel_err(ga, g64, n) <= 1e-5
+    assert_exact_zeros(x, ga, n, 300)
+
+
+def test_stride_not_multiple_of_anything():
+    a, n, g = synthetic.custom([37, 1, 36, 13], 37, seed=5)
+    for cfg in [(1, 1, 0), (1, 2, 0), (4, 8, 0)]:
+        x, ga = run(a, n, g, cfg)
+        x64, g64 = oracle(a, n, g)
+        assert np.abs(x - x64).max() <= coord_tol(x64)
+        assert grad_rel_err(ga, g64, n) <= 1e-5
+        assert_exact_zeros(x, ga, n, 37)
+
+
+def test_mixed1000_full_size():
+    """B=1024, lengths log-uniform in [30,1000]: tuned launch shape."""
+    a, n, g = synthetic.workload("mixed1000")
+    x, ga = run(a, n, g)
+    x64, g64 = oracle(a, n, g)
+    err = np.abs(x - x64).max()
+    print(f"mixed1000: max coord err {err:.3e} A (max |x| {np.abs(x64).max():.1f}), "
+          f"grad rel err {grad_rel_err(ga, g64, n):.3e}, config {_tuning.cached_config(torch.empty(1024, 3, 1000, device=DEV))}")
+    assert err <= 1e-4
+    assert grad_rel_err(ga, g64, n) <= 1e-5
+    assert_exact_zeros(x, ga, n, 1000)
+
+
+def test_uniform700_with_finite_differences():
+    """B=256, L=700: oracle parity plus central FD (h=1e-3, FP64 oracle) on 64 angles."""
+    a, n, g = synthetic.workload("uniform700")
+    x, ga = run(a, n, g)
+    x64, g64 = oracle(a, n, g)
+    assert np.abs(x - x64).max() <= 1e-4
+    assert grad_rel_err(ga, g64, n) <= 1e-5
+    rng = np.random.default_rng(64)
+    a64, g64in = a.astype(np.float64), g.astype(np.float64)
+    h = 1e-3
+    worst = 0.0
+    for _ in range(64):
+        b = int(rng.integers(0, 256))
+        k = int(rng.integers(0, 3))
+        j = int(rng.integers(0, n[b] - 1 if k else n[b]))
+        ap, am = a64[b:b + 1].copy(), a64[b:b + 1].copy()
+        ap[0, k, j] += h
+        am[0, k, j] -= h
+        fd = (O.forward(ap, n[b:b + 1])[0] - O.forward(am, n[b:b + 1])[0]) @ g64in[b] / (2 * h)
+        scale = np.abs(g64[b]).max()
+        worst = max(worst, abs(ga[b, k, j] - fd) / scale)
+    print(f"uniform700 FD: max |g - fd| / ||g||_inf = {worst:.3e}")
+    assert worst <= 1e-4
+
+
+def test_long_chains_lookback():
+    """B=64 chains of 4096-16384 residues: multi-workgroup look-back on both
+    passes; report coordinate error growth with L."""
+    a, n, g = synthetic.workload("long16k")
+    for cfg in [(4, 4, 0), (2, 8, 0)]:
+        x, ga = run(a, n, g, cfg)
+        x64, g64 = oracle(a, n, g)
+        per_chain = np.abs(x - x64).max(axis=1)
+        order = np.argsort(n)
+        print("long16k", cfg, "coord err vs L:",
+              [(int(n[i]), float(per_chain[i])) for i in order[:: max(1, len(order) // 6)]])
+        assert np.all(per_chain <= 1.2e-7 * np.abs(x64).max(axis=1) + 1e-6)
+        assert grad_rel_err(ga, g64, n) <= 1e-5
+        assert_exact_zeros(x, ga, n, 16384)
+
+
+def test_float64_io():
+    a, n, g = synthetic.custom([50, 3, 0, 49, 17], 50, seed=8)
+    a = a.astype(np.float64) + 0.01  # non-f32-representable values
+    for i, k in enumerate(n):
+        a[i, :, k:] = 0
+    for cfg in [(1, 1, 0), (4, 2, 0)]:
+        x, ga = run(a, n, g.astype(np.float64), cfg, dtype=torch.float64)
+        x64, g64 = oracle(a, n, g)
+        assert np.abs(x - x64).max() <= 1e-10
+        assert grad_rel_err(ga, g64, n) <= 1e-10
+        assert_exact_zeros(x, ga, n, 50)
+
+
+def test_plain_form():
+    """The baseline kernels (FP32 serial forward, O(L^2) per-pair backward)."""
+    a, n, g = synthetic.custom([200, 120, 1, 0, 77, 2, 199, 64], 200, seed=12)
+    x, ga = run(a, n, g, form="plain")
+    x64, g64 = oracle(a, n, g)
+    assert np.abs(x - x64).max() <= 5e-3  # FP32 serial recurrence
+    assert grad_rel_err(ga, g64, n) <= 1e-3
+    assert_exact_zeros(x, ga, n, 200)
+    x, ga = run(a, n, g.astype(np.float64), dtype=torch.float64, form="plain")
+    assert np.abs(x - x64).max() <= 1e-10
+    assert grad_rel_err(ga, g64, n) <= 1e-9
+
+
+def test_reference_gradcheck_fp32():
+    """UnitTests/ReducedModel/Angles2Backbone/test.py:72-82: FP32 gradcheck,
+    B=8, L=8, lengths in [4,8), eps=1e-3, atol=1e-2, rtol=0.01."""
+    torch.manual_seed(0)
+    A = torch.randn(8, 3, 8, dtype=torch.float, device=DEV).requires_grad_()
+    N = torch.zeros(8, dtype=torch.int, device=DEV).random_(4, 8)
+    assert torch.autograd.gradcheck(Angles2Backbone(), (A, N), eps=1e-3, atol=1e-2, rtol=0.01)
+
+
+def test_gradcheck_fp64_tight():
+    torch.manual_seed(1)
+    A = torch.randn(4, 3, 7, dtype=torch.float64, device=DEV).requires_grad_()
+    N = torch.tensor([7, 5, 1, 3], dtype=torch.int, device=DEV)
+    for cfg in [(1, 1, 0), (4, 1, 0)]:
+        _tuning.set_config(A, cfg)
+        assert torch.autograd.gradcheck(Angles2Backbone(), (A, N), eps=1e-6, atol=1e-8, rtol=1e-7)
+
+
+def test_errors():
+    m = Angles2Backbone()
+    a, n, _ = synthetic.custom([5, 4], 5, seed=1)
+    A = torch.tensor(a, device=DEV)
+    N = torch.tensor(n, device=DEV)
+    bad = A.clone()
+    bad[0, 1, 2] = float("nan")
+    with pytest.raises(Exception, match="forward Nan"):
+        m(bad, N)
+    with pytest.raises(ValueError):
+        m(A, torch.tensor([6, 1], dtype=torch.int32, device=DEV))
+    with pytest.raises(RuntimeError):
+        m(A.transpose(1, 2).contiguous().transpose(1, 2), N)
+    with pytest.raises(RuntimeError):
+        m(A, N.long())
+    with pytest.raises(Exception, match="Incorrect input size"):
+        m(A[0], N)
+    with pytest.raises(RuntimeError):
+        m(A, N[:1])
+
+
+def test_graph_capture_replays_match_eager():
+    a, n, g = synthetic.workload("fit500", batch=64)
+    A = torch.tensor(a, device=DEV).requires_grad_(True)
+    N = torch.tensor(n, device=DEV)
+    G = torch.tensor(g, device=DEV)
+    m = Angles2Backbone()
+    x_eager = m(A, N)
+    x_eager.backward(G)
+    want_x, want_g = x_eager.detach().clone(), A.grad.detach().clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for _ in range(2):
+            A.grad = None
+            m(A, N).backward(G)
+    torch.cuda.current_stream().wait_stream(s)
+    graph = torch.cuda.CUDAGraph()
+    A.grad = None
+    with torch.cuda.graph(graph):
+        x_static = m(A, N)
+        x_static.backward(G)
+    for rep in range(4):
+        junk = torch.randn(1000, device=DEV).sum()  # eager work between replays
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(x_static, want_x), f"replay {rep}: coords differ"
+        assert torch.equal(A.grad, want_g), f"replay {rep}: grads differ"
+        del junk
+
+
+def test_tuner_picks_a_candidate():
+    a, n, g = synthetic.workload("fit500", batch=128)
+    A = torch.tensor(a, device=DEV)
+    N = torch.tensor(n, device=DEV)
+    cfg = _tuning.tune_now(A, N)
+    assert cfg in [tuple(c) for c in _tuning.candidates(500)]
+    t = _tuning.timings(A)
+    assert t and all(v > 0 for v in t.values())
+    print("fit500 B=128 tuner timings (ms):", t)

@@ -1,0 +1,6 @@
+"""Reference: TorchProteinLibrary/ReducedModel/Angles2Backbone/__init__.py:1."""
+from .Angles2Backbone import (  # noqa: F401
+    Angles2Backbone,
+    Angles2BackboneGPUFunction,
+    Angles2BackbonePlainGPUFunction,
+)

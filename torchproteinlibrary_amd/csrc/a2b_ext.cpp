@@ -1,0 +1,226 @@
+// a2b_ext.cpp -- pybind11 module `_ReducedModel`: the tensor-facing glue over
+// the C ABI in include/tpl_a2b.h.
+//
+// Mirrors the reference module of the same name (Layers/ReducedModel/main.cpp:4-9)
+// and its checks (Layers/ReducedModel/Angles2Backbone/angles2backbone_interface.cpp:12-18,
+// Math/nUtil.h:49-57): device, contiguity and dtype checks raise RuntimeError.
+// The ndim check that the reference's no-op ERROR() macro skips is enforced.
+// Work goes to torch's current HIP stream of the input's device, not the
+// legacy default stream (cBackboneProteinCUDAKernels.cu:166,172,177).
+#include <torch/extension.h>
+#include <c10/hip/HIPStream.h>
+
+#include <tuple>
+#include <vector>
+
+#include "tpl_a2b.h"
+
+namespace {
+
+#define A2B_CHECK_GPU(x)                                                      \
+    TORCH_CHECK((x).is_cuda(), #x " must be a GPU tensor");                   \
+    TORCH_CHECK((x).is_contiguous(), #x " must be contiguous")
+
+void check_rc(int rc, const char* what) {
+    TORCH_CHECK(rc == TPL_A2B_OK, what, " failed: ", tpl_a2b_strerror(rc));
+}
+
+int dtype_code(const torch::Tensor& t) {
+    if (t.scalar_type() == torch::kFloat) return TPL_A2B_F32;
+    if (t.scalar_type() == torch::kDouble) return TPL_A2B_F64;
+    TORCH_CHECK(false, "Angles2Backbone: angles must be float32 or float64, got ",
+                t.scalar_type());
+    return -1;
+}
+
+tpl_a2b_config to_config(const std::vector<int64_t>& c) {
+    TORCH_CHECK(c.size() == 3, "config must be (waves, res_per_lane, max_wgs)");
+    tpl_a2b_config cfg{};
+    cfg.waves = (int32_t)c[0];
+    cfg.res_per_lane = (int32_t)c[1];
+    cfg.max_wgs = (int32_t)c[2];
+    cfg.flags = 0;
+    TORCH_CHECK(tpl_a2b_tile_residues(&cfg) > 0, "unsupported config (waves=", c[0],
+                ", res_per_lane=", c[1], ", max_wgs=", c[2], ")");
+    return cfg;
+}
+
+std::vector<int64_t> from_config(const tpl_a2b_config& c) {
+    return {c.waves, c.res_per_lane, c.max_wgs};
+}
+
+// Validates angles [B,3,L] + lengths int32 [B] on one device; returns (B, L).
+std::pair<int, int> check_inputs(const torch::Tensor& angles, const torch::Tensor& lengths) {
+    A2B_CHECK_GPU(angles);
+    A2B_CHECK_GPU(lengths);
+    TORCH_CHECK(lengths.scalar_type() == torch::kInt, "angles_length must be int32");
+    TORCH_CHECK(angles.dim() == 3 && angles.size(1) == 3,
+                "input_angles must be [batch, 3, max_length], got ", angles.sizes());
+    TORCH_CHECK(lengths.dim() == 1 && lengths.size(0) == angles.size(0),
+                "angles_length must be [batch]");
+    TORCH_CHECK(lengths.device() == angles.device(), "angles_length must be on the device of input_angles");
+    TORCH_CHECK(angles.size(0) <= INT32_MAX && angles.size(2) <= INT32_MAX, "input too large");
+    return {(int)angles.size(0), (int)angles.size(2)};
+}
+
+void check_out(const torch::Tensor& t, const torch::Tensor& like, std::vector<int64_t> shape,
+               const char* name) {
+    A2B_CHECK_GPU(t);
+    TORCH_CHECK(t.scalar_type() == like.scalar_type(), name, " must have the dtype of the angles");
+    TORCH_CHECK(t.device() == like.device(), name, " must be on the device of the angles");
+    TORCH_CHECK(t.sizes() == torch::IntArrayRef(shape), name, " has shape ", t.sizes());
+}
+
+void* opt_ptr(const torch::Tensor& t, size_t need, const char* name, const torch::Tensor& like) {
+    if (need == 0) return nullptr;
+    A2B_CHECK_GPU(t);
+    TORCH_CHECK(t.device() == like.device(), name, " must be on the device of the angles");
+    TORCH_CHECK((size_t)t.nbytes() >= need, name, " needs ", need, " bytes, has ", t.nbytes());
+    return t.data_ptr();
+}
+
+uint32_t* status_ptr(const c10::optional<torch::Tensor>& status, const torch::Tensor& like) {
+    if (!status.has_value() || !status->defined()) return nullptr;
+    A2B_CHECK_GPU(*status);
+    TORCH_CHECK(status->scalar_type() == torch::kInt && status->numel() >= 1,
+                "status must be an int32 tensor");
+    TORCH_CHECK(status->device() == like.device(), "status must be on the device of the angles");
+    return reinterpret_cast<uint32_t*>(status->data_ptr());
+}
+
+hipStream_t stream_of(const torch::Tensor& t) {
+    return c10::hip::getCurrentHIPStream(t.device().index()).stream();
+}
+
+// ----------------------------------------------------------------- bindings
+
+// Angles2BackboneGPU_forward (reference angles2backbone_interface.cpp:7-26):
+// (input_angles, output_coords, angles_length, A) with A replaced by the scan
+// checkpoint state, plus the look-back scratch, optional status and config.
+void forward(torch::Tensor input_angles, torch::Tensor output_coords,
+             torch::Tensor angles_length, torch::Tensor state, torch::Tensor sync,
+             c10::optional<torch::Tensor> status, std::vector<int64_t> config) {
+    auto [B, L] = check_inputs(input_angles, angles_length);
+    check_out(output_coords, input_angles, {B, 9 * (int64_t)L}, "output_coords");
+    const tpl_a2b_config cfg = to_config(config);
+    c10::DeviceGuard guard(input_angles.device());
+    void* st = opt_ptr(state, tpl_a2b_state_bytes(B, L, &cfg), "state", input_angles);
+    void* sy = opt_ptr(sync, tpl_a2b_sync_bytes(B, L, &cfg), "sync", input_angles);
+    check_rc(tpl_a2b_forward(dtype_code(input_angles), input_angles.data_ptr(),
+                             output_coords.data_ptr(), angles_length.data_ptr<int32_t>(), st, sy,
+                             status_ptr(status, input_angles), B, L, &cfg,
+                             stream_of(input_angles)),
+             "Angles2BackboneGPU_forward");
+}
+
+// Angles2BackboneGPU_backward (reference angles2backbone_interface.cpp:27-56):
+// (gradInput, gradOutput, input_angles, angles_length, A, dr_dangle) with A ->
+// state and dr_dangle -> look-back scratch.
+void backward(torch::Tensor grad_input, torch::Tensor grad_output, torch::Tensor input_angles,
+              torch::Tensor angles_length, torch::Tensor state, torch::Tensor sync,
+              c10::optional<torch::Tensor> status, std::vector<int64_t> config) {
+    auto [B, L] = check_inputs(input_angles, angles_length);
+    check_out(grad_input, input_angles, {B, 3, L}, "gradInput");
+    check_out(grad_output, input_angles, {B, 9 * (int64_t)L}, "gradOutput");
+    const tpl_a2b_config cfg = to_config(config);
+    c10::DeviceGuard guard(input_angles.device());
+    void* st = opt_ptr(state, tpl_a2b_state_bytes(B, L, &cfg), "state", input_angles);
+    void* sy = opt_ptr(sync, tpl_a2b_sync_bytes(B, L, &cfg), "sync", input_angles);
+    check_rc(tpl_a2b_backward(dtype_code(input_angles), grad_input.data_ptr(),
+                              grad_output.data_ptr(), input_angles.data_ptr(),
+                              angles_length.data_ptr<int32_t>(), st, sy,
+                              status_ptr(status, input_angles), B, L, &cfg,
+                              stream_of(input_angles)),
+             "Angles2BackboneGPU_backward");
+}
+
+void forward_plain(torch::Tensor input_angles, torch::Tensor output_coords,
+                   torch::Tensor angles_length, torch::Tensor frames,
+                   c10::optional<torch::Tensor> status) {
+    auto [B, L] = check_inputs(input_angles, angles_length);
+    check_out(output_coords, input_angles, {B, 9 * (int64_t)L}, "output_coords");
+    const int dt = dtype_code(input_angles);
+    c10::DeviceGuard guard(input_angles.device());
+    void* fr = opt_ptr(frames, tpl_a2b_plain_frames_bytes(dt, B, L), "frames", input_angles);
+    check_rc(tpl_a2b_forward_plain(dt, input_angles.data_ptr(), output_coords.data_ptr(),
+                                   angles_length.data_ptr<int32_t>(), fr,
+                                   status_ptr(status, input_angles), B, L,
+                                   stream_of(input_angles)),
+             "Angles2BackboneGPU_forward_plain");
+}
+
+void backward_plain(torch::Tensor grad_input, torch::Tensor grad_output,
+                    torch::Tensor input_angles, torch::Tensor angles_length,
+                    torch::Tensor frames, c10::optional<torch::Tensor> status) {
+    auto [B, L] = check_inputs(input_angles, angles_length);
+    check_out(grad_input, input_angles, {B, 3, L}, "gradInput");
+    check_out(grad_output, input_angles, {B, 9 * (int64_t)L}, "gradOutput");
+    const int dt = dtype_code(input_angles);
+    c10::DeviceGuard guard(input_angles.device());
+    void* fr = opt_ptr(frames, tpl_a2b_plain_frames_bytes(dt, B, L), "frames", input_angles);
+    check_rc(tpl_a2b_backward_plain(dt, grad_input.data_ptr(), grad_output.data_ptr(),
+                                    input_angles.data_ptr(), angles_length.data_ptr<int32_t>(),
+                                    fr, status_ptr(status, input_angles), B, L,
+                                    stream_of(input_angles)),
+             "Angles2BackboneGPU_backward_plain");
+}
+
+std::tuple<int64_t, int64_t> workspace_bytes(int64_t B, int64_t L, std::vector<int64_t> config) {
+    const tpl_a2b_config cfg = to_config(config);
+    return {(int64_t)tpl_a2b_state_bytes((int)B, (int)L, &cfg),
+            (int64_t)tpl_a2b_sync_bytes((int)B, (int)L, &cfg)};
+}
+
+int64_t plain_frames_bytes(bool f64, int64_t B, int64_t L) {
+    return (int64_t)tpl_a2b_plain_frames_bytes(f64 ? TPL_A2B_F64 : TPL_A2B_F32, (int)B, (int)L);
+}
+
+std::vector<int64_t> default_config(int64_t B, int64_t L) {
+    tpl_a2b_config cfg{};
+    check_rc(tpl_a2b_default_config((int)B, (int)L, &cfg), "default_config");
+    return from_config(cfg);
+}
+
+std::tuple<std::vector<int64_t>, std::vector<double>> tune(
+    torch::Tensor input_angles, torch::Tensor angles_length, torch::Tensor grad_output,
+    torch::Tensor output_coords, torch::Tensor grad_input, torch::Tensor state, torch::Tensor sync,
+    std::vector<std::vector<int64_t>> candidates, int64_t reps) {
+    auto [B, L] = check_inputs(input_angles, angles_length);
+    check_out(output_coords, input_angles, {B, 9 * (int64_t)L}, "output_coords");
+    check_out(grad_input, input_angles, {B, 3, L}, "gradInput");
+    check_out(grad_output, input_angles, {B, 9 * (int64_t)L}, "gradOutput");
+    TORCH_CHECK(!candidates.empty(), "no candidates");
+    std::vector<tpl_a2b_config> cands;
+    size_t need_state = 0, need_sync = 0;
+    for (auto& c : candidates) {
+        cands.push_back(to_config(c));
+        need_state = std::max(need_state, tpl_a2b_state_bytes(B, L, &cands.back()));
+        need_sync = std::max(need_sync, tpl_a2b_sync_bytes(B, L, &cands.back()));
+    }
+    c10::DeviceGuard guard(input_angles.device());
+    void* st = opt_ptr(state, need_state, "state", input_angles);
+    void* sy = opt_ptr(sync, need_sync, "sync", input_angles);
+    std::vector<float> ms(cands.size(), -1.f);
+    tpl_a2b_config best{};
+    check_rc(tpl_a2b_tune(dtype_code(input_angles), input_angles.data_ptr(),
+                          angles_length.data_ptr<int32_t>(), grad_output.data_ptr(),
+                          output_coords.data_ptr(), grad_input.data_ptr(), st, sy, B, L,
+                          cands.data(), (int)cands.size(), (int)reps, &best, ms.data(),
+                          stream_of(input_angles)),
+             "tune");
+    return {from_config(best), std::vector<double>(ms.begin(), ms.end())};
+}
+
+}  // namespace
+
+PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
+    m.doc() = "Angles2Backbone for MI355X: tensor glue over the tpl_a2b C ABI";
+    m.def("Angles2BackboneGPU_forward", &forward, "Angles2Backbone gpu forward (scan)");
+    m.def("Angles2BackboneGPU_backward", &backward, "Angles2Backbone gpu backward (O(L) adjoint)");
+    m.def("Angles2BackboneGPU_forward_plain", &forward_plain, "plain serial forward");
+    m.def("Angles2BackboneGPU_backward_plain", &backward_plain, "plain O(L^2) backward");
+    m.def("workspace_bytes", &workspace_bytes, "(state_bytes, sync_bytes) for a config");
+    m.def("plain_frames_bytes", &plain_frames_bytes, "frame buffer bytes of the plain form");
+    m.def("default_config", &default_config, "untuned config for (batch, max_length)");
+    m.def("tune", &tune, "time candidate configs; returns (best, ms per candidate)");
+}

@@ -3,10 +3,11 @@
 A step is one forward (angles -> coords) plus one backward (grad coords ->
 grad angles) of ``Angles2Backbone`` over one batch of the mixed1000 workload:
 B=1024 chains, lengths log-uniform in [30,1000], padded to 1000, float32
-angles/gradients, inputs resident in HBM.  The step is captured once per
-buffer set in a HIP graph (torch.cuda.graph) and replayed round-robin over
-enough buffer sets to exceed the 256 MB Infinity Cache, so replays stream
-from HBM.
+angles/gradients, inputs resident in HBM.  Steps are launched back to back
+from a native loop (the extension's ``bench_steps``: forward launch, backward
+launch, next buffer set) and timed with HIP events on the launch stream.  The
+loop rotates over enough buffer sets to exceed the 256 MB Infinity Cache, so
+every step streams from HBM.
 
 Prints one JSON line.  ``value`` is residues/s (sum of chain lengths per
 second).  The line also carries:
@@ -63,39 +64,6 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--config", default="", help="pin tile config 'waves,res_per_lane,max_wgs'")
     return p.parse_args()
-
-
-def capture_step(m, A, N, G):
-    import torch
-    s = torch.cuda.Stream()
-    s.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(s):
-        for _ in range(2):
-            A.grad = None
-            m(A, N).backward(G)
-    torch.cuda.current_stream().wait_stream(s)
-    g = torch.cuda.CUDAGraph()
-    A.grad = None
-    with torch.cuda.graph(g):
-        x = m(A, N)
-        x.backward(G)
-    return g, x
-
-
-def time_graphs(graphs, steps, warmup):
-    import torch
-    for i in range(warmup):
-        graphs[i % len(graphs)].replay()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record()
-    for i in range(steps):
-        graphs[i % len(graphs)].replay()
-    e1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    return e0.elapsed_time(e1) / 1e3, wall
 
 
 def time_kernels(sets, cfg, reps):
@@ -190,7 +158,7 @@ def main():
         _tuning.set_config(A0, tuple(int(v) for v in args.config.split(",")))
     cfg = _tuning.config_for(A0, N0)  # tunes on the real inputs (eager)
 
-    # Buffer sets for the graphs and for the per-kernel timing.
+    # Buffer sets: autograd tensors (gsets) and raw launch buffers (ksets).
     gsets, ksets = [], []
     st_b, sy_b = ext.workspace_bytes(B, L, list(cfg))
     for _ in range(sets_n):
@@ -204,10 +172,15 @@ def main():
                       torch.empty(max(sy_b, 1), dtype=torch.uint8, device=dev)))
 
     m = Angles2Backbone()
-    graphs = [capture_step(m, A, N, G)[0] for A, N, G in gsets]
-    t_ev, t_wall = time_graphs(graphs, args.steps, args.warmup)
+    for A, N, G in gsets[:1]:  # one eager autograd step through the public API
+        m(A, N).backward(G)
+    torch.cuda.synchronize()
+    ev_ms, wall_ms = ext.bench_steps([list(k) for k in ksets], list(cfg), args.steps,
+                                     args.warmup, False)
+    t_ev = ev_ms / 1e3
     value = residues * args.steps / t_ev
-    ms_step = 1e3 * t_ev / args.steps
+    ms_step = ev_ms / args.steps
+    t_wall = wall_ms / 1e3
 
     kern = time_kernels(ksets, cfg, args.kernel_reps)
     dom = max(("forward", "backward"), key=lambda k: kern[k]["avg_ms"])
@@ -239,7 +212,7 @@ def main():
             "residues": residues,
             "io_dtype": "f32",
             "tile": list(cfg),
-            "graph": True,
+            "launch": "native loop, back-to-back fwd+bwd launches",
             "buffer_sets": sets_n,
             "step_alg_bytes": fwd_b + bwd_b,
             "step_alg_GBs": (fwd_b + bwd_b) / (ms_step * 1e-3) / 1e9,
@@ -260,9 +233,11 @@ def main():
     }
 
     if not args.no_plain:
-        pm = Angles2Backbone("plain")
-        pg = [capture_step(pm, A, N, G)[0] for A, N, G in gsets[:2]]
-        pt, _ = time_graphs(pg, args.plain_steps, 2)
+        fr_b = ext.plain_frames_bytes(False, B, L)
+        psets = [[A, N, G, X, GA, torch.empty(fr_b, dtype=torch.uint8, device=dev), st]
+                 for (A, N, G, X, GA, _, st) in ksets[:2]]
+        pms, _ = ext.bench_steps(psets, list(cfg), args.plain_steps, 2, True)
+        pt = pms / 1e3
         pv = residues * args.plain_steps / pt
         out["plain"] = {"value": pv, "ms_per_step": 1e3 * pt / args.plain_steps,
                         "speedup_scan_over_plain": value / pv}

@@ -10,6 +10,7 @@
 #include <torch/extension.h>
 #include <c10/hip/HIPStream.h>
 
+#include <chrono>
 #include <tuple>
 #include <vector>
 
@@ -211,6 +212,70 @@ std::tuple<std::vector<int64_t>, std::vector<double>> tune(
     return {from_config(best), std::vector<double>(ms.begin(), ms.end())};
 }
 
+// Times `steps` back-to-back steps (forward then backward launch) on the
+// current stream, rotating over the buffer sets, after `warmup` untimed steps.
+// Each set is [angles, lengths, grad_coords, coords, grad_angles, state, sync]
+// (or, for plain=true, [..., frames, unused]).  Returns (event ms, wall ms).
+std::tuple<double, double> bench_steps(std::vector<std::vector<torch::Tensor>> sets,
+                                       std::vector<int64_t> config, int64_t steps,
+                                       int64_t warmup, bool plain) {
+    TORCH_CHECK(!sets.empty() && steps > 0 && warmup >= 0, "bench_steps: bad arguments");
+    for (auto& s : sets) TORCH_CHECK(s.size() == 7, "bench_steps: a set has 7 tensors");
+    const tpl_a2b_config cfg = to_config(config);
+    const torch::Tensor& a0 = sets[0][0];
+    auto [B, L] = check_inputs(a0, sets[0][1]);
+    const int dt = dtype_code(a0);
+    c10::DeviceGuard guard(a0.device());
+    hipStream_t stream = stream_of(a0);
+    auto one = [&](const std::vector<torch::Tensor>& s) {
+        int rc;
+        if (plain) {
+            rc = tpl_a2b_forward_plain(dt, s[0].data_ptr(), s[3].data_ptr(), s[1].data_ptr<int32_t>(),
+                                       s[5].data_ptr(), nullptr, B, L, stream);
+            if (rc == TPL_A2B_OK)
+                rc = tpl_a2b_backward_plain(dt, s[4].data_ptr(), s[2].data_ptr(), s[0].data_ptr(),
+                                            s[1].data_ptr<int32_t>(), s[5].data_ptr(), nullptr, B,
+                                            L, stream);
+        } else {
+            rc = tpl_a2b_forward(dt, s[0].data_ptr(), s[3].data_ptr(), s[1].data_ptr<int32_t>(),
+                                 s[5].data_ptr(), s[6].data_ptr(), nullptr, B, L, &cfg, stream);
+            if (rc == TPL_A2B_OK)
+                rc = tpl_a2b_backward(dt, s[4].data_ptr(), s[2].data_ptr(), s[0].data_ptr(),
+                                      s[1].data_ptr<int32_t>(), s[5].data_ptr(), s[6].data_ptr(),
+                                      nullptr, B, L, &cfg, stream);
+        }
+        check_rc(rc, "bench_steps");
+    };
+    for (auto& s : sets) {  // every set is checked once, as the bindings above do
+        check_inputs(s[0], s[1]);
+        check_out(s[2], a0, {B, 9 * (int64_t)L}, "gradOutput");
+        check_out(s[3], a0, {B, 9 * (int64_t)L}, "output_coords");
+        check_out(s[4], a0, {B, 3, L}, "gradInput");
+        if (plain) {
+            opt_ptr(s[5], tpl_a2b_plain_frames_bytes(dt, B, L), "frames", a0);
+        } else {
+            opt_ptr(s[5], tpl_a2b_state_bytes(B, L, &cfg), "state", a0);
+            opt_ptr(s[6], tpl_a2b_sync_bytes(B, L, &cfg), "sync", a0);
+        }
+    }
+    for (int64_t i = 0; i < warmup; ++i) one(sets[i % sets.size()]);
+    hipEvent_t e0, e1;
+    TORCH_CHECK(hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess,
+                "hipEventCreate failed");
+    TORCH_CHECK(hipStreamSynchronize(stream) == hipSuccess, "hipStreamSynchronize failed");
+    const auto t0 = std::chrono::steady_clock::now();
+    TORCH_CHECK(hipEventRecord(e0, stream) == hipSuccess, "hipEventRecord failed");
+    for (int64_t i = 0; i < steps; ++i) one(sets[i % sets.size()]);
+    TORCH_CHECK(hipEventRecord(e1, stream) == hipSuccess, "hipEventRecord failed");
+    TORCH_CHECK(hipEventSynchronize(e1) == hipSuccess, "hipEventSynchronize failed");
+    const auto t1 = std::chrono::steady_clock::now();
+    float ms = 0.f;
+    TORCH_CHECK(hipEventElapsedTime(&ms, e0, e1) == hipSuccess, "hipEventElapsedTime failed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return {(double)ms, std::chrono::duration<double, std::milli>(t1 - t0).count()};
+}
+
 }  // namespace
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
@@ -223,4 +288,5 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("plain_frames_bytes", &plain_frames_bytes, "frame buffer bytes of the plain form");
     m.def("default_config", &default_config, "untuned config for (batch, max_length)");
     m.def("tune", &tune, "time candidate configs; returns (best, ms per candidate)");
+    m.def("bench_steps", &bench_steps, "time back-to-back fwd+bwd steps (event ms, wall ms)");
 }

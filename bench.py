@@ -63,6 +63,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--config", default="", help="pin tile config 'waves,res_per_lane,max_wgs'")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="also run 5 device copies of 256 MiB (counter calibration)")
     return p.parse_args()
 
 
@@ -244,6 +246,14 @@ def main():
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a, n, g, args.cpu_seconds)
+
+    if args.calibrate_copy:
+        src = torch.ones(256 << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["calibration_copy_bytes"] = src.numel()
 
     print(json.dumps(out), flush=True)
 

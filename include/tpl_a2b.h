@@ -62,7 +62,9 @@ enum tpl_a2b_error {
 
 /* Launch shape of the scan kernels.  A tile is 64*waves*res_per_lane
  * consecutive residues of one chain.  One workgroup scans one tile.  Chains
- * longer than a tile are chained across workgroups by decoupled look-back. */
+ * longer than a tile are chained across workgroups by decoupled look-back.
+ * A tile holds at most 1024 residues for TPL_A2B_F32 and 512 for TPL_A2B_F64:
+ * its I/O is staged in LDS.  Larger tiles return TPL_A2B_ERR_CONFIG. */
 typedef struct tpl_a2b_config {
     int32_t waves;        /* 64-lane waves per workgroup: 1, 2 or 4 */
     int32_t res_per_lane; /* residues composed serially per lane: 1, 2, 4 or 8 */

@@ -59,6 +59,11 @@ def test_argument_errors_return_before_launch(lib):
     assert lib.tpl_a2b_forward(0, p, p, p, None, None, None, -1, 1, ctypes.byref(c), None) == 1
     bad = abi.Config(3, 1, 0, 0)
     assert lib.tpl_a2b_forward(0, p, p, p, None, None, None, 1, 1, ctypes.byref(bad), None) == 2
+    # tiles above the LDS staging limit: 1024 residues (f32), 512 (f64)
+    big = abi.Config(4, 8, 0, 0)
+    assert lib.tpl_a2b_forward(0, p, p, p, p, p, None, 1, 1, ctypes.byref(big), None) == 2
+    f64big = abi.Config(4, 4, 0, 0)
+    assert lib.tpl_a2b_backward(1, p, p, p, p, p, p, None, 1, 1, ctypes.byref(f64big), None) == 2
     # multi-tile launch without state/scratch buffers
     assert lib.tpl_a2b_forward(0, p, p, p, None, None, None, 2, 1000, ctypes.byref(c), None) == 1
     assert lib.tpl_a2b_backward(0, p, p, p, p, None, None, None, 2, 1000, ctypes.byref(c), None) == 1

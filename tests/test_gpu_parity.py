@@ -68,7 +68,7 @@ CONFIGS = [
     (4, 2, 0),   # 512: one tile per chain
     (4, 4, 0),
     (1, 8, 0),
-    (4, 8, 0),
+    (2, 8, 0),   # 1024-residue tiles, the float32 maximum
     (1, 1, 3),   # persistent: 3 workgroups walk 45 tiles with look-back
     (2, 1, 7),
 ]
@@ -86,7 +86,7 @@ def test_ragged_batch_all_configs(cfg):
 
 def test_stride_not_multiple_of_anything():
     a, n, g = synthetic.custom([37, 1, 36, 13], 37, seed=5)
-    for cfg in [(1, 1, 0), (1, 2, 0), (4, 8, 0)]:
+    for cfg in [(1, 1, 0), (1, 2, 0), (2, 8, 0)]:
         x, ga = run(a, n, g, cfg)
         x64, g64 = oracle(a, n, g)
         assert np.abs(x - x64).max() <= coord_tol(x64)
@@ -251,7 +251,7 @@ def test_tuner_picks_a_candidate():
     A = torch.tensor(a, device=DEV)
     N = torch.tensor(n, device=DEV)
     cfg = _tuning.tune_now(A, N)
-    assert cfg in [tuple(c) for c in _tuning.candidates(500)]
+    assert cfg in [tuple(c) for c in _tuning.candidates(500, torch.float32)]
     t = _tuning.timings(A)
     assert t and all(v > 0 for v in t.values())
     print("fit500 B=128 tuner timings (ms):", t)

@@ -25,8 +25,10 @@ def test_bad_form():
 
 def test_candidates_cover_the_length():
     for L in (1, 30, 64, 200, 700, 1000, 16384):
-        c = _tuning.candidates(L)
-        assert c and all(w in (1, 2, 4) and r in (1, 2, 4, 8) for w, r, _ in c)
+        for dt, cap in ((torch.float32, 1024), (torch.float64, 512)):
+            c = _tuning.candidates(L, dt)
+            assert c and all(w in (1, 2, 4) and r in (1, 2, 4, 8) for w, r, _ in c)
+            assert all(64 * w * r <= cap for w, r, _ in c)
 
 
 def test_workspace_bytes_binding():

@@ -27,13 +27,17 @@ WAVES = (1, 2, 4)
 RES_PER_LANE = (1, 2, 4, 8)
 
 
-def candidates(max_length: int):
+MAX_TILE = {torch.float32: 1024, torch.float64: 512}  # LDS staging limit per dtype
+
+
+def candidates(max_length: int, dtype=torch.float32):
     """Tile shapes worth timing for chains padded to max_length: every
     (waves, res_per_lane) whose tile is at most the next power of two >= L
-    (larger tiles only add idle lanes), plus the smallest tile >= L."""
+    (larger tiles only add idle lanes) and within the dtype's tile limit."""
     cap = 64
     while cap < max_length:
         cap *= 2
+    cap = min(cap, MAX_TILE.get(dtype, 512))
     out = []
     for w in WAVES:
         for r in RES_PER_LANE:
@@ -72,7 +76,10 @@ def clear() -> None:
 
 def default_config(angles: torch.Tensor):
     B, _, L = angles.shape
-    return tuple(ext.default_config(int(B), int(L)))
+    w, r, g = ext.default_config(int(B), int(L))
+    while 64 * w * r > MAX_TILE.get(angles.dtype, 512):
+        r //= 2
+    return (w, max(r, 1), g)
 
 
 def config_for(angles: torch.Tensor, lengths: torch.Tensor):
@@ -102,7 +109,7 @@ def _tune(angles, lengths, reps: int = 5):
     B, _, L = angles.shape
     if B == 0 or L == 0:
         return default_config(angles)
-    cands = candidates(int(L))
+    cands = candidates(int(L), angles.dtype)
     st = sy = 0
     for c in cands:
         s, y = ext.workspace_bytes(int(B), int(L), list(c))

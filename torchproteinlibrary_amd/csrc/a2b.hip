@@ -10,12 +10,15 @@
 // N_{r+1}, so every lane reads its own angles and writes its own gradients.
 //
 // Scan kernels (a2b_fwd_scan / a2b_bwd_scan):
-//   * a tile is 64*W*R consecutive residues of one chain; lane l composes its R
-//     residues serially in FP64 (the structured 3x4 step below), the 64 lane
-//     aggregates are scanned with DPP (row_shr 1/2/4/8, row_bcast 15/31), the W
-//     wave aggregates through LDS, and tiles of one chain through a decoupled
-//     look-back on flags in global memory (tickets taken in chain order, so every
-//     tile waits only on tiles already being processed);
+//   * a tile is 64*W*R consecutive residues of one chain; its angles, coordinates
+//     and coordinate gradients pass through LDS so that every global access is
+//     a coalesced row segment;
+//   * lane l composes its R residues serially in FP64 (the structured 3x4 step
+//     below), the 64 lane aggregates are scanned with DPP (row_shr 1/2/4/8,
+//     row_bcast 15/31), the W wave aggregates through LDS, and tiles of one
+//     chain through a decoupled look-back on flags in global memory (tickets
+//     taken in chain order, so every tile waits only on tiles already being
+//     processed);
 //   * the backward recomputes frames from the forward's per-tile FP64 prefix
 //     checkpoints and runs the O(L) adjoint
 //       dL/dtheta_m = u_m . T_m + (x_m x u_m) . F_m,
@@ -53,9 +56,13 @@ __device__ __host__ constexpr Bond bond(int k) {
 // R e_y, f[6..8] = R e_z, f[9..11] = t.
 constexpr int kF = 12;
 
+__device__ __forceinline__ double ident(int e) {
+    return (e == 0 || e == 4 || e == 8) ? 1.0 : 0.0;
+}
+
 __device__ __forceinline__ void frame_identity(double f[kF]) {
 #pragma unroll
-    for (int e = 0; e < kF; ++e) f[e] = (e == 0 || e == 4 || e == 8) ? 1.0 : 0.0;
+    for (int e = 0; e < kF; ++e) f[e] = ident(e);
 }
 
 // f <- f * B(theta, beta_K, d_K) given (cos theta, sin theta).
@@ -77,16 +84,86 @@ __device__ __forceinline__ void frame_step(double f[kF], double c, double s) {
     }
 }
 
-// out = A * B (apply B first, then A).
-__device__ __forceinline__ void frame_compose(const double A[kF], const double B[kF],
-                                              double out[kF]) {
+// f <- M * f, in place one column at a time (M applied after f).  M may live
+// in registers, LDS or global memory.
+template <typename P>
+__device__ __forceinline__ void frame_lmul(const P& M, double f[kF]) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < 4; ++j) {
+        const double a = f[3 * j], b = f[3 * j + 1], c = f[3 * j + 2];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            double v = A[i] * B[3 * j] + A[3 + i] * B[3 * j + 1] + A[6 + i] * B[3 * j + 2];
-            out[3 * j + i] = (j == 3) ? v + A[9 + i] : v;
+            const double v = M[i] * a + M[3 + i] * b + M[6 + i] * c;
+            f[3 * j + i] = (j == 3) ? v + M[9 + i] : v;
         }
+    }
+}
+
+// out <- f * M, written column by column to memory (f in registers, M and
+// out in LDS or global memory); needs no full temporary frame.
+__device__ __forceinline__ void frame_rmul_to(const double f[kF], const double* M, double* out) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const double a = M[3 * j], b = M[3 * j + 1], c = M[3 * j + 2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double v = f[i] * a + f[3 + i] * b + f[6 + i] * c;
+            out[3 * j + i] = (j == 3) ? v + f[9 + i] : v;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ sincos
+// sin and cos of x in FP64 without library calls on the common path.
+// Cody-Waite reduction x = k*pi/2 + r with pi/2 split into three parts: two
+// parts of 33 significant bits, so k*P1 and k*P2 are exact for |k| < 2^20,
+// plus the rounded remainder.  Then Taylor series on |r| <= pi/4: sin to r^15
+// and cos to r^16.  The truncation error is below 5e-17, so the result is within
+// ~2 ulp of the true value for |x| < 1.6e6.  Larger |x| fall back to the
+// library's sincos.
+constexpr double kTwoOverPi = 0x1.45f306dc9c883p-1;
+constexpr double kPio2_1 = 0x1.921fb54400000p+0;
+constexpr double kPio2_2 = 0x1.0b4611a600000p-34;
+constexpr double kPio2_3 = 0x1.3198a2e037073p-69;
+// (-1)^i / (2i+1)!, i = 1..7 and (-1)^i / (2i)!, i = 1..8, rounded to double.
+constexpr double kSin[7] = {-0x1.5555555555555p-3, 0x1.1111111111111p-7, -0x1.a01a01a01a01ap-13,
+                            0x1.71de3a556c734p-19, -0x1.ae64567f544e4p-26, 0x1.6124613a86d09p-33,
+                            -0x1.ae7f3e733b81fp-41};
+constexpr double kCos[8] = {-0x1.0000000000000p-1, 0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10,
+                            0x1.a01a01a01a01ap-16, -0x1.27e4fb7789f5cp-22, 0x1.1eed8eff8d898p-29,
+                            -0x1.93974a8c07c9dp-37, 0x1.ae7f3e733b81fp-45};
+
+__device__ __forceinline__ void sincos_fp64(double x, double* s_out, double* c_out) {
+    const double k = rint(x * kTwoOverPi);
+    if (!(fabs(k) < 1048576.0)) {  // also catches NaN/inf
+        sincos(x, s_out, c_out);
+        return;
+    }
+    double r = fma(-k, kPio2_1, x);
+    r = fma(-k, kPio2_2, r);
+    r = fma(-k, kPio2_3, r);
+    const double r2 = r * r;
+    double ps = kSin[6];
+#pragma unroll
+    for (int i = 5; i >= 0; --i) ps = fma(ps, r2, kSin[i]);
+    double pc = kCos[7];
+#pragma unroll
+    for (int i = 6; i >= 0; --i) pc = fma(pc, r2, kCos[i]);
+    const double sr = fma(r * r2, ps, r);
+    const double cr = fma(r2, pc, 1.0);
+    const int q = static_cast<int>(k) & 3;
+    const double s = (q & 1) ? cr : sr;
+    const double c = (q & 1) ? sr : cr;
+    *s_out = (q & 2) ? -s : s;
+    *c_out = ((q + 1) & 2) ? -c : c;
+}
+
+// f <- f * B(angle) for row K of the tile's staged angles.
+template <int K, typename T>
+__device__ __forceinline__ void step_angle(double f[kF], T angle) {
+    double s, c;
+    sincos_fp64(static_cast<double>(angle), &s, &c);
+    frame_step<K>(f, c, s);
 }
 
 // ------------------------------------------------------------ cross-lane ops
@@ -114,15 +191,14 @@ __device__ __forceinline__ double read_lane(double v, int lane) {
 constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
 constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143, kWaveShr1 = 0x138;
 
+// One Hillis-Steele step: f <- (frame of the source lane) * f; lanes that
+// receive nothing compose with the identity.
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ void frame_scan_step(double f[kF]) {
-    double g[kF], o[kF];
+    double g[kF];
 #pragma unroll
-    for (int e = 0; e < kF; ++e)
-        g[e] = dpp_move<CTRL, ROW_MASK>(f[e], (e == 0 || e == 4 || e == 8) ? 1.0 : 0.0);
-    frame_compose(g, f, o);
-#pragma unroll
-    for (int e = 0; e < kF; ++e) f[e] = o[e];
+    for (int e = 0; e < kF; ++e) g[e] = dpp_move<CTRL, ROW_MASK>(f[e], ident(e));
+    frame_lmul(g, f);
 }
 
 // Inclusive scan of frames over the 64 lanes of a wave, in lane order
@@ -136,11 +212,11 @@ __device__ __forceinline__ void wave_scan_frames(double f[kF]) {
     frame_scan_step<kRowBcast31, 0xc>(f);
 }
 
-// Exclusive version from an inclusive scan: lane l gets lane l-1 (lane 0: I).
-__device__ __forceinline__ void wave_shift_frames(const double f[kF], double out[kF]) {
+// Exclusive version from an inclusive scan, in place: lane l gets lane l-1
+// (lane 0: I).
+__device__ __forceinline__ void wave_shift_frames(double f[kF]) {
 #pragma unroll
-    for (int e = 0; e < kF; ++e)
-        out[e] = dpp_move<kWaveShr1, 0xf>(f[e], (e == 0 || e == 4 || e == 8) ? 1.0 : 0.0);
+    for (int e = 0; e < kF; ++e) f[e] = dpp_move<kWaveShr1, 0xf>(f[e], ident(e));
 }
 
 // Inclusive prefix sum over the wave of N doubles.
@@ -160,15 +236,6 @@ __device__ __forceinline__ void wave_scan_sum(double v[N]) {
 }
 
 // --------------------------------------------------------------- I/O helpers
-template <typename T>
-__device__ __forceinline__ double ld(const T* p) {
-    return static_cast<double>(*p);
-}
-template <typename T>
-__device__ __forceinline__ void st(T* p, double v) {
-    *p = static_cast<T>(v);
-}
-
 __device__ __forceinline__ void set_status(uint32_t* status, uint32_t bits) {
     if (status) __hip_atomic_fetch_or(status, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -242,62 +309,69 @@ __device__ __forceinline__ int next_tile(const ScanArgs& a, int iter, int* s_til
     return tile < total ? tile : -1;
 }
 
-// Loads the R residues' angles of this lane, evaluates cos/sin in FP64 and
-// composes the lane aggregate f.  Residue r contributes phi_r, psi_r and (if
-// r < n-1) omega_r; residues >= n contribute nothing.
+// Dynamic LDS: the tile's angles [3][TR] and atom xyz [9*TR] (coords or
+// grad_coords) in the I/O type, then (backward) grad angles [3][TR].
+template <typename T>
+__host__ __device__ constexpr size_t lds_bytes(int tile_res, bool fwd) {
+    return sizeof(T) * (size_t)tile_res * (fwd ? 12 : 15);
+}
+
+// Coalesced copy of the tile's valid angles into LDS: row k, residues
+// [r0, r0+nvalid) -> s_ang[k*TR + j].
+template <typename T>
+__device__ __forceinline__ void stage_angles(const T* ang_b, int L, int r0, int nvalid, int TR,
+                                             T* s_ang) {
+    for (int k = 0; k < 3; ++k)
+        for (int j = threadIdx.x; j < nvalid; j += blockDim.x)
+            s_ang[k * TR + j] = ang_b[(size_t)k * L + r0 + j];
+}
+
+// f <- product of this lane's residue transforms (local residues q0..q0+R-1);
+// residue r contributes phi_r, psi_r and, if r < n-1, omega_r.
 template <typename T, int R>
-__device__ __forceinline__ void lane_aggregate(const T* ang_b, int L, int n, int rl,
-                                               double cs[R][3][2], double f[kF]) {
+__device__ __forceinline__ void lane_aggregate(const T* s_ang, int TR, int r0, int n, int q0,
+                                               double f[kF]) {
     frame_identity(f);
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-        const int r = rl + q;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            cs[q][k][0] = 1.0;
-            cs[q][k][1] = 0.0;
-        }
+        const int r = r0 + q0 + q;
         if (r < n) {
-            double s, c;
-            sincos(ld(ang_b + r), &s, &c);
-            cs[q][0][0] = c; cs[q][0][1] = s;
-            frame_step<0>(f, c, s);
-            sincos(ld(ang_b + L + r), &s, &c);
-            cs[q][1][0] = c; cs[q][1][1] = s;
-            frame_step<1>(f, c, s);
-            if (r < n - 1) {
-                sincos(ld(ang_b + 2 * L + r), &s, &c);
-                cs[q][2][0] = c; cs[q][2][1] = s;
-                frame_step<2>(f, c, s);
-            }
+            step_angle<0>(f, s_ang[q0 + q]);
+            step_angle<1>(f, s_ang[TR + q0 + q]);
+            if (r < n - 1) step_angle<2>(f, s_ang[2 * TR + q0 + q]);
         }
     }
 }
 
-// Frame prefix of this lane: tile_prefix * (waves before this one) * (lanes
-// before this one).  Uses s_tot (wave inclusive aggregates).
+// E <- tile_pfx * wave_aggr[0] * ... * wave_aggr[wave-1] * E, built right to
+// left with in-place left products (E enters as the lane's exclusive prefix
+// within its wave).
 __device__ __forceinline__ void lane_prefix(const double (*s_tot)[kF], int wave,
-                                            const double tile_pfx[kF], const double excl[kF],
-                                            double E[kF]) {
-    double P[kF], tmp[kF];
-#pragma unroll
-    for (int e = 0; e < kF; ++e) P[e] = tile_pfx[e];
-    for (int w = 0; w < wave; ++w) {
-        frame_compose(P, s_tot[w], tmp);
-#pragma unroll
-        for (int e = 0; e < kF; ++e) P[e] = tmp[e];
-    }
-    frame_compose(P, excl, E);
+                                            const double* tile_pfx, double E[kF]) {
+    for (int w = wave - 1; w >= 0; --w) frame_lmul(s_tot[w], E);
+    if (tile_pfx) frame_lmul(tile_pfx, E);
 }
 
 // ------------------------------------------------------------ forward scan
+// One tile = TR = 64*W*R residues of chain b.  Steps:
+//   1. stage the tile's angles in LDS (coalesced row loads);
+//   2. per lane: FP64 sincos and the serial product of its R residues; waves
+//      whose first residue is past the chain skip steps 2-4 (identity);
+//   3. DPP scan over the wave, LDS over the waves, look-back over tiles;
+//   4. per lane: walk its residues from the prefix (sincos recomputed),
+//      positions into LDS;
+//   5. coalesced store of the tile's coordinates, zeros past the chain.
 template <typename T, int R, int MAXT>
 __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
+    extern __shared__ __align__(16) unsigned char s_dyn[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nw = blockDim.x >> 6;
-    const int tile_res = blockDim.x * R;
+    const int TR = blockDim.x * R;
+    T* s_ang = reinterpret_cast<T*>(s_dyn);
+    T* s_xyz = s_ang + 3 * TR;
     __shared__ double s_tot[4][kF];
     __shared__ double s_pfx[kF];
+    __shared__ double s_agg[kF];
     __shared__ int s_tile;
     const T* angles = static_cast<const T*>(a.angles);
     T* coords = static_cast<T*>(a.out);
@@ -308,64 +382,58 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
         if (tile < 0) break;
         const int b = tile % a.B, t = tile / a.B;
         const int n = clamp_length(a, b, t == 0 && tid == 0);
-        const int r0 = t * tile_res;
-        const int rl = r0 + tid * R;
-        const T* ang_b = angles + (size_t)b * 3 * L;
+        const int r0 = t * TR;
+        const int nvalid = min(max(n - r0, 0), TR);
         T* x_b = coords + (size_t)b * 9 * L;
 
-        double cs[R][3][2];
-        double E[kF];
-        frame_identity(E);
-        if (r0 < n) {  // uniform: the tile holds residues of the chain
-            double f[kF], excl[kF];
-            lane_aggregate<T, R>(ang_b, L, n, rl, cs, f);
-            wave_scan_frames(f);
-            wave_shift_frames(f, excl);
-            if (lane == 63) {
+        if (nvalid > 0) {  // uniform: the tile holds residues of the chain
+            stage_angles(angles + (size_t)b * 3 * L, L, r0, nvalid, TR, s_ang);
+            __syncthreads();
+            const int q0 = tid * R;
+            const bool wave_active = wave * 64 * R < nvalid;  // uniform per wave
+            double E[kF];
+            if (wave_active) {
+                lane_aggregate<T, R>(s_ang, TR, r0, n, q0, E);
+                wave_scan_frames(E);
+                if (lane == 63) {
 #pragma unroll
-                for (int e = 0; e < kF; ++e) s_tot[wave][e] = f[e];
+                    for (int e = 0; e < kF; ++e) s_tot[wave][e] = E[e];
+                }
+                wave_shift_frames(E);
+            } else if (lane == 63) {
+#pragma unroll
+                for (int e = 0; e < kF; ++e) s_tot[wave][e] = ident(e);
             }
             __syncthreads();
-            double tile_pfx[kF];
-            frame_identity(tile_pfx);
             if (a.lookback) {
                 if (tid == 0) {
-                    double agg[kF], tmp[kF], acc[kF];
-                    frame_identity(agg);
-                    for (int w = 0; w < nw; ++w) {
-                        frame_compose(agg, s_tot[w], tmp);
-#pragma unroll
-                        for (int e = 0; e < kF; ++e) agg[e] = tmp[e];
-                    }
                     const size_t s0 = (size_t)b * a.ntiles;
                     uint32_t* fl = a.flags + s0;
                     double* pay = a.payload + s0 * kPayload;
-                    frame_identity(acc);
-                    if (t == 0) {
+                    double* my = pay + (size_t)t * kPayload;
+                    double acc[kF];
+                    {   // tile aggregate = product of the wave aggregates
 #pragma unroll
-                        for (int e = 0; e < kF; ++e) pay[kF + e] = agg[e];
-                        flag_store(fl, kFlagInclusive);
-                    } else {
-                        double* my = pay + (size_t)t * kPayload;
+                        for (int e = 0; e < kF; ++e) acc[e] = s_tot[nw - 1][e];
+                        for (int w = nw - 2; w >= 0; --w) frame_lmul(s_tot[w], acc);
 #pragma unroll
-                        for (int e = 0; e < kF; ++e) my[e] = agg[e];
-                        flag_store(fl + t, kFlagAggregate);
+                        for (int e = 0; e < kF; ++e) {
+                            s_agg[e] = acc[e];
+                            my[(t == 0 ? kF : 0) + e] = acc[e];
+                        }
+                    }
+                    flag_store(fl + t, t == 0 ? kFlagInclusive : kFlagAggregate);
+                    frame_identity(acc);  // becomes this tile's exclusive prefix
+                    if (t > 0) {
                         for (int j = t - 1; j >= 0; --j) {
                             const uint32_t fj = wait_flag(fl + j, a.status);
                             if (fj == 0) continue;
                             const double* pj = pay + (size_t)j * kPayload;
-                            double v[kF];
                             const bool incl = (fj == kFlagInclusive);
-#pragma unroll
-                            for (int e = 0; e < kF; ++e) v[e] = incl ? pj[kF + e] : pj[e];
-                            frame_compose(v, acc, tmp);
-#pragma unroll
-                            for (int e = 0; e < kF; ++e) acc[e] = tmp[e];
+                            frame_lmul(incl ? pj + kF : pj, acc);
                             if (incl) break;
                         }
-                        frame_compose(acc, agg, tmp);
-#pragma unroll
-                        for (int e = 0; e < kF; ++e) my[kF + e] = tmp[e];
+                        frame_rmul_to(acc, s_agg, my + kF);  // inclusive = prefix * aggregate
                         flag_store(fl + t, kFlagInclusive);
                     }
                     double* ck = a.state + ((size_t)b * a.ntiles + t) * kF;
@@ -376,55 +444,69 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
                     }
                 }
                 __syncthreads();
-#pragma unroll
-                for (int e = 0; e < kF; ++e) tile_pfx[e] = s_pfx[e];
             }
-            lane_prefix(s_tot, wave, tile_pfx, excl, E);
+            if (wave_active) {
+                // Residue r (local j) places CA_r, C_r and N_{r+1}: LDS floats
+                // 9j .. 9j+8 stand for global floats 9r+3 .. 9r+11.
+                lane_prefix(s_tot, wave, a.lookback ? s_pfx : nullptr, E);
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const int r = r0 + q0 + q;
+                    if (r < n) {
+                        T* o = s_xyz + 9 * (q0 + q);
+                        step_angle<0>(E, s_ang[q0 + q]);
+                        o[0] = (T)E[9]; o[1] = (T)E[10]; o[2] = (T)E[11];
+                        step_angle<1>(E, s_ang[TR + q0 + q]);
+                        o[3] = (T)E[9]; o[4] = (T)E[10]; o[5] = (T)E[11];
+                        if (r < n - 1) {
+                            step_angle<2>(E, s_ang[2 * TR + q0 + q]);
+                            o[6] = (T)E[9]; o[7] = (T)E[10]; o[8] = (T)E[11];
+                        }
+                    }
+                }
+            }
+            __syncthreads();
         }
 
-        // Coordinates: residue r writes CA_r, C_r into [9r+3, 9r+9) and N_{r+1}
-        // into [9r+9, 9r+12); the first lane of the chain writes N_0 = 0.
+        // Coalesced store of global floats [9*r0+3, 9*r0+3+9*TR) within [0, 9L):
+        // atoms < 3n from LDS, the rest 0.  Tile 0 also writes N_0 = 0.
         bool bad = false;
-        if (t == 0 && tid == 0) {
-            st(x_b + 0, 0.0); st(x_b + 1, 0.0); st(x_b + 2, 0.0);
-        }
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const int r = rl + q;
-            if (r >= L) break;
-            double o[9];
-#pragma unroll
-            for (int e = 0; e < 9; ++e) o[e] = 0.0;
-            if (r < n) {
-                frame_step<0>(E, cs[q][0][0], cs[q][0][1]);
-                o[0] = E[9]; o[1] = E[10]; o[2] = E[11];
-                frame_step<1>(E, cs[q][1][0], cs[q][1][1]);
-                o[3] = E[9]; o[4] = E[10]; o[5] = E[11];
-                if (r < n - 1) {
-                    frame_step<2>(E, cs[q][2][0], cs[q][2][1]);
-                    o[6] = E[9]; o[7] = E[10]; o[8] = E[11];
-                }
+        if (t == 0 && tid < 3) x_b[tid] = (T)0;
+        const int g0 = 9 * r0 + 3;
+        const int gend = min(g0 + 9 * TR, 9 * L);
+        const int gvalid = 9 * n;
+        for (int g = g0 + tid; g < gend; g += blockDim.x) {
+            T v = (T)0;
+            if (g < gvalid) {
+                v = s_xyz[g - g0];
+                bad |= (v != v);
             }
-            T* dst = x_b + 9 * (size_t)r + 3;
-            const int cnt = (r + 1 < L) ? 9 : 6;
-#pragma unroll
-            for (int e = 0; e < 9; ++e) {
-                if (e < cnt) {
-                    st(dst + e, o[e]);
-                    bad |= (o[e] != o[e]);
-                }
-            }
+            x_b[g] = v;
         }
         if (bad) set_status(a.status, TPL_A2B_STATUS_NAN);
     }
 }
 
 // ----------------------------------------------------------- backward scan
+// Same tiling, tickets from the last tile of each chain to the first.  Steps:
+//   1. stage angles and the tile's grad_coords in LDS (coalesced);
+//   2. lane/wave/tile prefix as in the forward, the tile prefix read from the
+//      forward's checkpoint;
+//   3. pass 2: positions, and this lane's wrench sum (sum x x g, sum g);
+//   4. suffix sums of wrenches over lanes (DPP), waves (LDS) and tiles
+//      (reverse look-back);
+//   5. pass 3: per atom m, S_m = sum over atoms after m; the gradient is
+//      u_m . S_T + (x_m x u_m) . S_F; gradients to LDS;
+//   6. coalesced store of the tile's gradients, zeros past the chain.
 template <typename T, int R, int MAXT>
 __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
+    extern __shared__ __align__(16) unsigned char s_dyn[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nw = blockDim.x >> 6;
-    const int tile_res = blockDim.x * R;
+    const int TR = blockDim.x * R;
+    T* s_ang = reinterpret_cast<T*>(s_dyn);
+    T* s_g = s_ang + 3 * TR;
+    T* s_out = s_g + 9 * TR;
     __shared__ double s_tot[4][kF];
     __shared__ double s_wsum[4][6];
     __shared__ double s_carry[6];
@@ -437,198 +519,179 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
     for (int iter = 0;; ++iter) {
         const int tile = next_tile(a, iter, &s_tile);
         if (tile < 0) break;
-        // Tickets run from the last tile of each chain to the first.
         const int b = tile % a.B, t = a.ntiles - 1 - tile / a.B;
         const int n = clamp_length(a, b, t == 0 && tid == 0);
-        const int r0 = t * tile_res;
-        const int rl = r0 + tid * R;
+        const int r0 = t * TR;
+        const int nvalid = min(max(n - r0, 0), TR);
         T* ga_b = gangles + (size_t)b * 3 * L;
 
-        if (r0 >= n) {  // uniform: padding only
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const int r = rl + q;
-                if (r < L) {
-                    st(ga_b + r, 0.0); st(ga_b + L + r, 0.0); st(ga_b + 2 * L + r, 0.0);
-                }
-            }
-            continue;
-        }
-
-        const T* ang_b = angles + (size_t)b * 3 * L;
-        const T* g_b = gcoords + (size_t)b * 9 * L;
-        double cs[R][3][2];
-        double E[kF];
-        {
-            double f[kF], excl[kF];
-            lane_aggregate<T, R>(ang_b, L, n, rl, cs, f);
-            wave_scan_frames(f);
-            wave_shift_frames(f, excl);
-            if (lane == 63) {
-#pragma unroll
-                for (int e = 0; e < kF; ++e) s_tot[wave][e] = f[e];
+        if (nvalid > 0) {  // uniform
+            stage_angles(angles + (size_t)b * 3 * L, L, r0, nvalid, TR, s_ang);
+            {
+                const T* g_b = gcoords + (size_t)b * 9 * L;
+                const int g0 = 9 * r0 + 3;
+                const int gend = min(g0 + 9 * TR, 9 * n);
+                for (int g = g0 + tid; g < gend; g += blockDim.x) s_g[g - g0] = g_b[g];
             }
             __syncthreads();
-            double tile_pfx[kF];
-            if (a.lookback) {
-                const double* ck = a.state + ((size_t)b * a.ntiles + t) * kF;
+            const int q0 = tid * R;
+            const bool wave_active = wave * 64 * R < nvalid;
+            double E[kF];
+            if (wave_active) {
+                lane_aggregate<T, R>(s_ang, TR, r0, n, q0, E);
+                wave_scan_frames(E);
+                if (lane == 63) {
 #pragma unroll
-                for (int e = 0; e < kF; ++e) tile_pfx[e] = ck[e];
-            } else {
-                frame_identity(tile_pfx);
-            }
-            lane_prefix(s_tot, wave, tile_pfx, excl, E);
-        }
-
-        // Pass 2: positions of this lane's atoms and its total wrench.
-        float gv[R][3][3];
-        double W[6] = {0, 0, 0, 0, 0, 0};  // (sum x x g, sum g)
-        {
-            double X[kF];
-#pragma unroll
-            for (int e = 0; e < kF; ++e) X[e] = E[e];
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const int r = rl + q;
-#pragma unroll
-                for (int k = 0; k < 3; ++k)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) gv[q][k][c] = 0.f;
-                if (r < n) {
-                    const int na = (r < n - 1) ? 3 : 2;
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        if (k < na) {
-                            if (k == 0) frame_step<0>(X, cs[q][0][0], cs[q][0][1]);
-                            if (k == 1) frame_step<1>(X, cs[q][1][0], cs[q][1][1]);
-                            if (k == 2) frame_step<2>(X, cs[q][2][0], cs[q][2][1]);
-                            const T* gp = g_b + 9 * (size_t)r + 3 + 3 * k;
-                            const double g0 = ld(gp), g1 = ld(gp + 1), g2 = ld(gp + 2);
-                            gv[q][k][0] = (float)g0; gv[q][k][1] = (float)g1; gv[q][k][2] = (float)g2;
-                            const double x0 = X[9], x1 = X[10], x2 = X[11];
-                            W[0] += x1 * g2 - x2 * g1;
-                            W[1] += x2 * g0 - x0 * g2;
-                            W[2] += x0 * g1 - x1 * g0;
-                            W[3] += g0; W[4] += g1; W[5] += g2;
-                        }
-                    }
+                    for (int e = 0; e < kF; ++e) s_tot[wave][e] = E[e];
                 }
-            }
-        }
-        // gv holds grad_coords rounded to float; for T = double keep full
-        // precision by re-reading below instead.
-
-        // Suffix sums of wrenches: lanes after this one, waves after this
-        // one, tiles after this one.
-        double Sincl[6];
-#pragma unroll
-        for (int e = 0; e < 6; ++e) Sincl[e] = W[e];
-        wave_scan_sum<6>(Sincl);
-        double wtot[6];
-#pragma unroll
-        for (int e = 0; e < 6; ++e) wtot[e] = read_lane(Sincl[e], 63);
-        if (lane == 0) {
-#pragma unroll
-            for (int e = 0; e < 6; ++e) s_wsum[wave][e] = wtot[e];
-        }
-        __syncthreads();
-        double C[6];
-#pragma unroll
-        for (int e = 0; e < 6; ++e) C[e] = wtot[e] - Sincl[e];
-        for (int w = wave + 1; w < nw; ++w) {
-#pragma unroll
-            for (int e = 0; e < 6; ++e) C[e] += s_wsum[w][e];
-        }
-        if (a.lookback) {
-            if (tid == 0) {
-                double tot[6] = {0, 0, 0, 0, 0, 0}, carry[6] = {0, 0, 0, 0, 0, 0};
-                for (int w = 0; w < nw; ++w) {
-#pragma unroll
-                    for (int e = 0; e < 6; ++e) tot[e] += s_wsum[w][e];
-                }
-                const int ntv = (n + tile_res - 1) / tile_res;  // tiles holding residues
-                const size_t s0 = (size_t)b * a.ntiles;
-                uint32_t* fl = a.flags + s0;
-                double* pay = a.payload + s0 * kPayload;
-                double* my = pay + (size_t)t * kPayload;
-                if (t == ntv - 1) {
-#pragma unroll
-                    for (int e = 0; e < 6; ++e) my[kF + e] = tot[e];
-                    flag_store(fl + t, kFlagInclusive);
-                } else {
-#pragma unroll
-                    for (int e = 0; e < 6; ++e) my[e] = tot[e];
-                    flag_store(fl + t, kFlagAggregate);
-                    for (int j = t + 1; j < ntv; ++j) {
-                        const uint32_t fj = wait_flag(fl + j, a.status);
-                        if (fj == 0) continue;
-                        const double* pj = pay + (size_t)j * kPayload;
-                        const bool incl = (fj == kFlagInclusive);
-#pragma unroll
-                        for (int e = 0; e < 6; ++e) carry[e] += incl ? pj[kF + e] : pj[e];
-                        if (incl) break;
-                    }
-#pragma unroll
-                    for (int e = 0; e < 6; ++e) my[kF + e] = carry[e] + tot[e];
-                    flag_store(fl + t, kFlagInclusive);
-                }
-#pragma unroll
-                for (int e = 0; e < 6; ++e) s_carry[e] = carry[e];
+                wave_shift_frames(E);
             }
             __syncthreads();
+            double W[6] = {0, 0, 0, 0, 0, 0};  // (sum x x g, sum g) of this lane
+            if (wave_active) {
+                lane_prefix(s_tot, wave,
+                            a.lookback ? a.state + ((size_t)b * a.ntiles + t) * kF : nullptr, E);
+                double X[kF];
 #pragma unroll
-            for (int e = 0; e < 6; ++e) C[e] += s_carry[e];
-        }
-
-        // Pass 3: S_m = sum_{i>m} wrench_i; grad_m = u.S_T + (x x u).S_F.
-        double S[6];
+                for (int e = 0; e < kF; ++e) X[e] = E[e];
 #pragma unroll
-        for (int e = 0; e < 6; ++e) S[e] = C[e] + W[e];
-        bool bad = false;
-        {
-            double X[kF];
+                for (int q = 0; q < R; ++q) {
+                    const int r = r0 + q0 + q;
+                    if (r < n) {
+                        const int na = (r < n - 1) ? 3 : 2;
 #pragma unroll
-            for (int e = 0; e < kF; ++e) X[e] = E[e];
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const int r = rl + q;
-                if (r >= L) break;
-                double gr[3] = {0.0, 0.0, 0.0};
-                if (r < n) {
-                    const int na = (r < n - 1) ? 3 : 2;
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        if (k < na) {
-                            if (k == 0) frame_step<0>(X, cs[q][0][0], cs[q][0][1]);
-                            if (k == 1) frame_step<1>(X, cs[q][1][0], cs[q][1][1]);
-                            if (k == 2) frame_step<2>(X, cs[q][2][0], cs[q][2][1]);
-                            double g0, g1, g2;
-                            if (sizeof(T) == sizeof(float)) {
-                                g0 = gv[q][k][0]; g1 = gv[q][k][1]; g2 = gv[q][k][2];
-                            } else {
-                                const T* gp = g_b + 9 * (size_t)r + 3 + 3 * k;
-                                g0 = ld(gp); g1 = ld(gp + 1); g2 = ld(gp + 2);
+                        for (int k = 0; k < 3; ++k) {
+                            if (k < na) {
+                                if (k == 0) step_angle<0>(X, s_ang[q0 + q]);
+                                if (k == 1) step_angle<1>(X, s_ang[TR + q0 + q]);
+                                if (k == 2) step_angle<2>(X, s_ang[2 * TR + q0 + q]);
+                                const T* gp = s_g + 9 * (q0 + q) + 3 * k;
+                                const double g0 = gp[0], g1 = gp[1], g2 = gp[2];
+                                const double x0 = X[9], x1 = X[10], x2 = X[11];
+                                W[0] += x1 * g2 - x2 * g1;
+                                W[1] += x2 * g0 - x0 * g2;
+                                W[2] += x0 * g1 - x1 * g0;
+                                W[3] += g0; W[4] += g1; W[5] += g2;
                             }
-                            const double x0 = X[9], x1 = X[10], x2 = X[11];
-                            const double u0 = X[0], u1 = X[1], u2 = X[2];
-                            S[0] -= x1 * g2 - x2 * g1;
-                            S[1] -= x2 * g0 - x0 * g2;
-                            S[2] -= x0 * g1 - x1 * g0;
-                            S[3] -= g0; S[4] -= g1; S[5] -= g2;
-                            const double w0 = x1 * u2 - x2 * u1;
-                            const double w1 = x2 * u0 - x0 * u2;
-                            const double w2 = x0 * u1 - x1 * u0;
-                            gr[k] = u0 * S[0] + u1 * S[1] + u2 * S[2] + w0 * S[3] + w1 * S[4] +
-                                    w2 * S[5];
                         }
                     }
-                    // psi of the last residue moves no atom: exactly 0.
-                    if (r == n - 1) gr[1] = 0.0;
                 }
-                st(ga_b + r, gr[0]);
-                st(ga_b + L + r, gr[1]);
-                st(ga_b + 2 * L + r, gr[2]);
-                bad |= (gr[0] != gr[0]) | (gr[1] != gr[1]) | (gr[2] != gr[2]);
+            }
+            // Suffix sums of wrenches: lanes, waves and tiles after this one.
+            double C[6];
+#pragma unroll
+            for (int e = 0; e < 6; ++e) C[e] = W[e];
+            if (wave_active) {
+                wave_scan_sum<6>(C);  // inclusive prefix over lanes
+#pragma unroll
+                for (int e = 0; e < 6; ++e) {
+                    const double tot = read_lane(C[e], 63);
+                    if (lane == 0) s_wsum[wave][e] = tot;
+                    C[e] = tot - C[e];  // sum over later lanes
+                }
+            } else if (lane == 0) {
+#pragma unroll
+                for (int e = 0; e < 6; ++e) s_wsum[wave][e] = 0.0;
+            }
+            __syncthreads();
+            for (int w = wave + 1; w < nw; ++w) {
+#pragma unroll
+                for (int e = 0; e < 6; ++e) C[e] += s_wsum[w][e];
+            }
+            if (a.lookback) {
+                if (tid == 0) {
+                    double tot[6] = {0, 0, 0, 0, 0, 0}, carry[6] = {0, 0, 0, 0, 0, 0};
+                    for (int w = 0; w < nw; ++w) {
+#pragma unroll
+                        for (int e = 0; e < 6; ++e) tot[e] += s_wsum[w][e];
+                    }
+                    const int ntv = (n + TR - 1) / TR;  // tiles holding residues
+                    const size_t s0 = (size_t)b * a.ntiles;
+                    uint32_t* fl = a.flags + s0;
+                    double* pay = a.payload + s0 * kPayload;
+                    double* my = pay + (size_t)t * kPayload;
+                    if (t == ntv - 1) {
+#pragma unroll
+                        for (int e = 0; e < 6; ++e) my[kF + e] = tot[e];
+                        flag_store(fl + t, kFlagInclusive);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 6; ++e) my[e] = tot[e];
+                        flag_store(fl + t, kFlagAggregate);
+                        for (int j = t + 1; j < ntv; ++j) {
+                            const uint32_t fj = wait_flag(fl + j, a.status);
+                            if (fj == 0) continue;
+                            const double* pj = pay + (size_t)j * kPayload;
+                            const bool incl = (fj == kFlagInclusive);
+#pragma unroll
+                            for (int e = 0; e < 6; ++e) carry[e] += incl ? pj[kF + e] : pj[e];
+                            if (incl) break;
+                        }
+#pragma unroll
+                        for (int e = 0; e < 6; ++e) my[kF + e] = carry[e] + tot[e];
+                        flag_store(fl + t, kFlagInclusive);
+                    }
+#pragma unroll
+                    for (int e = 0; e < 6; ++e) s_carry[e] = carry[e];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int e = 0; e < 6; ++e) C[e] += s_carry[e];
+            }
+            if (wave_active) {
+                // Pass 3: S_m = sum over atoms after m; gradients into LDS.
+                double S[6];
+#pragma unroll
+                for (int e = 0; e < 6; ++e) S[e] = C[e] + W[e];
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const int r = r0 + q0 + q;
+                    if (r < n) {
+                        double gr[3] = {0.0, 0.0, 0.0};
+                        const int na = (r < n - 1) ? 3 : 2;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            if (k < na) {
+                                if (k == 0) step_angle<0>(E, s_ang[q0 + q]);
+                                if (k == 1) step_angle<1>(E, s_ang[TR + q0 + q]);
+                                if (k == 2) step_angle<2>(E, s_ang[2 * TR + q0 + q]);
+                                const T* gp = s_g + 9 * (q0 + q) + 3 * k;
+                                const double g0 = gp[0], g1 = gp[1], g2 = gp[2];
+                                const double x0 = E[9], x1 = E[10], x2 = E[11];
+                                const double u0 = E[0], u1 = E[1], u2 = E[2];
+                                S[0] -= x1 * g2 - x2 * g1;
+                                S[1] -= x2 * g0 - x0 * g2;
+                                S[2] -= x0 * g1 - x1 * g0;
+                                S[3] -= g0; S[4] -= g1; S[5] -= g2;
+                                const double w0 = x1 * u2 - x2 * u1;
+                                const double w1 = x2 * u0 - x0 * u2;
+                                const double w2 = x0 * u1 - x1 * u0;
+                                gr[k] = u0 * S[0] + u1 * S[1] + u2 * S[2] + w0 * S[3] +
+                                        w1 * S[4] + w2 * S[5];
+                            }
+                        }
+                        if (r == n - 1) gr[1] = 0.0;  // psi of the last residue moves no atom
+                        s_out[q0 + q] = (T)gr[0];
+                        s_out[TR + q0 + q] = (T)gr[1];
+                        s_out[2 * TR + q0 + q] = (T)gr[2];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+
+        // Coalesced store of the tile's gradients, zeros past the chain.
+        bool bad = false;
+        const int jend = min(TR, L - r0);
+        for (int k = 0; k < 3; ++k) {
+            T* dst = ga_b + (size_t)k * L + r0;
+            for (int j = tid; j < jend; j += blockDim.x) {
+                T v = (T)0;
+                if (j < nvalid) {
+                    v = s_out[k * TR + j];
+                    bad |= (v != v);
+                }
+                dst[j] = v;
             }
         }
         if (bad) set_status(a.status, TPL_A2B_STATUS_NAN);
@@ -788,6 +851,14 @@ bool valid_config(const tpl_a2b_config* c) {
     return wok && rok && c->max_wgs >= 0 && c->flags == 0;
 }
 
+// Largest tile whose LDS staging (see lds_bytes) stays within 64 KiB.
+constexpr int kMaxTileF32 = 1024, kMaxTileF64 = 512;
+
+bool fits_dtype(int dtype, const tpl_a2b_config* c) {
+    const int tr = 64 * c->waves * c->res_per_lane;
+    return tr <= (dtype == TPL_A2B_F32 ? kMaxTileF32 : kMaxTileF64);
+}
+
 int tiles_for(int L, const tpl_a2b_config* c) {
     const int tr = 64 * c->waves * c->res_per_lane;
     return L <= 0 ? 0 : (L + tr - 1) / tr;
@@ -855,8 +926,11 @@ int launch_scan(bool fwd, int dtype, ScanArgs args, void* sync, const tpl_a2b_co
     int grid = total;
     if (cfg->max_wgs > 0) grid = std::min(grid, cfg->max_wgs);
     const int threads = 64 * cfg->waves;
-    KernelFn k = dtype == TPL_A2B_F32 ? pick_kernel<float>(fwd, cfg) : pick_kernel<double>(fwd, cfg);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(threads), 0, stream, args);
+    const int tr = threads * cfg->res_per_lane;
+    const bool f32 = dtype == TPL_A2B_F32;
+    KernelFn k = f32 ? pick_kernel<float>(fwd, cfg) : pick_kernel<double>(fwd, cfg);
+    const size_t lds = f32 ? lds_bytes<float>(tr, fwd) : lds_bytes<double>(tr, fwd);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(threads), lds, stream, args);
     return hipGetLastError() == hipSuccess ? TPL_A2B_OK : TPL_A2B_ERR_LAUNCH;
 }
 
@@ -910,7 +984,7 @@ int tpl_a2b_forward(int dtype, const void* angles, void* coords, const int32_t* 
                     const tpl_a2b_config* cfg, tpl_stream_t stream) {
     int rc = check_common(dtype, batch, angles_stride);
     if (rc) return rc;
-    if (!valid_config(cfg)) return TPL_A2B_ERR_CONFIG;
+    if (!valid_config(cfg) || !fits_dtype(dtype, cfg)) return TPL_A2B_ERR_CONFIG;
     if (batch == 0 || angles_stride == 0) return TPL_A2B_OK;
     if (!angles || !coords || !lengths) return TPL_A2B_ERR_ARG;
     const int nt = tiles_for(angles_stride, cfg);
@@ -933,7 +1007,7 @@ int tpl_a2b_backward(int dtype, void* grad_angles, const void* grad_coords, cons
                      tpl_stream_t stream) {
     int rc = check_common(dtype, batch, angles_stride);
     if (rc) return rc;
-    if (!valid_config(cfg)) return TPL_A2B_ERR_CONFIG;
+    if (!valid_config(cfg) || !fits_dtype(dtype, cfg)) return TPL_A2B_ERR_CONFIG;
     if (batch == 0 || angles_stride == 0) return TPL_A2B_OK;
     if (!grad_angles || !grad_coords || !angles || !lengths) return TPL_A2B_ERR_ARG;
     const int nt = tiles_for(angles_stride, cfg);
@@ -1024,7 +1098,7 @@ int tpl_a2b_tune(int dtype, const void* angles, const int32_t* lengths,
     for (int c = 0; c < n_candidates && rc == TPL_A2B_OK; ++c) {
         const tpl_a2b_config* cf = candidates + c;
         float ms = -1.f;
-        if (valid_config(cf)) {
+        if (valid_config(cf) && fits_dtype(dtype, cf)) {
             times.clear();
             for (int it = 0; it <= reps && rc == TPL_A2B_OK; ++it) {
                 if (hipEventRecord(e0, s) != hipSuccess) { rc = TPL_A2B_ERR_HIP; break; }

@@ -162,7 +162,7 @@ def main():
 
     # Buffer sets: autograd tensors (gsets) and raw launch buffers (ksets).
     gsets, ksets = [], []
-    st_b, sy_b = ext.workspace_bytes(B, L, list(cfg))
+    st_b, sy_b = ext.workspace_bytes(False, B, L, list(cfg))
     for _ in range(sets_n):
         A = A0.clone().requires_grad_(True)
         N = N0.clone()

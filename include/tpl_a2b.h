@@ -60,11 +60,14 @@ enum tpl_a2b_error {
 #define TPL_A2B_STATUS_LENGTH 2u /* some lengths[b] < 0 or > angles_stride (clamped) */
 #define TPL_A2B_STATUS_TIMEOUT 4u /* a look-back wait gave up (result invalid) */
 
-/* Launch shape of the scan kernels.  A tile is 64*waves*res_per_lane
- * consecutive residues of one chain.  One workgroup scans one tile.  Chains
+/* Launch shape of the scan kernels.  A forward tile is 64*waves*res_per_lane
+ * consecutive residues of one chain; one workgroup scans one tile, and chains
  * longer than a tile are chained across workgroups by decoupled look-back.
- * A tile holds at most 1024 residues for TPL_A2B_F32 and 512 for TPL_A2B_F64:
- * its I/O is staged in LDS.  Larger tiles return TPL_A2B_ERR_CONFIG. */
+ * Forward tiles hold at most 1024 residues for TPL_A2B_F32 and 512 for
+ * TPL_A2B_F64, since their I/O is staged in LDS; larger tiles return
+ * TPL_A2B_ERR_CONFIG.  The backward stages more per residue and uses tiles of
+ * at most 512 (F32) / 256 (F64) residues: the forward's tile, halved when
+ * needed. */
 typedef struct tpl_a2b_config {
     int32_t waves;        /* 64-lane waves per workgroup: 1, 2 or 4 */
     int32_t res_per_lane; /* residues composed serially per lane: 1, 2, 4 or 8 */
@@ -82,16 +85,19 @@ int tpl_a2b_default_config(int batch, int angles_stride, tpl_a2b_config *cfg);
 /* Residues per tile of cfg, or -1 if cfg is invalid. */
 int64_t tpl_a2b_tile_residues(const tpl_a2b_config *cfg);
 
-/* Bytes of the forward->backward state: one FP64 3x4 checkpoint (the scan
- * prefix) per tile start.  It replaces the reference's per-atom 4x4 frame
+/* Bytes of the forward->backward state for this dtype and config: one FP64
+ * 3x4 frame checkpoint (the scan prefix) per backward-tile start, plus the
+ * backward's look-back words.  The forward initialises all of it; the
+ * backward leaves its look-back words as it found them, so one state can serve
+ * repeated backward calls.  It replaces the reference's per-atom 4x4 frame
  * buffer A (Angles2Backbone.py:21) and is 0 when every chain fits in one
- * tile. */
-size_t tpl_a2b_state_bytes(int batch, int angles_stride, const tpl_a2b_config *cfg);
+ * backward tile. */
+size_t tpl_a2b_state_bytes(int dtype, int batch, int angles_stride, const tpl_a2b_config *cfg);
 
-/* Bytes of the look-back scratch used during one call (0 when every chain fits
- * in one tile).  The library clears what must start at zero itself, with a
- * kernel on `stream`; its contents need not survive between calls. */
-size_t tpl_a2b_sync_bytes(int batch, int angles_stride, const tpl_a2b_config *cfg);
+/* Bytes of the forward's look-back scratch for one call (0 when every chain
+ * fits in one forward tile).  The library clears what must start at zero
+ * itself, with a kernel on `stream`; contents need not survive the call. */
+size_t tpl_a2b_sync_bytes(int dtype, int batch, int angles_stride, const tpl_a2b_config *cfg);
 
 /* Forward: angles -> coords.  Replaces Angles2BackboneGPU_forward
  * (Layers/ReducedModel/Angles2Backbone/angles2backbone_interface.cpp:7-26) and
@@ -107,7 +113,8 @@ int tpl_a2b_forward(int dtype, const void *angles, void *coords, const int32_t *
  * (angles2backbone_interface.cpp:27-56) and its kernels
  * computeGradientsOptimizedBackbone + backwardFromCoordinatesBackbone
  * (cBackboneProteinCUDAKernels.cu:84-163,169-178).  `state` is what the forward
- * wrote for the same angles/lengths/cfg.  grad_angles is fully overwritten. */
+ * wrote for the same angles/lengths/cfg.  grad_angles is fully overwritten.
+ * `sync` is not used by the backward (may be NULL). */
 int tpl_a2b_backward(int dtype, void *grad_angles, const void *grad_coords, const void *angles,
                      const int32_t *lengths, const void *state, void *sync, uint32_t *status,
                      int batch, int angles_stride, const tpl_a2b_config *cfg,

@@ -29,7 +29,7 @@ def load():
     L.tpl_a2b_tile_residues.argtypes = [cfgp]
     for f in (L.tpl_a2b_state_bytes, L.tpl_a2b_sync_bytes):
         f.restype = ctypes.c_size_t
-        f.argtypes = [i32, i32, cfgp]
+        f.argtypes = [i32, i32, i32, cfgp]
     L.tpl_a2b_plain_frames_bytes.restype = ctypes.c_size_t
     L.tpl_a2b_plain_frames_bytes.argtypes = [i32, i32, i32]
     L.tpl_a2b_forward.argtypes = [i32, vp, vp, vp, vp, vp, vp, i32, i32, cfgp, vp]

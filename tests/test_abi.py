@@ -33,12 +33,25 @@ def test_config_validation_and_sizes(lib):
     for bad in [abi.Config(3, 1, 0, 0), abi.Config(8, 1, 0, 0), abi.Config(1, 3, 0, 0),
                 abi.Config(1, 1, -1, 0), abi.Config(1, 1, 0, 1)]:
         assert lib.tpl_a2b_tile_residues(ctypes.byref(bad)) == -1
-    # one tile per chain -> no state, no scratch
-    assert lib.tpl_a2b_state_bytes(1024, 1000, ctypes.byref(c)) == 0
-    assert lib.tpl_a2b_sync_bytes(1024, 1000, ctypes.byref(c)) == 0
-    # 64 chains of up to 16384 residues in 1024-residue tiles: 16 tiles each
-    assert lib.tpl_a2b_state_bytes(64, 16384, ctypes.byref(c)) == 64 * 16 * 12 * 8
-    assert lib.tpl_a2b_sync_bytes(64, 16384, ctypes.byref(c)) >= 64 * 16 * (4 + 24 * 8)
+    def align(x):
+        return (x + 255) // 256 * 256
+
+    def state(B, ntb):  # checkpoints + payload (12 doubles each) + flags + tickets + done
+        return 0 if ntb <= 1 else align(B * ntb * (96 + 96 + 4) + 8 * B)
+
+    def sync(B, ntf):  # flags + tickets (zeroed), then 24-double payload per tile
+        return 0 if ntf <= 1 else align(4 * (B * ntf + B)) + B * ntf * 24 * 8
+
+    # (4,4): forward tiles of 1024, backward tiles of 512 (f32) / forward 512-cap (f64)
+    assert lib.tpl_a2b_state_bytes(0, 1024, 1000, ctypes.byref(c)) == state(1024, 2)
+    assert lib.tpl_a2b_sync_bytes(0, 1024, 1000, ctypes.byref(c)) == 0
+    assert lib.tpl_a2b_state_bytes(0, 64, 16384, ctypes.byref(c)) == state(64, 32)
+    assert lib.tpl_a2b_sync_bytes(0, 64, 16384, ctypes.byref(c)) == sync(64, 16)
+    c2 = abi.Config(2, 4, 0, 0)  # 512-residue tiles both ways
+    assert lib.tpl_a2b_state_bytes(0, 512, 500, ctypes.byref(c2)) == 0
+    assert lib.tpl_a2b_sync_bytes(0, 512, 500, ctypes.byref(c2)) == 0
+    assert lib.tpl_a2b_state_bytes(1, 512, 500, ctypes.byref(c2)) == state(512, 2)  # f64: 256
+    assert lib.tpl_a2b_state_bytes(7, 512, 500, ctypes.byref(c2)) == 0  # unknown dtype
     assert lib.tpl_a2b_plain_frames_bytes(0, 2, 10) == 2 * 30 * 12 * 4
     assert lib.tpl_a2b_plain_frames_bytes(1, 2, 10) == 2 * 30 * 12 * 8
 

@@ -27,8 +27,8 @@ def test_forward_backward_via_c_abi(cfg):
     G = torch.tensor(g, device=dev)
     B, _, L = a.shape
     c = abi.Config(*cfg, 0)
-    st = torch.empty(max(1, lib.tpl_a2b_state_bytes(B, L, ctypes.byref(c))), dtype=torch.uint8, device=dev)
-    sy = torch.empty(max(1, lib.tpl_a2b_sync_bytes(B, L, ctypes.byref(c))), dtype=torch.uint8, device=dev)
+    st = torch.empty(max(1, lib.tpl_a2b_state_bytes(0, B, L, ctypes.byref(c))), dtype=torch.uint8, device=dev)
+    sy = torch.empty(max(1, lib.tpl_a2b_sync_bytes(0, B, L, ctypes.byref(c))), dtype=torch.uint8, device=dev)
     X = torch.full((B, 9 * L), float("nan"), device=dev)
     GA = torch.full((B, 3, L), float("nan"), device=dev)
     status = torch.zeros(1, dtype=torch.int32, device=dev)

@@ -32,11 +32,11 @@ def test_candidates_cover_the_length():
 
 
 def test_workspace_bytes_binding():
-    assert _native.ext.workspace_bytes(1024, 1000, [4, 4, 0]) == (0, 0)
-    st, sy = _native.ext.workspace_bytes(64, 16384, [4, 4, 0])
-    assert st == 64 * 16 * 96 and sy > 0
+    assert _native.ext.workspace_bytes(False, 512, 500, [2, 4, 0]) == (0, 0)
+    st, sy = _native.ext.workspace_bytes(False, 64, 16384, [4, 4, 0])
+    assert st > 64 * 32 * 96 and sy > 64 * 16 * 24 * 8
     with pytest.raises(RuntimeError):
-        _native.ext.workspace_bytes(4, 4, [3, 1, 0])
+        _native.ext.workspace_bytes(False, 4, 4, [3, 1, 0])
 
 
 def test_synthetic_workloads_are_deterministic():

@@ -91,7 +91,8 @@ class Angles2BackboneGPUFunction(Function):
         cfg = _tuning.config_for(input_angles, angles_length)
         dev = input_angles.device
         coords = torch.empty(B, 9 * L, dtype=input_angles.dtype, device=dev)
-        st_bytes, sy_bytes = ext.workspace_bytes(B, L, list(cfg))
+        st_bytes, sy_bytes = ext.workspace_bytes(input_angles.dtype == torch.float64, B, L,
+                                                 list(cfg))
         state = _empty_bytes(st_bytes, dev)
         sync = _empty_bytes(sy_bytes, dev)
         status = _status_word(dev)
@@ -100,7 +101,6 @@ class Angles2BackboneGPUFunction(Function):
         _raise_on_status(status, "forward")
         ctx.save_for_backward(input_angles, angles_length, state)
         ctx.cfg = cfg
-        ctx.sync_bytes = sy_bytes
         return coords
 
     @staticmethod
@@ -113,7 +113,7 @@ class Angles2BackboneGPUFunction(Function):
         B, _, L = input_angles.shape
         dev = input_angles.device
         gradInput = torch.empty(B, 3, L, dtype=input_angles.dtype, device=dev)
-        sync = _empty_bytes(ctx.sync_bytes, dev)
+        sync = _empty_bytes(0, dev)  # the backward keeps its look-back words in `state`
         status = _status_word(dev)
         ext.Angles2BackboneGPU_backward(gradInput, gradOutput, input_angles, angles_length, state,
                                         sync, status, list(ctx.cfg))

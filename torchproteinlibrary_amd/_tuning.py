@@ -112,7 +112,7 @@ def _tune(angles, lengths, reps: int = 5):
     cands = candidates(int(L), angles.dtype)
     st = sy = 0
     for c in cands:
-        s, y = ext.workspace_bytes(int(B), int(L), list(c))
+        s, y = ext.workspace_bytes(angles.dtype == torch.float64, int(B), int(L), list(c))
         st, sy = max(st, s), max(sy, y)
     dev, dt = angles.device, angles.dtype
     coords = torch.empty(B, 9 * L, dtype=dt, device=dev)

@@ -114,56 +114,109 @@ __device__ __forceinline__ void frame_rmul_to(const double f[kF], const double* 
 }
 
 // ------------------------------------------------------------------ sincos
-// sin and cos of x in FP64 without library calls on the common path.
-// Cody-Waite reduction x = k*pi/2 + r with pi/2 split into three parts: two
-// parts of 33 significant bits, so k*P1 and k*P2 are exact for |k| < 2^20,
-// plus the rounded remainder.  Then Taylor series on |r| <= pi/4: sin to r^15
-// and cos to r^16.  The truncation error is below 5e-17, so the result is within
-// ~2 ulp of the true value for |x| < 1.6e6.  Larger |x| fall back to the
+// sin and cos of x in FP64, table-based.  x = k*(pi/32) + d with pi/32 split
+// into three parts (two of 33 significant bits, so k*P1 and k*P2 are exact for
+// |k| < 2^20, plus the rounded remainder), |d| <= pi/64.  Then
+//   sin x = s_j cos d + c_j sin d,   cos x = c_j cos d - s_j sin d,
+// with (c_j, s_j) = (cos, sin)(2 pi j / 64), j = k mod 64, from a table, and
+// Taylor series for sin d (to d^7) and cos d (to d^8), truncation < 5e-18.
+// Table entries are the exact values rounded to double.  Agreement with libm
+// is within 2.2e-16 for |x| < 1e4.  |k| >= 2^20 (|x| > 1e5) falls back to the
 // library's sincos.
-constexpr double kTwoOverPi = 0x1.45f306dc9c883p-1;
-constexpr double kPio2_1 = 0x1.921fb54400000p+0;
-constexpr double kPio2_2 = 0x1.0b4611a600000p-34;
-constexpr double kPio2_3 = 0x1.3198a2e037073p-69;
-// (-1)^i / (2i+1)!, i = 1..7 and (-1)^i / (2i)!, i = 1..8, rounded to double.
-constexpr double kSin[7] = {-0x1.5555555555555p-3, 0x1.1111111111111p-7, -0x1.a01a01a01a01ap-13,
-                            0x1.71de3a556c734p-19, -0x1.ae64567f544e4p-26, 0x1.6124613a86d09p-33,
-                            -0x1.ae7f3e733b81fp-41};
-constexpr double kCos[8] = {-0x1.0000000000000p-1, 0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10,
-                            0x1.a01a01a01a01ap-16, -0x1.27e4fb7789f5cp-22, 0x1.1eed8eff8d898p-29,
-                            -0x1.93974a8c07c9dp-37, 0x1.ae7f3e733b81fp-45};
+constexpr double kTabInvStep = 0x1.45f306dc9c883p+3;  // 32/pi
+constexpr double kTabStep1 = 0x1.921fb54400000p-4;
+constexpr double kTabStep2 = 0x1.0b4611a600000p-38;
+constexpr double kTabStep3 = 0x1.3198a2e037073p-73;
+// cos, sin of 2*pi*j/64 for j = 0..63, interleaved.
+__constant__ double kTabCS[128] = {
+    0x1.0000000000000p+0, 0x0.0p+0,
+    0x1.fd88da3d12526p-1, 0x1.917a6bc29b42cp-4,
+    0x1.f6297cff75cb0p-1, 0x1.8f8b83c69a60bp-3,
+    0x1.e9f4156c62ddap-1, 0x1.294062ed59f06p-2,
+    0x1.d906bcf328d46p-1, 0x1.87de2a6aea963p-2,
+    0x1.c38b2f180bdb1p-1, 0x1.e2b5d3806f63bp-2,
+    0x1.a9b66290ea1a3p-1, 0x1.1c73b39ae68c8p-1,
+    0x1.8bc806b151741p-1, 0x1.44cf325091dd6p-1,
+    0x1.6a09e667f3bcdp-1, 0x1.6a09e667f3bcdp-1,
+    0x1.44cf325091dd6p-1, 0x1.8bc806b151741p-1,
+    0x1.1c73b39ae68c8p-1, 0x1.a9b66290ea1a3p-1,
+    0x1.e2b5d3806f63bp-2, 0x1.c38b2f180bdb1p-1,
+    0x1.87de2a6aea963p-2, 0x1.d906bcf328d46p-1,
+    0x1.294062ed59f06p-2, 0x1.e9f4156c62ddap-1,
+    0x1.8f8b83c69a60bp-3, 0x1.f6297cff75cb0p-1,
+    0x1.917a6bc29b42cp-4, 0x1.fd88da3d12526p-1,
+    -0x1.2ff3e0ec6474dp-192, 0x1.0000000000000p+0,
+    -0x1.917a6bc29b42cp-4, 0x1.fd88da3d12526p-1,
+    -0x1.8f8b83c69a60bp-3, 0x1.f6297cff75cb0p-1,
+    -0x1.294062ed59f06p-2, 0x1.e9f4156c62ddap-1,
+    -0x1.87de2a6aea963p-2, 0x1.d906bcf328d46p-1,
+    -0x1.e2b5d3806f63bp-2, 0x1.c38b2f180bdb1p-1,
+    -0x1.1c73b39ae68c8p-1, 0x1.a9b66290ea1a3p-1,
+    -0x1.44cf325091dd6p-1, 0x1.8bc806b151741p-1,
+    -0x1.6a09e667f3bcdp-1, 0x1.6a09e667f3bcdp-1,
+    -0x1.8bc806b151741p-1, 0x1.44cf325091dd6p-1,
+    -0x1.a9b66290ea1a3p-1, 0x1.1c73b39ae68c8p-1,
+    -0x1.c38b2f180bdb1p-1, 0x1.e2b5d3806f63bp-2,
+    -0x1.d906bcf328d46p-1, 0x1.87de2a6aea963p-2,
+    -0x1.e9f4156c62ddap-1, 0x1.294062ed59f06p-2,
+    -0x1.f6297cff75cb0p-1, 0x1.8f8b83c69a60bp-3,
+    -0x1.fd88da3d12526p-1, 0x1.917a6bc29b42cp-4,
+    -0x1.0000000000000p+0, 0x1.08469d45fa26fp-186,
+    -0x1.fd88da3d12526p-1, -0x1.917a6bc29b42cp-4,
+    -0x1.f6297cff75cb0p-1, -0x1.8f8b83c69a60bp-3,
+    -0x1.e9f4156c62ddap-1, -0x1.294062ed59f06p-2,
+    -0x1.d906bcf328d46p-1, -0x1.87de2a6aea963p-2,
+    -0x1.c38b2f180bdb1p-1, -0x1.e2b5d3806f63bp-2,
+    -0x1.a9b66290ea1a3p-1, -0x1.1c73b39ae68c8p-1,
+    -0x1.8bc806b151741p-1, -0x1.44cf325091dd6p-1,
+    -0x1.6a09e667f3bcdp-1, -0x1.6a09e667f3bcdp-1,
+    -0x1.44cf325091dd6p-1, -0x1.8bc806b151741p-1,
+    -0x1.1c73b39ae68c8p-1, -0x1.a9b66290ea1a3p-1,
+    -0x1.e2b5d3806f63bp-2, -0x1.c38b2f180bdb1p-1,
+    -0x1.87de2a6aea963p-2, -0x1.d906bcf328d46p-1,
+    -0x1.294062ed59f06p-2, -0x1.e9f4156c62ddap-1,
+    -0x1.8f8b83c69a60bp-3, -0x1.f6297cff75cb0p-1,
+    -0x1.917a6bc29b42cp-4, -0x1.fd88da3d12526p-1,
+    -0x1.2f78b1bbc4ef2p-184, -0x1.0000000000000p+0,
+    0x1.917a6bc29b42cp-4, -0x1.fd88da3d12526p-1,
+    0x1.8f8b83c69a60bp-3, -0x1.f6297cff75cb0p-1,
+    0x1.294062ed59f06p-2, -0x1.e9f4156c62ddap-1,
+    0x1.87de2a6aea963p-2, -0x1.d906bcf328d46p-1,
+    0x1.e2b5d3806f63bp-2, -0x1.c38b2f180bdb1p-1,
+    0x1.1c73b39ae68c8p-1, -0x1.a9b66290ea1a3p-1,
+    0x1.44cf325091dd6p-1, -0x1.8bc806b151741p-1,
+    0x1.6a09e667f3bcdp-1, -0x1.6a09e667f3bcdp-1,
+    0x1.8bc806b151741p-1, -0x1.44cf325091dd6p-1,
+    0x1.a9b66290ea1a3p-1, -0x1.1c73b39ae68c8p-1,
+    0x1.c38b2f180bdb1p-1, -0x1.e2b5d3806f63bp-2,
+    0x1.d906bcf328d46p-1, -0x1.87de2a6aea963p-2,
+    0x1.e9f4156c62ddap-1, -0x1.294062ed59f06p-2,
+    0x1.f6297cff75cb0p-1, -0x1.8f8b83c69a60bp-3,
+    0x1.fd88da3d12526p-1, -0x1.917a6bc29b42cp-4,
+};
+constexpr double kS3 = -0x1.5555555555555p-3, kS5 = 0x1.1111111111111p-7,
+                 kS7 = -0x1.a01a01a01a01ap-13;
+constexpr double kC2 = -0x1.0000000000000p-1, kC4 = 0x1.5555555555555p-5,
+                 kC6 = -0x1.6c16c16c16c17p-10, kC8 = 0x1.a01a01a01a01ap-16;
 
-__device__ __forceinline__ void sincos_fp64(double x, double* s_out, double* c_out) {
-    const double k = rint(x * kTwoOverPi);
+// tab: the 128 doubles of kTabCS (in LDS).
+__device__ __forceinline__ void sincos_tab(double x, const double* tab, double* s_out,
+                                           double* c_out) {
+    const double k = rint(x * kTabInvStep);
     if (!(fabs(k) < 1048576.0)) {  // also catches NaN/inf
         sincos(x, s_out, c_out);
         return;
     }
-    double r = fma(-k, kPio2_1, x);
-    r = fma(-k, kPio2_2, r);
-    r = fma(-k, kPio2_3, r);
-    const double r2 = r * r;
-    double ps = kSin[6];
-#pragma unroll
-    for (int i = 5; i >= 0; --i) ps = fma(ps, r2, kSin[i]);
-    double pc = kCos[7];
-#pragma unroll
-    for (int i = 6; i >= 0; --i) pc = fma(pc, r2, kCos[i]);
-    const double sr = fma(r * r2, ps, r);
-    const double cr = fma(r2, pc, 1.0);
-    const int q = static_cast<int>(k) & 3;
-    const double s = (q & 1) ? cr : sr;
-    const double c = (q & 1) ? sr : cr;
-    *s_out = (q & 2) ? -s : s;
-    *c_out = ((q + 1) & 2) ? -c : c;
-}
-
-// f <- f * B(angle) for row K of the tile's staged angles.
-template <int K, typename T>
-__device__ __forceinline__ void step_angle(double f[kF], T angle) {
-    double s, c;
-    sincos_fp64(static_cast<double>(angle), &s, &c);
-    frame_step<K>(f, c, s);
+    double d = fma(-k, kTabStep1, x);
+    d = fma(-k, kTabStep2, d);
+    d = fma(-k, kTabStep3, d);
+    const int j = static_cast<int>(k) & 63;
+    const double cj = tab[2 * j], sj = tab[2 * j + 1];
+    const double d2 = d * d;
+    const double sd = fma(d * d2, fma(d2, fma(d2, kS7, kS5), kS3), d);
+    const double cd = fma(d2, fma(d2, fma(d2, fma(d2, kC8, kC6), kC4), kC2), 1.0);
+    *s_out = fma(sj, cd, cj * sd);
+    *c_out = fma(cj, cd, -(sj * sd));
 }
 
 // ------------------------------------------------------------ cross-lane ops
@@ -246,6 +299,9 @@ __device__ __forceinline__ uint32_t flag_load(const uint32_t* p) {
 __device__ __forceinline__ void flag_store(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ uint32_t atomic_inc(uint32_t* p) {
+    return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 constexpr uint32_t kFlagAggregate = 1, kFlagInclusive = 2;
 
@@ -264,22 +320,97 @@ __device__ __forceinline__ uint32_t wait_flag(const uint32_t* p, uint32_t* statu
     return 0;
 }
 
-// Look-back payload per (chain, tile): aggregate and inclusive value, 12
-// doubles each (frames in the forward, wrenches use the first 6).
-constexpr int kPayload = 2 * kF;
+// 16-byte vector of T.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    using type = float4;
+    static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+    using type = double2;
+    static constexpr int n = 2;
+};
+
+// Workgroup-wide store of dst[0, count): element i is src[i] (LDS) for
+// i < nsrc and 0 otherwise.  Aligned body in 16-byte stores.
+template <typename T>
+__device__ __forceinline__ bool store_span(T* dst, int count, const T* src, int nsrc) {
+    using V = typename Vec16<T>::type;
+    constexpr int E = Vec16<T>::n;
+    const int mis = (int)((reinterpret_cast<uintptr_t>(dst) / sizeof(T)) % E);
+    const int head = min((E - mis) % E, count);
+    const int nvec = (count - head) / E;
+    const int tail0 = head + nvec * E;
+    bool bad = false;
+    auto elem = [&](int i) -> T {
+        if (i < nsrc) {
+            const T v = src[i];
+            bad |= (v != v);
+            return v;
+        }
+        return (T)0;
+    };
+    for (int i = threadIdx.x; i < head; i += blockDim.x) dst[i] = elem(i);
+    for (int v = threadIdx.x; v < nvec; v += blockDim.x) {
+        const int i0 = head + v * E;
+        V w;
+        T* wp = reinterpret_cast<T*>(&w);
+#pragma unroll
+        for (int e = 0; e < E; ++e) wp[e] = elem(i0 + e);
+        *reinterpret_cast<V*>(dst + i0) = w;
+    }
+    for (int i = tail0 + threadIdx.x; i < count; i += blockDim.x) dst[i] = elem(i);
+    return bad;
+}
+
+// Workgroup-wide copy of src[0, count) (global) into LDS dst; aligned body in
+// 16-byte loads.
+template <typename T>
+__device__ __forceinline__ void load_span(const T* src, int count, T* dst) {
+    using V = typename Vec16<T>::type;
+    constexpr int E = Vec16<T>::n;
+    const int mis = (int)((reinterpret_cast<uintptr_t>(src) / sizeof(T)) % E);
+    const int head = min((E - mis) % E, count);
+    const int nvec = (count - head) / E;
+    const int tail0 = head + nvec * E;
+    for (int i = threadIdx.x; i < head; i += blockDim.x) dst[i] = src[i];
+    for (int v = threadIdx.x; v < nvec; v += blockDim.x) {
+        const int i0 = head + v * E;
+        const V w = *reinterpret_cast<const V*>(src + i0);
+        const T* wp = reinterpret_cast<const T*>(&w);
+#pragma unroll
+        for (int e = 0; e < E; ++e) dst[i0 + e] = wp[e];
+    }
+    for (int i = tail0 + threadIdx.x; i < count; i += blockDim.x) dst[i] = src[i];
+}
+
+// Look-back payload per (chain, tile): aggregate and inclusive value.  The
+// forward uses 12 doubles each (frames), the backward 6 (wrenches).
+constexpr int kFwdPayload = 2 * kF;
+constexpr int kBwdPayload = 12;
 
 struct ScanArgs {
     const void* angles;
     void* out;             // coords (forward) / grad_angles (backward)
     const void* grad;      // grad_coords (backward)
     const int32_t* lengths;
-    double* state;         // [B][ntiles][12] prefix checkpoints (ntiles > 1)
-    uint32_t* flags;       // [B][ntiles] look-back flags, zero at launch
-    uint32_t* ticket;      // tile ticket counter, zero at launch
-    double* payload;       // [B][ntiles][kPayload]
     uint32_t* status;
-    int B, L, ntiles;
-    int lookback;          // chains may span several tiles
+    int B, L;
+    int ntf, trf;          // forward tiles per chain, residues per forward tile
+    int ntb, trb;          // backward tiles per chain, residues per backward tile
+    // forward look-back (ntf > 1), in the per-call scratch, zeroed before launch
+    uint32_t* fflags;      // [B][ntf]
+    uint32_t* ftickets;    // [B]
+    double* fpay;          // [B][ntf][kFwdPayload]
+    // forward -> backward state (ntb > 1)
+    double* ckpt;          // [B][ntb][12]: frame prefix at each backward tile start
+    uint32_t* bflags;      // [B][ntb]   zeroed by the forward, cleaned by the backward
+    uint32_t* btickets;    // [B]
+    uint32_t* bdone;       // [B]
+    double* bpay;          // [B][ntb][kBwdPayload]
 };
 
 __device__ __forceinline__ int clamp_length(const ScanArgs& a, int b, bool report) {
@@ -291,56 +422,23 @@ __device__ __forceinline__ int clamp_length(const ScanArgs& a, int b, bool repor
     return n;
 }
 
-// Next tile of this workgroup: a ticket (look-back mode, chain order) or a
-// grid-stride index.  Returns -1 when no work is left.
-__device__ __forceinline__ int next_tile(const ScanArgs& a, int iter, int* s_tile) {
-    const int total = a.B * a.ntiles;
-    __syncthreads();  // LDS of the previous tile is no longer read
-    int tile;
-    if (a.lookback) {
-        if (threadIdx.x == 0)
-            *s_tile = (int)__hip_atomic_fetch_add(a.ticket, 1u, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
+// Work item -> (chain, tile).  Item i belongs to chain i % B.  With several
+// tiles per chain, the tile comes from the chain's ticket counter, so a tile
+// is taken only after every earlier ticket of its chain was taken by a
+// running workgroup (forward: ascending tiles; backward: descending).
+__device__ __forceinline__ void item_tile(int item, int B, int nt, uint32_t* tickets,
+                                          bool descending, int* s_tile, int* b_out,
+                                          int* t_out) {
+    const int b = item % B;
+    int t = 0;
+    if (nt > 1) {
+        if (threadIdx.x == 0) *s_tile = (int)atomic_inc(tickets + b);
         __syncthreads();
-        tile = __builtin_amdgcn_readfirstlane(*s_tile);
-    } else {
-        tile = blockIdx.x + iter * gridDim.x;
+        t = __builtin_amdgcn_readfirstlane(*s_tile);
+        if (descending) t = nt - 1 - t;
     }
-    return tile < total ? tile : -1;
-}
-
-// Dynamic LDS: the tile's angles [3][TR] and atom xyz [9*TR] (coords or
-// grad_coords) in the I/O type, then (backward) grad angles [3][TR].
-template <typename T>
-__host__ __device__ constexpr size_t lds_bytes(int tile_res, bool fwd) {
-    return sizeof(T) * (size_t)tile_res * (fwd ? 12 : 15);
-}
-
-// Coalesced copy of the tile's valid angles into LDS: row k, residues
-// [r0, r0+nvalid) -> s_ang[k*TR + j].
-template <typename T>
-__device__ __forceinline__ void stage_angles(const T* ang_b, int L, int r0, int nvalid, int TR,
-                                             T* s_ang) {
-    for (int k = 0; k < 3; ++k)
-        for (int j = threadIdx.x; j < nvalid; j += blockDim.x)
-            s_ang[k * TR + j] = ang_b[(size_t)k * L + r0 + j];
-}
-
-// f <- product of this lane's residue transforms (local residues q0..q0+R-1);
-// residue r contributes phi_r, psi_r and, if r < n-1, omega_r.
-template <typename T, int R>
-__device__ __forceinline__ void lane_aggregate(const T* s_ang, int TR, int r0, int n, int q0,
-                                               double f[kF]) {
-    frame_identity(f);
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-        const int r = r0 + q0 + q;
-        if (r < n) {
-            step_angle<0>(f, s_ang[q0 + q]);
-            step_angle<1>(f, s_ang[TR + q0 + q]);
-            if (r < n - 1) step_angle<2>(f, s_ang[2 * TR + q0 + q]);
-        }
-    }
+    *b_out = b;
+    *t_out = t;
 }
 
 // E <- tile_pfx * wave_aggr[0] * ... * wave_aggr[wave-1] * E, built right to
@@ -352,14 +450,61 @@ __device__ __forceinline__ void lane_prefix(const double (*s_tot)[kF], int wave,
     if (tile_pfx) frame_lmul(tile_pfx, E);
 }
 
+// Pass 1 of both kernels: the lane composes its residues (local q0..q0+R-1)
+// from the identity and records each atom's position in that lane-local
+// frame: p[9*q + 3*k + c] for atom k (CA, C, N_next) of local residue q.
+template <typename T, int R>
+__device__ __forceinline__ void lane_pass1(const T* s_ang, int TR, int r0, int n, int q0,
+                                           const double* s_tab, T* s_p, double f[kF]) {
+    frame_identity(f);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int r = r0 + q0 + q;
+        if (r < n) {
+            const int na = (r < n - 1) ? 3 : 2;
+            T* p = s_p + 9 * (q0 + q);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (k < na) {
+                    double s, c;
+                    sincos_tab(static_cast<double>(s_ang[k * TR + q0 + q]), s_tab, &s, &c);
+                    if (k == 0) frame_step<0>(f, c, s);
+                    if (k == 1) frame_step<1>(f, c, s);
+                    if (k == 2) frame_step<2>(f, c, s);
+                    p[3 * k] = (T)f[9];
+                    p[3 * k + 1] = (T)f[10];
+                    p[3 * k + 2] = (T)f[11];
+                }
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void affine(const double E[kF], double p0, double p1, double p2,
+                                       double x[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) x[i] = E[i] * p0 + E[3 + i] * p1 + E[6 + i] * p2 + E[9 + i];
+}
+
+// Dynamic LDS bytes (the sincos table and small arrays are static).
+template <typename T>
+__host__ __device__ constexpr size_t fwd_lds_bytes(int tr) {
+    return sizeof(T) * (size_t)tr * 12;  // angles [3][TR], positions [9*TR]
+}
+template <typename T>
+__host__ __device__ constexpr size_t bwd_lds_bytes(int tr) {
+    return sizeof(T) * (size_t)tr * 21;  // angles/grads [3][TR], grad_coords [9*TR], positions [9*TR]
+}
+
 // ------------------------------------------------------------ forward scan
 // One tile = TR = 64*W*R residues of chain b.  Steps:
 //   1. stage the tile's angles in LDS (coalesced row loads);
-//   2. per lane: FP64 sincos and the serial product of its R residues; waves
-//      whose first residue is past the chain skip steps 2-4 (identity);
+//   2. pass 1: per lane, FP64 sincos and the serial product of its R residues,
+//      atom positions in the lane's local frame into LDS; waves whose first
+//      residue is past the chain skip passes 1-2;
 //   3. DPP scan over the wave, LDS over the waves, look-back over tiles;
-//   4. per lane: walk its residues from the prefix (sincos recomputed),
-//      positions into LDS;
+//   4. pass 2: x = E p with E the lane's global prefix frame, in place in LDS;
+//      the prefix frames at backward-tile starts go to the checkpoints;
 //   5. coalesced store of the tile's coordinates, zeros past the chain.
 template <typename T, int R, int MAXT>
 __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
@@ -369,6 +514,7 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
     const int TR = blockDim.x * R;
     T* s_ang = reinterpret_cast<T*>(s_dyn);
     T* s_xyz = s_ang + 3 * TR;
+    __shared__ double s_tab[128];
     __shared__ double s_tot[4][kF];
     __shared__ double s_pfx[kF];
     __shared__ double s_agg[kF];
@@ -376,24 +522,40 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
     const T* angles = static_cast<const T*>(a.angles);
     T* coords = static_cast<T*>(a.out);
     const int L = a.L;
+    for (int i = tid; i < 128; i += blockDim.x) s_tab[i] = kTabCS[i];
 
-    for (int iter = 0;; ++iter) {
-        const int tile = next_tile(a, iter, &s_tile);
-        if (tile < 0) break;
-        const int b = tile % a.B, t = tile / a.B;
+    for (int item = blockIdx.x; item < a.B * a.ntf; item += gridDim.x) {
+        __syncthreads();  // LDS of the previous tile is no longer read
+        int b, t;
+        item_tile(item, a.B, a.ntf, a.ftickets, false, &s_tile, &b, &t);
         const int n = clamp_length(a, b, t == 0 && tid == 0);
         const int r0 = t * TR;
         const int nvalid = min(max(n - r0, 0), TR);
         T* x_b = coords + (size_t)b * 9 * L;
 
+        // Initialise the backward's look-back words of the backward tiles
+        // inside this forward tile (the backward cleans them after use).
+        if (a.ntb > 1) {
+            const int per = TR / a.trb;
+            for (int j = tid; j < per; j += blockDim.x) {
+                const int tb = t * per + j;
+                if (tb < a.ntb) a.bflags[(size_t)b * a.ntb + tb] = 0u;
+            }
+            if (t == 0 && tid == 0) {
+                a.btickets[b] = 0u;
+                a.bdone[b] = 0u;
+            }
+        }
+
         if (nvalid > 0) {  // uniform: the tile holds residues of the chain
-            stage_angles(angles + (size_t)b * 3 * L, L, r0, nvalid, TR, s_ang);
+            for (int k = 0; k < 3; ++k)
+                load_span(angles + ((size_t)b * 3 + k) * L + r0, nvalid, s_ang + k * TR);
             __syncthreads();
             const int q0 = tid * R;
             const bool wave_active = wave * 64 * R < nvalid;  // uniform per wave
             double E[kF];
             if (wave_active) {
-                lane_aggregate<T, R>(s_ang, TR, r0, n, q0, E);
+                lane_pass1<T, R>(s_ang, TR, r0, n, q0, s_tab, s_xyz, E);
                 wave_scan_frames(E);
                 if (lane == 63) {
 #pragma unroll
@@ -405,12 +567,12 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
                 for (int e = 0; e < kF; ++e) s_tot[wave][e] = ident(e);
             }
             __syncthreads();
-            if (a.lookback) {
+            if (a.ntf > 1) {
                 if (tid == 0) {
-                    const size_t s0 = (size_t)b * a.ntiles;
-                    uint32_t* fl = a.flags + s0;
-                    double* pay = a.payload + s0 * kPayload;
-                    double* my = pay + (size_t)t * kPayload;
+                    const size_t s0 = (size_t)b * a.ntf;
+                    uint32_t* fl = a.fflags + s0;
+                    double* pay = a.fpay + s0 * kFwdPayload;
+                    double* my = pay + (size_t)t * kFwdPayload;
                     double acc[kF];
                     {   // tile aggregate = product of the wave aggregates
 #pragma unroll
@@ -428,7 +590,7 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
                         for (int j = t - 1; j >= 0; --j) {
                             const uint32_t fj = wait_flag(fl + j, a.status);
                             if (fj == 0) continue;
-                            const double* pj = pay + (size_t)j * kPayload;
+                            const double* pj = pay + (size_t)j * kFwdPayload;
                             const bool incl = (fj == kFlagInclusive);
                             frame_lmul(incl ? pj + kF : pj, acc);
                             if (incl) break;
@@ -436,31 +598,36 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
                         frame_rmul_to(acc, s_agg, my + kF);  // inclusive = prefix * aggregate
                         flag_store(fl + t, kFlagInclusive);
                     }
-                    double* ck = a.state + ((size_t)b * a.ntiles + t) * kF;
 #pragma unroll
-                    for (int e = 0; e < kF; ++e) {
-                        s_pfx[e] = acc[e];
-                        ck[e] = acc[e];
-                    }
+                    for (int e = 0; e < kF; ++e) s_pfx[e] = acc[e];
                 }
                 __syncthreads();
             }
             if (wave_active) {
-                // Residue r (local j) places CA_r, C_r and N_{r+1}: LDS floats
-                // 9j .. 9j+8 stand for global floats 9r+3 .. 9r+11.
-                lane_prefix(s_tot, wave, a.lookback ? s_pfx : nullptr, E);
+                lane_prefix(s_tot, wave, a.ntf > 1 ? s_pfx : nullptr, E);
+                const int rl = r0 + q0;  // first residue of this lane
+                if (a.ntb > 1 && rl < n && rl % a.trb == 0) {
+                    double* ck = a.ckpt + ((size_t)b * a.ntb + rl / a.trb) * kF;
+#pragma unroll
+                    for (int e = 0; e < kF; ++e) ck[e] = E[e];
+                }
+                // Pass 2: local positions -> global, in place.  Residue r (local
+                // j) holds CA_r, C_r, N_{r+1} = global floats 9r+3 .. 9r+11.
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
-                    const int r = r0 + q0 + q;
+                    const int r = rl + q;
                     if (r < n) {
-                        T* o = s_xyz + 9 * (q0 + q);
-                        step_angle<0>(E, s_ang[q0 + q]);
-                        o[0] = (T)E[9]; o[1] = (T)E[10]; o[2] = (T)E[11];
-                        step_angle<1>(E, s_ang[TR + q0 + q]);
-                        o[3] = (T)E[9]; o[4] = (T)E[10]; o[5] = (T)E[11];
-                        if (r < n - 1) {
-                            step_angle<2>(E, s_ang[2 * TR + q0 + q]);
-                            o[6] = (T)E[9]; o[7] = (T)E[10]; o[8] = (T)E[11];
+                        const int na = (r < n - 1) ? 3 : 2;
+                        T* p = s_xyz + 9 * (q0 + q);
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            if (k < na) {
+                                double x[3];
+                                affine(E, p[3 * k], p[3 * k + 1], p[3 * k + 2], x);
+                                p[3 * k] = (T)x[0];
+                                p[3 * k + 1] = (T)x[1];
+                                p[3 * k + 2] = (T)x[2];
+                            }
                         }
                     }
                 }
@@ -468,35 +635,27 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
             __syncthreads();
         }
 
-        // Coalesced store of global floats [9*r0+3, 9*r0+3+9*TR) within [0, 9L):
-        // atoms < 3n from LDS, the rest 0.  Tile 0 also writes N_0 = 0.
-        bool bad = false;
+        // Store global floats [9*r0+3, 9*r0+3+9*TR) within [0, 9L): atoms < 3n
+        // from LDS, the rest 0.  Tile 0 also writes N_0 = 0.
         if (t == 0 && tid < 3) x_b[tid] = (T)0;
         const int g0 = 9 * r0 + 3;
-        const int gend = min(g0 + 9 * TR, 9 * L);
-        const int gvalid = 9 * n;
-        for (int g = g0 + tid; g < gend; g += blockDim.x) {
-            T v = (T)0;
-            if (g < gvalid) {
-                v = s_xyz[g - g0];
-                bad |= (v != v);
-            }
-            x_b[g] = v;
-        }
-        if (bad) set_status(a.status, TPL_A2B_STATUS_NAN);
+        const int cnt = min(9 * TR, 9 * L - g0);
+        const int nsrc = min(max(9 * n - g0, 0), cnt);
+        if (store_span(x_b + g0, cnt, s_xyz, nsrc)) set_status(a.status, TPL_A2B_STATUS_NAN);
     }
 }
 
 // ----------------------------------------------------------- backward scan
-// Same tiling, tickets from the last tile of each chain to the first.  Steps:
+// Backward tiles of TR = 64*W*R residues, taken from the last tile of each
+// chain to the first.  Steps:
 //   1. stage angles and the tile's grad_coords in LDS (coalesced);
-//   2. lane/wave/tile prefix as in the forward, the tile prefix read from the
-//      forward's checkpoint;
-//   3. pass 2: positions, and this lane's wrench sum (sum x x g, sum g);
+//   2. pass 1 as in the forward (local positions p), DPP/LDS scan; the tile
+//      prefix is the forward's checkpoint;
+//   3. pass 2: x = E p and this lane's wrench sum W = (sum x x g, sum g);
 //   4. suffix sums of wrenches over lanes (DPP), waves (LDS) and tiles
 //      (reverse look-back);
-//   5. pass 3: per atom m, S_m = sum over atoms after m; the gradient is
-//      u_m . S_T + (x_m x u_m) . S_F; gradients to LDS;
+//   5. pass 3: per atom m, S_m = sum over atoms after m, u_m = E.R (p_m -
+//      p_{m-1}) / d_m, gradient u_m . S_T + (x_m x u_m) . S_F, into LDS;
 //   6. coalesced store of the tile's gradients, zeros past the chain.
 template <typename T, int R, int MAXT>
 __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
@@ -504,9 +663,10 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nw = blockDim.x >> 6;
     const int TR = blockDim.x * R;
-    T* s_ang = reinterpret_cast<T*>(s_dyn);
+    T* s_ang = reinterpret_cast<T*>(s_dyn);  // angles, then gradients
     T* s_g = s_ang + 3 * TR;
-    T* s_out = s_g + 9 * TR;
+    T* s_p = s_g + 9 * TR;
+    __shared__ double s_tab[128];
     __shared__ double s_tot[4][kF];
     __shared__ double s_wsum[4][6];
     __shared__ double s_carry[6];
@@ -515,30 +675,30 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
     const T* gcoords = static_cast<const T*>(a.grad);
     T* gangles = static_cast<T*>(a.out);
     const int L = a.L;
+    for (int i = tid; i < 128; i += blockDim.x) s_tab[i] = kTabCS[i];
 
-    for (int iter = 0;; ++iter) {
-        const int tile = next_tile(a, iter, &s_tile);
-        if (tile < 0) break;
-        const int b = tile % a.B, t = a.ntiles - 1 - tile / a.B;
+    for (int item = blockIdx.x; item < a.B * a.ntb; item += gridDim.x) {
+        __syncthreads();
+        int b, t;
+        item_tile(item, a.B, a.ntb, a.btickets, true, &s_tile, &b, &t);
         const int n = clamp_length(a, b, t == 0 && tid == 0);
         const int r0 = t * TR;
         const int nvalid = min(max(n - r0, 0), TR);
         T* ga_b = gangles + (size_t)b * 3 * L;
 
         if (nvalid > 0) {  // uniform
-            stage_angles(angles + (size_t)b * 3 * L, L, r0, nvalid, TR, s_ang);
+            for (int k = 0; k < 3; ++k)
+                load_span(angles + ((size_t)b * 3 + k) * L + r0, nvalid, s_ang + k * TR);
             {
-                const T* g_b = gcoords + (size_t)b * 9 * L;
                 const int g0 = 9 * r0 + 3;
-                const int gend = min(g0 + 9 * TR, 9 * n);
-                for (int g = g0 + tid; g < gend; g += blockDim.x) s_g[g - g0] = g_b[g];
+                load_span(gcoords + (size_t)b * 9 * L + g0, min(9 * TR, 9 * n - g0), s_g);
             }
             __syncthreads();
             const int q0 = tid * R;
             const bool wave_active = wave * 64 * R < nvalid;
             double E[kF];
             if (wave_active) {
-                lane_aggregate<T, R>(s_ang, TR, r0, n, q0, E);
+                lane_pass1<T, R>(s_ang, TR, r0, n, q0, s_tab, s_p, E);
                 wave_scan_frames(E);
                 if (lane == 63) {
 #pragma unroll
@@ -550,27 +710,24 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
             double W[6] = {0, 0, 0, 0, 0, 0};  // (sum x x g, sum g) of this lane
             if (wave_active) {
                 lane_prefix(s_tot, wave,
-                            a.lookback ? a.state + ((size_t)b * a.ntiles + t) * kF : nullptr, E);
-                double X[kF];
-#pragma unroll
-                for (int e = 0; e < kF; ++e) X[e] = E[e];
+                            (a.ntb > 1 && t > 0) ? a.ckpt + ((size_t)b * a.ntb + t) * kF : nullptr,
+                            E);
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
                     const int r = r0 + q0 + q;
                     if (r < n) {
                         const int na = (r < n - 1) ? 3 : 2;
+                        const T* p = s_p + 9 * (q0 + q);
+                        const T* g = s_g + 9 * (q0 + q);
 #pragma unroll
                         for (int k = 0; k < 3; ++k) {
                             if (k < na) {
-                                if (k == 0) step_angle<0>(X, s_ang[q0 + q]);
-                                if (k == 1) step_angle<1>(X, s_ang[TR + q0 + q]);
-                                if (k == 2) step_angle<2>(X, s_ang[2 * TR + q0 + q]);
-                                const T* gp = s_g + 9 * (q0 + q) + 3 * k;
-                                const double g0 = gp[0], g1 = gp[1], g2 = gp[2];
-                                const double x0 = X[9], x1 = X[10], x2 = X[11];
-                                W[0] += x1 * g2 - x2 * g1;
-                                W[1] += x2 * g0 - x0 * g2;
-                                W[2] += x0 * g1 - x1 * g0;
+                                double x[3];
+                                affine(E, p[3 * k], p[3 * k + 1], p[3 * k + 2], x);
+                                const double g0 = g[3 * k], g1 = g[3 * k + 1], g2 = g[3 * k + 2];
+                                W[0] += x[1] * g2 - x[2] * g1;
+                                W[1] += x[2] * g0 - x[0] * g2;
+                                W[2] += x[0] * g1 - x[1] * g0;
                                 W[3] += g0; W[4] += g1; W[5] += g2;
                             }
                         }
@@ -598,7 +755,7 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
 #pragma unroll
                 for (int e = 0; e < 6; ++e) C[e] += s_wsum[w][e];
             }
-            if (a.lookback) {
+            if (a.ntb > 1) {
                 if (tid == 0) {
                     double tot[6] = {0, 0, 0, 0, 0, 0}, carry[6] = {0, 0, 0, 0, 0, 0};
                     for (int w = 0; w < nw; ++w) {
@@ -606,13 +763,13 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
                         for (int e = 0; e < 6; ++e) tot[e] += s_wsum[w][e];
                     }
                     const int ntv = (n + TR - 1) / TR;  // tiles holding residues
-                    const size_t s0 = (size_t)b * a.ntiles;
-                    uint32_t* fl = a.flags + s0;
-                    double* pay = a.payload + s0 * kPayload;
-                    double* my = pay + (size_t)t * kPayload;
+                    const size_t s0 = (size_t)b * a.ntb;
+                    uint32_t* fl = a.bflags + s0;
+                    double* pay = a.bpay + s0 * kBwdPayload;
+                    double* my = pay + (size_t)t * kBwdPayload;
                     if (t == ntv - 1) {
 #pragma unroll
-                        for (int e = 0; e < 6; ++e) my[kF + e] = tot[e];
+                        for (int e = 0; e < 6; ++e) my[6 + e] = tot[e];
                         flag_store(fl + t, kFlagInclusive);
                     } else {
 #pragma unroll
@@ -621,14 +778,14 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
                         for (int j = t + 1; j < ntv; ++j) {
                             const uint32_t fj = wait_flag(fl + j, a.status);
                             if (fj == 0) continue;
-                            const double* pj = pay + (size_t)j * kPayload;
+                            const double* pj = pay + (size_t)j * kBwdPayload;
                             const bool incl = (fj == kFlagInclusive);
 #pragma unroll
-                            for (int e = 0; e < 6; ++e) carry[e] += incl ? pj[kF + e] : pj[e];
+                            for (int e = 0; e < 6; ++e) carry[e] += incl ? pj[6 + e] : pj[e];
                             if (incl) break;
                         }
 #pragma unroll
-                        for (int e = 0; e < 6; ++e) my[kF + e] = carry[e] + tot[e];
+                        for (int e = 0; e < 6; ++e) my[6 + e] = carry[e] + tot[e];
                         flag_store(fl + t, kFlagInclusive);
                     }
 #pragma unroll
@@ -639,62 +796,72 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
                 for (int e = 0; e < 6; ++e) C[e] += s_carry[e];
             }
             if (wave_active) {
-                // Pass 3: S_m = sum over atoms after m; gradients into LDS.
+                // Pass 3: S_m = sum over atoms after m; gradients into LDS
+                // (over the staged angles, which are no longer read).
                 double S[6];
 #pragma unroll
                 for (int e = 0; e < 6; ++e) S[e] = C[e] + W[e];
+                double pp0 = 0.0, pp1 = 0.0, pp2 = 0.0;  // previous atom (the lane's N)
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
                     const int r = r0 + q0 + q;
                     if (r < n) {
                         double gr[3] = {0.0, 0.0, 0.0};
                         const int na = (r < n - 1) ? 3 : 2;
+                        const T* p = s_p + 9 * (q0 + q);
+                        const T* g = s_g + 9 * (q0 + q);
 #pragma unroll
                         for (int k = 0; k < 3; ++k) {
                             if (k < na) {
-                                if (k == 0) step_angle<0>(E, s_ang[q0 + q]);
-                                if (k == 1) step_angle<1>(E, s_ang[TR + q0 + q]);
-                                if (k == 2) step_angle<2>(E, s_ang[2 * TR + q0 + q]);
-                                const T* gp = s_g + 9 * (q0 + q) + 3 * k;
-                                const double g0 = gp[0], g1 = gp[1], g2 = gp[2];
-                                const double x0 = E[9], x1 = E[10], x2 = E[11];
-                                const double u0 = E[0], u1 = E[1], u2 = E[2];
-                                S[0] -= x1 * g2 - x2 * g1;
-                                S[1] -= x2 * g0 - x0 * g2;
-                                S[2] -= x0 * g1 - x1 * g0;
+                                constexpr double kInvD[3] = {1.0 / 1.460, 1.0 / 1.525, 1.0 / 1.330};
+                                const double p0 = p[3 * k], p1 = p[3 * k + 1], p2 = p[3 * k + 2];
+                                double x[3];
+                                affine(E, p0, p1, p2, x);
+                                const double d0 = (p0 - pp0) * kInvD[k];
+                                const double d1 = (p1 - pp1) * kInvD[k];
+                                const double d2 = (p2 - pp2) * kInvD[k];
+                                pp0 = p0; pp1 = p1; pp2 = p2;
+                                double u[3];
+#pragma unroll
+                                for (int i = 0; i < 3; ++i)
+                                    u[i] = E[i] * d0 + E[3 + i] * d1 + E[6 + i] * d2;
+                                const double g0 = g[3 * k], g1 = g[3 * k + 1], g2 = g[3 * k + 2];
+                                S[0] -= x[1] * g2 - x[2] * g1;
+                                S[1] -= x[2] * g0 - x[0] * g2;
+                                S[2] -= x[0] * g1 - x[1] * g0;
                                 S[3] -= g0; S[4] -= g1; S[5] -= g2;
-                                const double w0 = x1 * u2 - x2 * u1;
-                                const double w1 = x2 * u0 - x0 * u2;
-                                const double w2 = x0 * u1 - x1 * u0;
-                                gr[k] = u0 * S[0] + u1 * S[1] + u2 * S[2] + w0 * S[3] +
+                                const double w0 = x[1] * u[2] - x[2] * u[1];
+                                const double w1 = x[2] * u[0] - x[0] * u[2];
+                                const double w2 = x[0] * u[1] - x[1] * u[0];
+                                gr[k] = u[0] * S[0] + u[1] * S[1] + u[2] * S[2] + w0 * S[3] +
                                         w1 * S[4] + w2 * S[5];
                             }
                         }
                         if (r == n - 1) gr[1] = 0.0;  // psi of the last residue moves no atom
-                        s_out[q0 + q] = (T)gr[0];
-                        s_out[TR + q0 + q] = (T)gr[1];
-                        s_out[2 * TR + q0 + q] = (T)gr[2];
+                        s_ang[q0 + q] = (T)gr[0];
+                        s_ang[TR + q0 + q] = (T)gr[1];
+                        s_ang[2 * TR + q0 + q] = (T)gr[2];
                     }
                 }
             }
             __syncthreads();
         }
 
-        // Coalesced store of the tile's gradients, zeros past the chain.
+        // Store the tile's gradients, zeros past the chain.
         bool bad = false;
-        const int jend = min(TR, L - r0);
-        for (int k = 0; k < 3; ++k) {
-            T* dst = ga_b + (size_t)k * L + r0;
-            for (int j = tid; j < jend; j += blockDim.x) {
-                T v = (T)0;
-                if (j < nvalid) {
-                    v = s_out[k * TR + j];
-                    bad |= (v != v);
-                }
-                dst[j] = v;
+        const int cnt = min(TR, L - r0);
+        for (int k = 0; k < 3; ++k)
+            bad |= store_span(ga_b + (size_t)k * L + r0, cnt, s_ang + k * TR, nvalid);
+        if (bad) set_status(a.status, TPL_A2B_STATUS_NAN);
+
+        // Last tile of the chain to finish returns its look-back words to 0.
+        if (a.ntb > 1 && tid == 0) {
+            if (atomic_inc(a.bdone + b) == (uint32_t)(a.ntb - 1)) {
+                for (int j = 0; j < a.ntb; ++j) a.bflags[(size_t)b * a.ntb + j] = 0u;
+                a.btickets[b] = 0u;
+                a.bdone[b] = 0u;
             }
         }
-        if (bad) set_status(a.status, TPL_A2B_STATUS_NAN);
     }
 }
 
@@ -851,33 +1018,66 @@ bool valid_config(const tpl_a2b_config* c) {
     return wok && rok && c->max_wgs >= 0 && c->flags == 0;
 }
 
-// Largest tile whose LDS staging (see lds_bytes) stays within 64 KiB.
-constexpr int kMaxTileF32 = 1024, kMaxTileF64 = 512;
+// Largest forward / backward tiles whose LDS staging (fwd_lds_bytes,
+// bwd_lds_bytes) stays within 48 KiB, so three workgroups fit one CU.
+constexpr int kMaxFwdTileF32 = 1024, kMaxFwdTileF64 = 512;
+constexpr int kMaxBwdTileF32 = 512, kMaxBwdTileF64 = 256;
 
 bool fits_dtype(int dtype, const tpl_a2b_config* c) {
     const int tr = 64 * c->waves * c->res_per_lane;
-    return tr <= (dtype == TPL_A2B_F32 ? kMaxTileF32 : kMaxTileF64);
+    return tr <= (dtype == TPL_A2B_F32 ? kMaxFwdTileF32 : kMaxFwdTileF64);
 }
 
-int tiles_for(int L, const tpl_a2b_config* c) {
-    const int tr = 64 * c->waves * c->res_per_lane;
-    return L <= 0 ? 0 : (L + tr - 1) / tr;
+// Tiling of one call: forward tiles of trf residues, backward tiles of trb
+// (= trf, or a divisor of it when the backward's LDS needs smaller tiles).
+struct Tiling {
+    int trf, ntf, trb, ntb, rb;
+};
+
+Tiling tiling(int dtype, int L, const tpl_a2b_config* c) {
+    Tiling t{};
+    t.trf = 64 * c->waves * c->res_per_lane;
+    t.trb = std::min(t.trf, dtype == TPL_A2B_F32 ? kMaxBwdTileF32 : kMaxBwdTileF64);
+    t.rb = t.trb / (64 * c->waves);
+    t.ntf = L <= 0 ? 0 : (L + t.trf - 1) / t.trf;
+    t.ntb = L <= 0 ? 0 : (L + t.trb - 1) / t.trb;
+    return t;
 }
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Per-call scratch of the forward's own look-back (ntf > 1): flags [B][ntf]
+// and tickets [B] (zeroed by a2b_zero_words before the forward), payload.
 struct SyncLayout {
-    size_t flags_off, ticket_off, words, payload_off, total;
+    size_t flags_off, tickets_off, words, pay_off, total;
 };
-
-SyncLayout sync_layout(int B, int nt) {
+SyncLayout sync_layout(int B, int ntf) {
     SyncLayout s{};
-    const size_t tiles = (size_t)B * nt;
+    if (ntf <= 1) return s;
+    const size_t tiles = (size_t)B * ntf;
     s.flags_off = 0;
-    s.ticket_off = tiles * 4;
-    s.words = tiles + 1;  // flags + ticket, all zeroed per call
-    s.payload_off = align256(s.words * 4);
-    s.total = s.payload_off + tiles * kPayload * sizeof(double);
+    s.tickets_off = tiles * 4;
+    s.words = tiles + B;
+    s.pay_off = align256(s.words * 4);
+    s.total = s.pay_off + tiles * kFwdPayload * sizeof(double);
+    return s;
+}
+
+// Forward -> backward state (ntb > 1): checkpoints, backward look-back payload,
+// flags, tickets and done counters.
+struct StateLayout {
+    size_t ckpt_off, pay_off, flags_off, tickets_off, done_off, total;
+};
+StateLayout state_layout(int B, int ntb) {
+    StateLayout s{};
+    if (ntb <= 1) return s;
+    const size_t tiles = (size_t)B * ntb;
+    s.ckpt_off = 0;
+    s.pay_off = tiles * kF * sizeof(double);
+    s.flags_off = s.pay_off + tiles * kBwdPayload * sizeof(double);
+    s.tickets_off = s.flags_off + tiles * 4;
+    s.done_off = s.tickets_off + (size_t)B * 4;
+    s.total = align256(s.done_off + (size_t)B * 4);
     return s;
 }
 
@@ -891,8 +1091,8 @@ KernelFn pick_kernel(bool fwd) {
 }
 
 template <typename T>
-KernelFn pick_kernel(bool fwd, const tpl_a2b_config* c) {
-    switch (c->res_per_lane) {
+KernelFn pick_kernel(bool fwd, int r) {
+    switch (r) {
         case 1: return pick_kernel<T, 1>(fwd);
         case 2: return pick_kernel<T, 2>(fwd);
         case 4: return pick_kernel<T, 4>(fwd);
@@ -903,33 +1103,47 @@ KernelFn pick_kernel(bool fwd, const tpl_a2b_config* c) {
 int check_common(int dtype, int B, int L) {
     if (dtype != TPL_A2B_F32 && dtype != TPL_A2B_F64) return TPL_A2B_ERR_ARG;
     if (B < 0 || L < 0) return TPL_A2B_ERR_ARG;
-    if ((int64_t)B * 9 * (int64_t)L > (int64_t)INT32_MAX * 4) return TPL_A2B_ERR_SIZE;
+    if ((int64_t)9 * L > INT32_MAX || (int64_t)B * L > INT32_MAX) return TPL_A2B_ERR_SIZE;
     return TPL_A2B_OK;
 }
 
-int launch_scan(bool fwd, int dtype, ScanArgs args, void* sync, const tpl_a2b_config* cfg,
-                hipStream_t stream) {
-    const int nt = args.ntiles;
-    const int total = args.B * nt;
-    args.lookback = nt > 1 ? 1 : 0;
-    if (args.lookback) {
-        const SyncLayout sl = sync_layout(args.B, nt);
+int launch_scan(bool fwd, int dtype, ScanArgs args, void* state, void* sync,
+                const tpl_a2b_config* cfg, hipStream_t stream) {
+    const Tiling tl = tiling(dtype, args.L, cfg);
+    args.trf = tl.trf;
+    args.ntf = tl.ntf;
+    args.trb = tl.trb;
+    args.ntb = tl.ntb;
+    if (tl.ntb > 1) {
+        const StateLayout st = state_layout(args.B, tl.ntb);
+        char* base = static_cast<char*>(state);
+        args.ckpt = reinterpret_cast<double*>(base + st.ckpt_off);
+        args.bpay = reinterpret_cast<double*>(base + st.pay_off);
+        args.bflags = reinterpret_cast<uint32_t*>(base + st.flags_off);
+        args.btickets = reinterpret_cast<uint32_t*>(base + st.tickets_off);
+        args.bdone = reinterpret_cast<uint32_t*>(base + st.done_off);
+    }
+    if (fwd && tl.ntf > 1) {
+        const SyncLayout sl = sync_layout(args.B, tl.ntf);
         char* base = static_cast<char*>(sync);
-        args.flags = reinterpret_cast<uint32_t*>(base + sl.flags_off);
-        args.ticket = reinterpret_cast<uint32_t*>(base + sl.ticket_off);
-        args.payload = reinterpret_cast<double*>(base + sl.payload_off);
+        args.fflags = reinterpret_cast<uint32_t*>(base + sl.flags_off);
+        args.ftickets = reinterpret_cast<uint32_t*>(base + sl.tickets_off);
+        args.fpay = reinterpret_cast<double*>(base + sl.pay_off);
         const int zb = (int)std::min<size_t>((sl.words + 255) / 256, 1024);
-        hipLaunchKernelGGL(a2b_zero_words, dim3(zb), dim3(256), 0, stream, args.flags,
+        hipLaunchKernelGGL(a2b_zero_words, dim3(zb), dim3(256), 0, stream, args.fflags,
                            (int)sl.words);
         if (hipGetLastError() != hipSuccess) return TPL_A2B_ERR_LAUNCH;
     }
-    int grid = total;
+    const int nt = fwd ? tl.ntf : tl.ntb;
+    int grid = args.B * nt;
     if (cfg->max_wgs > 0) grid = std::min(grid, cfg->max_wgs);
     const int threads = 64 * cfg->waves;
-    const int tr = threads * cfg->res_per_lane;
+    const int r = fwd ? cfg->res_per_lane : tl.rb;
     const bool f32 = dtype == TPL_A2B_F32;
-    KernelFn k = f32 ? pick_kernel<float>(fwd, cfg) : pick_kernel<double>(fwd, cfg);
-    const size_t lds = f32 ? lds_bytes<float>(tr, fwd) : lds_bytes<double>(tr, fwd);
+    KernelFn k = f32 ? pick_kernel<float>(fwd, r) : pick_kernel<double>(fwd, r);
+    const int tr = fwd ? tl.trf : tl.trb;
+    const size_t lds = fwd ? (f32 ? fwd_lds_bytes<float>(tr) : fwd_lds_bytes<double>(tr))
+                           : (f32 ? bwd_lds_bytes<float>(tr) : bwd_lds_bytes<double>(tr));
     hipLaunchKernelGGL(k, dim3(grid), dim3(threads), lds, stream, args);
     return hipGetLastError() == hipSuccess ? TPL_A2B_OK : TPL_A2B_ERR_LAUNCH;
 }
@@ -967,16 +1181,16 @@ int64_t tpl_a2b_tile_residues(const tpl_a2b_config* cfg) {
     return (int64_t)64 * cfg->waves * cfg->res_per_lane;
 }
 
-size_t tpl_a2b_state_bytes(int batch, int angles_stride, const tpl_a2b_config* cfg) {
-    if (!valid_config(cfg) || batch <= 0 || angles_stride <= 0) return 0;
-    const int nt = tiles_for(angles_stride, cfg);
-    return nt > 1 ? (size_t)batch * nt * kF * sizeof(double) : 0;
+size_t tpl_a2b_state_bytes(int dtype, int batch, int angles_stride, const tpl_a2b_config* cfg) {
+    if (check_common(dtype, batch, angles_stride) || !valid_config(cfg)) return 0;
+    if (batch == 0 || angles_stride == 0) return 0;
+    return state_layout(batch, tiling(dtype, angles_stride, cfg).ntb).total;
 }
 
-size_t tpl_a2b_sync_bytes(int batch, int angles_stride, const tpl_a2b_config* cfg) {
-    if (!valid_config(cfg) || batch <= 0 || angles_stride <= 0) return 0;
-    const int nt = tiles_for(angles_stride, cfg);
-    return nt > 1 ? sync_layout(batch, nt).total : 0;
+size_t tpl_a2b_sync_bytes(int dtype, int batch, int angles_stride, const tpl_a2b_config* cfg) {
+    if (check_common(dtype, batch, angles_stride) || !valid_config(cfg)) return 0;
+    if (batch == 0 || angles_stride == 0) return 0;
+    return sync_layout(batch, tiling(dtype, angles_stride, cfg).ntf).total;
 }
 
 int tpl_a2b_forward(int dtype, const void* angles, void* coords, const int32_t* lengths,
@@ -987,18 +1201,16 @@ int tpl_a2b_forward(int dtype, const void* angles, void* coords, const int32_t* 
     if (!valid_config(cfg) || !fits_dtype(dtype, cfg)) return TPL_A2B_ERR_CONFIG;
     if (batch == 0 || angles_stride == 0) return TPL_A2B_OK;
     if (!angles || !coords || !lengths) return TPL_A2B_ERR_ARG;
-    const int nt = tiles_for(angles_stride, cfg);
-    if (nt > 1 && (!state || !sync)) return TPL_A2B_ERR_ARG;
+    if (tpl_a2b_state_bytes(dtype, batch, angles_stride, cfg) && !state) return TPL_A2B_ERR_ARG;
+    if (tpl_a2b_sync_bytes(dtype, batch, angles_stride, cfg) && !sync) return TPL_A2B_ERR_ARG;
     ScanArgs a{};
     a.angles = angles;
     a.out = coords;
     a.lengths = lengths;
-    a.state = static_cast<double*>(state);
     a.status = status;
     a.B = batch;
     a.L = angles_stride;
-    a.ntiles = nt;
-    return launch_scan(true, dtype, a, sync, cfg, static_cast<hipStream_t>(stream));
+    return launch_scan(true, dtype, a, state, sync, cfg, static_cast<hipStream_t>(stream));
 }
 
 int tpl_a2b_backward(int dtype, void* grad_angles, const void* grad_coords, const void* angles,
@@ -1010,19 +1222,17 @@ int tpl_a2b_backward(int dtype, void* grad_angles, const void* grad_coords, cons
     if (!valid_config(cfg) || !fits_dtype(dtype, cfg)) return TPL_A2B_ERR_CONFIG;
     if (batch == 0 || angles_stride == 0) return TPL_A2B_OK;
     if (!grad_angles || !grad_coords || !angles || !lengths) return TPL_A2B_ERR_ARG;
-    const int nt = tiles_for(angles_stride, cfg);
-    if (nt > 1 && (!state || !sync)) return TPL_A2B_ERR_ARG;
+    if (tpl_a2b_state_bytes(dtype, batch, angles_stride, cfg) && !state) return TPL_A2B_ERR_ARG;
     ScanArgs a{};
     a.angles = angles;
     a.out = grad_angles;
     a.grad = grad_coords;
     a.lengths = lengths;
-    a.state = const_cast<double*>(static_cast<const double*>(state));
     a.status = status;
     a.B = batch;
     a.L = angles_stride;
-    a.ntiles = nt;
-    return launch_scan(false, dtype, a, sync, cfg, static_cast<hipStream_t>(stream));
+    return launch_scan(false, dtype, a, const_cast<void*>(state), sync, cfg,
+                       static_cast<hipStream_t>(stream));
 }
 
 size_t tpl_a2b_plain_frames_bytes(int dtype, int batch, int angles_stride) {

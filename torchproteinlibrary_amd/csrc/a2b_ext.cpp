@@ -105,8 +105,8 @@ void forward(torch::Tensor input_angles, torch::Tensor output_coords,
     check_out(output_coords, input_angles, {B, 9 * (int64_t)L}, "output_coords");
     const tpl_a2b_config cfg = to_config(config);
     c10::DeviceGuard guard(input_angles.device());
-    void* st = opt_ptr(state, tpl_a2b_state_bytes(B, L, &cfg), "state", input_angles);
-    void* sy = opt_ptr(sync, tpl_a2b_sync_bytes(B, L, &cfg), "sync", input_angles);
+    void* st = opt_ptr(state, tpl_a2b_state_bytes(dtype_code(input_angles), B, L, &cfg), "state", input_angles);
+    void* sy = opt_ptr(sync, tpl_a2b_sync_bytes(dtype_code(input_angles), B, L, &cfg), "sync", input_angles);
     check_rc(tpl_a2b_forward(dtype_code(input_angles), input_angles.data_ptr(),
                              output_coords.data_ptr(), angles_length.data_ptr<int32_t>(), st, sy,
                              status_ptr(status, input_angles), B, L, &cfg,
@@ -125,8 +125,8 @@ void backward(torch::Tensor grad_input, torch::Tensor grad_output, torch::Tensor
     check_out(grad_output, input_angles, {B, 9 * (int64_t)L}, "gradOutput");
     const tpl_a2b_config cfg = to_config(config);
     c10::DeviceGuard guard(input_angles.device());
-    void* st = opt_ptr(state, tpl_a2b_state_bytes(B, L, &cfg), "state", input_angles);
-    void* sy = opt_ptr(sync, tpl_a2b_sync_bytes(B, L, &cfg), "sync", input_angles);
+    void* st = opt_ptr(state, tpl_a2b_state_bytes(dtype_code(input_angles), B, L, &cfg), "state", input_angles);
+    void* sy = opt_ptr(sync, tpl_a2b_sync_bytes(dtype_code(input_angles), B, L, &cfg), "sync", input_angles);
     check_rc(tpl_a2b_backward(dtype_code(input_angles), grad_input.data_ptr(),
                               grad_output.data_ptr(), input_angles.data_ptr(),
                               angles_length.data_ptr<int32_t>(), st, sy,
@@ -166,10 +166,12 @@ void backward_plain(torch::Tensor grad_input, torch::Tensor grad_output,
              "Angles2BackboneGPU_backward_plain");
 }
 
-std::tuple<int64_t, int64_t> workspace_bytes(int64_t B, int64_t L, std::vector<int64_t> config) {
+std::tuple<int64_t, int64_t> workspace_bytes(bool f64, int64_t B, int64_t L,
+                                             std::vector<int64_t> config) {
     const tpl_a2b_config cfg = to_config(config);
-    return {(int64_t)tpl_a2b_state_bytes((int)B, (int)L, &cfg),
-            (int64_t)tpl_a2b_sync_bytes((int)B, (int)L, &cfg)};
+    const int dt = f64 ? TPL_A2B_F64 : TPL_A2B_F32;
+    return {(int64_t)tpl_a2b_state_bytes(dt, (int)B, (int)L, &cfg),
+            (int64_t)tpl_a2b_sync_bytes(dt, (int)B, (int)L, &cfg)};
 }
 
 int64_t plain_frames_bytes(bool f64, int64_t B, int64_t L) {
@@ -195,8 +197,8 @@ std::tuple<std::vector<int64_t>, std::vector<double>> tune(
     size_t need_state = 0, need_sync = 0;
     for (auto& c : candidates) {
         cands.push_back(to_config(c));
-        need_state = std::max(need_state, tpl_a2b_state_bytes(B, L, &cands.back()));
-        need_sync = std::max(need_sync, tpl_a2b_sync_bytes(B, L, &cands.back()));
+        need_state = std::max(need_state, tpl_a2b_state_bytes(dtype_code(input_angles), B, L, &cands.back()));
+        need_sync = std::max(need_sync, tpl_a2b_sync_bytes(dtype_code(input_angles), B, L, &cands.back()));
     }
     c10::DeviceGuard guard(input_angles.device());
     void* st = opt_ptr(state, need_state, "state", input_angles);
@@ -254,8 +256,8 @@ std::tuple<double, double> bench_steps(std::vector<std::vector<torch::Tensor>> s
         if (plain) {
             opt_ptr(s[5], tpl_a2b_plain_frames_bytes(dt, B, L), "frames", a0);
         } else {
-            opt_ptr(s[5], tpl_a2b_state_bytes(B, L, &cfg), "state", a0);
-            opt_ptr(s[6], tpl_a2b_sync_bytes(B, L, &cfg), "sync", a0);
+            opt_ptr(s[5], tpl_a2b_state_bytes(dt, B, L, &cfg), "state", a0);
+            opt_ptr(s[6], tpl_a2b_sync_bytes(dt, B, L, &cfg), "sync", a0);
         }
     }
     for (int64_t i = 0; i < warmup; ++i) one(sets[i % sets.size()]);

@@ -126,10 +126,10 @@ void backward(torch::Tensor grad_input, torch::Tensor grad_output, torch::Tensor
     const tpl_a2b_config cfg = to_config(config);
     c10::DeviceGuard guard(input_angles.device());
     void* st = opt_ptr(state, tpl_a2b_state_bytes(dtype_code(input_angles), B, L, &cfg), "state", input_angles);
-    void* sy = opt_ptr(sync, tpl_a2b_sync_bytes(dtype_code(input_angles), B, L, &cfg), "sync", input_angles);
+    (void)sync;  // the backward keeps its look-back words in `state`
     check_rc(tpl_a2b_backward(dtype_code(input_angles), grad_input.data_ptr(),
                               grad_output.data_ptr(), input_angles.data_ptr(),
-                              angles_length.data_ptr<int32_t>(), st, sy,
+                              angles_length.data_ptr<int32_t>(), st, nullptr,
                               status_ptr(status, input_angles), B, L, &cfg,
                               stream_of(input_angles)),
              "Angles2BackboneGPU_backward");

@@ -65,6 +65,9 @@ def parse():
     p.add_argument("--config", default="", help="pin tile config 'waves,res_per_lane,max_wgs'")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run 5 device copies of 256 MiB (counter calibration)")
+    p.add_argument("--fit", action="store_true",
+                   help="run the fit500 Adam optimisation loop instead (own JSON line)")
+    p.add_argument("--fit-steps", type=int, default=1000)
     return p.parse_args()
 
 
@@ -130,8 +133,77 @@ def cpu_baseline(a, n, g, seconds):
     }
 
 
+def fit_loop(steps: int):
+    """The fit500 optimisation loop: B=512 chains, lengths uniform in [50,500].
+    Targets are the FP64 coordinates of the seeded angles.  The start angles are
+    target + N(0, 0.3) rad on valid slots.  Loss = sum of squared coordinate
+    differences (padding is zero in both, so the sum is masked).  Adam,
+    lr 1e-2.  The same loop runs in float32 and in float64 from the same
+    start; the line reports end-to-end steps/s of each and the final losses.
+    Eager steps, with the per-call status read switched off."""
+    import numpy as np
+    import torch
+
+    from torchproteinlibrary_amd import synthetic
+    from torchproteinlibrary_amd.ReducedModel import Angles2Backbone
+
+    os.environ["TPL_A2B_CHECKS"] = "0"
+    dev = torch.device("cuda:0")
+    a, n, _ = synthetic.workload("fit500")
+    rng = np.random.default_rng(55)
+    start = a.astype(np.float64) + rng.normal(0.0, 0.3, a.shape)
+    start *= (np.arange(a.shape[2])[None, None, :] < n[:, None, None])
+    N = torch.tensor(n, device=dev)
+    m = Angles2Backbone()
+    with torch.no_grad():
+        target64 = m(torch.tensor(a, dtype=torch.float64, device=dev), N)
+    res = {}
+    for name, dt in (("f32", torch.float32), ("f64", torch.float64)):
+        target = target64.to(dt)
+        ang = torch.tensor(start, dtype=dt, device=dev).requires_grad_(True)
+        opt = torch.optim.Adam([ang], lr=1e-2)
+        loss0 = None
+
+        def step():
+            opt.zero_grad(set_to_none=True)
+            loss = ((m(ang, N) - target) ** 2).sum()
+            loss.backward()
+            opt.step()
+            return loss
+
+        for _ in range(3):  # warm-up (tuning, allocator)
+            step()
+        with torch.no_grad():
+            ang.copy_(torch.tensor(start, dtype=dt, device=dev))
+        opt = torch.optim.Adam([ang], lr=1e-2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            loss = step()
+            if i == 0:
+                loss0 = loss.detach()
+        torch.cuda.synchronize()
+        dt_s = time.perf_counter() - t0
+        with torch.no_grad():
+            final = float(((m(ang, N) - target) ** 2).sum())
+        res[name] = {"steps_per_s": steps / dt_s, "loss_start": float(loss0), "loss_final": final}
+    return res
+
+
 def main():
     args = parse()
+    if args.fit:
+        out = {"metric": "fit500 optimisation loop: Adam steps/s (fwd+bwd+update), B=512",
+               "unit": "steps/s", "steps": args.fit_steps, "n_gpus": 1, "data": "synthetic",
+               "config": {"workload": "fit500", "batch": 512, "max_length": 500, "lr": 1e-2,
+                          "optimizer": "Adam"}}
+        out["runs"] = fit_loop(args.fit_steps)
+        out["value"] = out["runs"]["f32"]["steps_per_s"]
+        out["final_loss_rel_diff_f32_vs_f64"] = (
+            abs(out["runs"]["f32"]["loss_final"] - out["runs"]["f64"]["loss_final"])
+            / max(out["runs"]["f64"]["loss_final"], 1e-30))
+        print(json.dumps(out), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); "
               "chains are independent, so batches shard across GPUs without exchange, "

@@ -255,3 +255,33 @@ def test_tuner_picks_a_candidate():
     t = _tuning.timings(A)
     assert t and all(v > 0 for v in t.values())
     print("fit500 B=128 tuner timings (ms):", t)
+
+
+def test_fit_loop_tracks_fp64():
+    """The fit500 usage pattern (docs/Examples/fitBackbone.py:44-63): Adam on the
+    angles toward FP64 target coordinates.  30 steps in float32 and in float64
+    from the same start; the loss falls and float32 tracks float64."""
+    a, n, _ = synthetic.workload("fit500", batch=64)
+    rng = np.random.default_rng(7)
+    start = a.astype(np.float64) + rng.normal(0.0, 0.3, a.shape)
+    start *= (np.arange(a.shape[2])[None, None, :] < n[:, None, None])
+    N = torch.tensor(n, device=DEV)
+    m = Angles2Backbone()
+    with torch.no_grad():
+        target = m(torch.tensor(a, dtype=torch.float64, device=DEV), N)
+    losses = {}
+    for dt in (torch.float32, torch.float64):
+        ang = torch.tensor(start, dtype=dt, device=DEV).requires_grad_(True)
+        opt = torch.optim.Adam([ang], lr=1e-2)
+        hist = []
+        for _ in range(30):
+            opt.zero_grad()
+            loss = ((m(ang, N) - target.to(dt)) ** 2).sum()
+            loss.backward()
+            opt.step()
+            hist.append(float(loss))
+        losses[dt] = hist
+    l32, l64 = losses[torch.float32], losses[torch.float64]
+    print(f"fit loop: loss {l64[0]:.4e} -> {l64[-1]:.4e} (f64), {l32[-1]:.4e} (f32)")
+    assert l64[-1] < 0.9 * l64[0]
+    assert abs(l32[-1] - l64[-1]) / l64[-1] < 1e-3

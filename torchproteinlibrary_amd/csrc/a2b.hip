@@ -480,6 +480,65 @@ __device__ __forceinline__ void lane_pass1(const T* s_ang, int TR, int r0, int n
     }
 }
 
+// Backward pass 1: as lane_pass1, and also the lane's moments of the
+// gradients in its local frame, M = sum_i p_i g_i^T (M[3d+c] = sum p_d g_c) and
+// F = sum_i g_i.  With the lane's global frame (R, t), its wrench total is
+// sum_i x_i x g_i = eps:(R M) + t x F (see wrench_total).
+template <typename T, int R>
+__device__ __forceinline__ void lane_pass1_moments(const T* s_ang, int TR, int r0, int n, int q0,
+                                                   const double* s_tab, T* s_p, const T* s_g,
+                                                   double f[kF], double M[9], double F[3]) {
+    frame_identity(f);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) M[e] = 0.0;
+    F[0] = F[1] = F[2] = 0.0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int r = r0 + q0 + q;
+        if (r < n) {
+            const int na = (r < n - 1) ? 3 : 2;
+            T* p = s_p + 9 * (q0 + q);
+            const T* g = s_g + 9 * (q0 + q);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (k < na) {
+                    double s, c;
+                    sincos_tab(static_cast<double>(s_ang[k * TR + q0 + q]), s_tab, &s, &c);
+                    if (k == 0) frame_step<0>(f, c, s);
+                    if (k == 1) frame_step<1>(f, c, s);
+                    if (k == 2) frame_step<2>(f, c, s);
+                    const T p0 = (T)f[9], p1 = (T)f[10], p2 = (T)f[11];
+                    p[3 * k] = p0;
+                    p[3 * k + 1] = p1;
+                    p[3 * k + 2] = p2;
+                    const double pd[3] = {p0, p1, p2};
+                    const double gc[3] = {g[3 * k], g[3 * k + 1], g[3 * k + 2]};
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+#pragma unroll
+                        for (int cc = 0; cc < 3; ++cc) M[3 * d + cc] = fma(pd[d], gc[cc], M[3 * d + cc]);
+                    F[0] += gc[0]; F[1] += gc[1]; F[2] += gc[2];
+                }
+            }
+        }
+    }
+}
+
+// W = (sum_i x_i x g_i, sum_i g_i) of a lane from its moments and global frame
+// E = (R, t): sum (R p_i + t) x g_i = eps:(R M) + t x F.
+__device__ __forceinline__ void wrench_total(const double E[kF], const double M[9],
+                                             const double F[3], double W[6]) {
+    auto rm = [&](int b, int c) {  // (R M)_{bc} = sum_d R_{bd} M_{dc}, R_{bd} = E[3d+b]
+        return E[b] * M[c] + E[3 + b] * M[3 + c] + E[6 + b] * M[6 + c];
+    };
+    W[0] = rm(1, 2) - rm(2, 1) + (E[10] * F[2] - E[11] * F[1]);
+    W[1] = rm(2, 0) - rm(0, 2) + (E[11] * F[0] - E[9] * F[2]);
+    W[2] = rm(0, 1) - rm(1, 0) + (E[9] * F[1] - E[10] * F[0]);
+    W[3] = F[0];
+    W[4] = F[1];
+    W[5] = F[2];
+}
+
 __device__ __forceinline__ void affine(const double E[kF], double p0, double p1, double p2,
                                        double x[3]) {
 #pragma unroll
@@ -651,7 +710,8 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
 //   1. stage angles and the tile's grad_coords in LDS (coalesced);
 //   2. pass 1 as in the forward (local positions p), DPP/LDS scan; the tile
 //      prefix is the forward's checkpoint;
-//   3. pass 2: x = E p and this lane's wrench sum W = (sum x x g, sum g);
+//   3. the lane's wrench sum W = (sum x x g, sum g) from moments gathered in
+//      pass 1: sum (R p + t) x g = eps:(R sum p g^T) + t x sum g;
 //   4. suffix sums of wrenches over lanes (DPP), waves (LDS) and tiles
 //      (reverse look-back);
 //   5. pass 3: per atom m, S_m = sum over atoms after m, u_m = E.R (p_m -
@@ -697,8 +757,9 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
             const int q0 = tid * R;
             const bool wave_active = wave * 64 * R < nvalid;
             double E[kF];
+            double M[9], F[3];
             if (wave_active) {
-                lane_pass1<T, R>(s_ang, TR, r0, n, q0, s_tab, s_p, E);
+                lane_pass1_moments<T, R>(s_ang, TR, r0, n, q0, s_tab, s_p, s_g, E, M, F);
                 wave_scan_frames(E);
                 if (lane == 63) {
 #pragma unroll
@@ -712,27 +773,7 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
                 lane_prefix(s_tot, wave,
                             (a.ntb > 1 && t > 0) ? a.ckpt + ((size_t)b * a.ntb + t) * kF : nullptr,
                             E);
-#pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    const int r = r0 + q0 + q;
-                    if (r < n) {
-                        const int na = (r < n - 1) ? 3 : 2;
-                        const T* p = s_p + 9 * (q0 + q);
-                        const T* g = s_g + 9 * (q0 + q);
-#pragma unroll
-                        for (int k = 0; k < 3; ++k) {
-                            if (k < na) {
-                                double x[3];
-                                affine(E, p[3 * k], p[3 * k + 1], p[3 * k + 2], x);
-                                const double g0 = g[3 * k], g1 = g[3 * k + 1], g2 = g[3 * k + 2];
-                                W[0] += x[1] * g2 - x[2] * g1;
-                                W[1] += x[2] * g0 - x[0] * g2;
-                                W[2] += x[0] * g1 - x[1] * g0;
-                                W[3] += g0; W[4] += g1; W[5] += g2;
-                            }
-                        }
-                    }
-                }
+                wrench_total(E, M, F, W);
             }
             // Suffix sums of wrenches: lanes, waves and tiles after this one.
             double C[6];

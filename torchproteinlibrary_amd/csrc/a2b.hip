@@ -293,14 +293,47 @@ __device__ __forceinline__ void set_status(uint32_t* status, uint32_t bits) {
     if (status) __hip_atomic_fetch_or(status, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Cross-workgroup look-back traffic.  On gfx950 an agent-scope release or
+// acquire fence writes back or invalidates the whole L2 of the XCD (the L2s of
+// the eight XCDs are not coherent with each other), which the streaming
+// output of every other workgroup makes expensive.  So the look-back never
+// fences at agent scope:
+//   * flags, tickets and payload words move with relaxed agent-scope atomic
+//     loads/stores, which are coherent at device scope by themselves;
+//   * a publisher orders its payload stores before the flag store with
+//     s_waitcnt vmcnt(0) (vector memory operations complete), plus a
+//     workgroup-scope fence that stops the compiler from reordering;
+//   * a reader waits for its flag load the same way before loading payload.
+constexpr int kWaitVmcnt0 = 0 | (0x7 << 4) | (0xf << 8);  // vmcnt(0); expcnt, lgkmcnt: no wait
+
 __device__ __forceinline__ uint32_t flag_load(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void word_store(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Publishes flag value v after every earlier store of this thread completed.
 __device__ __forceinline__ void flag_store(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    word_store(p, v);
+}
+// After a flag was seen set: later loads are issued after the flag load.
+__device__ __forceinline__ void after_flag() {
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+__device__ __forceinline__ void payload_store(double* p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p),
+                       static_cast<unsigned long long>(__double_as_longlong(v)), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double payload_load(const double* p) {
+    return __longlong_as_double(static_cast<long long>(__hip_atomic_load(
+        reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
 }
 __device__ __forceinline__ uint32_t atomic_inc(uint32_t* p) {
-    return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 constexpr uint32_t kFlagAggregate = 1, kFlagInclusive = 2;
@@ -313,7 +346,10 @@ constexpr int kSpinLimit = 1 << 21;
 __device__ __forceinline__ uint32_t wait_flag(const uint32_t* p, uint32_t* status) {
     for (int spins = 0; spins < kSpinLimit; ++spins) {
         const uint32_t f = flag_load(p);
-        if (f != 0) return f;
+        if (f != 0) {
+            after_flag();
+            return f;
+        }
         __builtin_amdgcn_s_sleep(1);
     }
     set_status(status, TPL_A2B_STATUS_TIMEOUT);
@@ -598,11 +634,11 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
             const int per = TR / a.trb;
             for (int j = tid; j < per; j += blockDim.x) {
                 const int tb = t * per + j;
-                if (tb < a.ntb) a.bflags[(size_t)b * a.ntb + tb] = 0u;
+                if (tb < a.ntb) word_store(a.bflags + (size_t)b * a.ntb + tb, 0u);
             }
             if (t == 0 && tid == 0) {
-                a.btickets[b] = 0u;
-                a.bdone[b] = 0u;
+                word_store(a.btickets + b, 0u);
+                word_store(a.bdone + b, 0u);
             }
         }
 
@@ -640,7 +676,7 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
 #pragma unroll
                         for (int e = 0; e < kF; ++e) {
                             s_agg[e] = acc[e];
-                            my[(t == 0 ? kF : 0) + e] = acc[e];
+                            payload_store(my + (t == 0 ? kF : 0) + e, acc[e]);
                         }
                     }
                     flag_store(fl + t, t == 0 ? kFlagInclusive : kFlagAggregate);
@@ -651,10 +687,16 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
                             if (fj == 0) continue;
                             const double* pj = pay + (size_t)j * kFwdPayload;
                             const bool incl = (fj == kFlagInclusive);
-                            frame_lmul(incl ? pj + kF : pj, acc);
+                            double v[kF];
+#pragma unroll
+                            for (int e = 0; e < kF; ++e) v[e] = payload_load(pj + (incl ? kF : 0) + e);
+                            frame_lmul(v, acc);
                             if (incl) break;
                         }
-                        frame_rmul_to(acc, s_agg, my + kF);  // inclusive = prefix * aggregate
+                        double incl_v[kF];  // inclusive = prefix * aggregate
+                        frame_rmul_to(acc, s_agg, incl_v);
+#pragma unroll
+                        for (int e = 0; e < kF; ++e) payload_store(my + kF + e, incl_v[e]);
                         flag_store(fl + t, kFlagInclusive);
                     }
 #pragma unroll
@@ -810,11 +852,11 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
                     double* my = pay + (size_t)t * kBwdPayload;
                     if (t == ntv - 1) {
 #pragma unroll
-                        for (int e = 0; e < 6; ++e) my[6 + e] = tot[e];
+                        for (int e = 0; e < 6; ++e) payload_store(my + 6 + e, tot[e]);
                         flag_store(fl + t, kFlagInclusive);
                     } else {
 #pragma unroll
-                        for (int e = 0; e < 6; ++e) my[e] = tot[e];
+                        for (int e = 0; e < 6; ++e) payload_store(my + e, tot[e]);
                         flag_store(fl + t, kFlagAggregate);
                         for (int j = t + 1; j < ntv; ++j) {
                             const uint32_t fj = wait_flag(fl + j, a.status);
@@ -822,11 +864,11 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
                             const double* pj = pay + (size_t)j * kBwdPayload;
                             const bool incl = (fj == kFlagInclusive);
 #pragma unroll
-                            for (int e = 0; e < 6; ++e) carry[e] += incl ? pj[6 + e] : pj[e];
+                            for (int e = 0; e < 6; ++e) carry[e] += payload_load(pj + (incl ? 6 : 0) + e);
                             if (incl) break;
                         }
 #pragma unroll
-                        for (int e = 0; e < 6; ++e) my[6 + e] = carry[e] + tot[e];
+                        for (int e = 0; e < 6; ++e) payload_store(my + 6 + e, carry[e] + tot[e]);
                         flag_store(fl + t, kFlagInclusive);
                     }
 #pragma unroll
@@ -898,9 +940,9 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
         // Last tile of the chain to finish returns its look-back words to 0.
         if (a.ntb > 1 && tid == 0) {
             if (atomic_inc(a.bdone + b) == (uint32_t)(a.ntb - 1)) {
-                for (int j = 0; j < a.ntb; ++j) a.bflags[(size_t)b * a.ntb + j] = 0u;
-                a.btickets[b] = 0u;
-                a.bdone[b] = 0u;
+                for (int j = 0; j < a.ntb; ++j) word_store(a.bflags + (size_t)b * a.ntb + j, 0u);
+                word_store(a.btickets + b, 0u);
+                word_store(a.bdone + b, 0u);
             }
         }
     }

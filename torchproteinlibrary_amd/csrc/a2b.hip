@@ -65,22 +65,29 @@ __device__ __forceinline__ void frame_identity(double f[kF]) {
     for (int e = 0; e < kF; ++e) f[e] = ident(e);
 }
 
-// f <- f * B(theta, beta_K, d_K) given (cos theta, sin theta).
+// f <- f * B(theta, beta_K, d_K) given (cos theta, sin theta), if `valid`;
+// else f unchanged.
 //   f.R Ry(beta): c0' = cb c0 - sb c2, c1' = c1, c2' = sb c0 + cb c2
 //   * Rx(theta):  c1'' = c c1' + s c2', c2'' = c c2' - s c1'
 //   t' = t + d c0'
+// An invalid step runs with beta = 0, d = 0, theta = 0 (the identity), so
+// every atom of a lane executes the same straight-line code and the scheduler
+// can overlap one atom's sincos with the previous atom's step.
 template <int K>
-__device__ __forceinline__ void frame_step(double f[kF], double c, double s) {
+__device__ __forceinline__ void frame_step_masked(double f[kF], double c, double s, bool valid) {
     constexpr Bond bd = bond(K);
+    const double cb = valid ? bd.cb : 1.0, sb = valid ? bd.sb : 0.0, d = valid ? bd.d : 0.0;
+    c = valid ? c : 1.0;
+    s = valid ? s : 0.0;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const double a0 = f[i], a1 = f[3 + i], a2 = f[6 + i];
-        const double n0 = bd.cb * a0 - bd.sb * a2;
-        const double p2 = bd.sb * a0 + bd.cb * a2;
+        const double n0 = cb * a0 - sb * a2;
+        const double p2 = sb * a0 + cb * a2;
         f[i] = n0;
         f[3 + i] = c * a1 + s * p2;
         f[6 + i] = c * p2 - s * a1;
-        f[9 + i] = fma(bd.d, n0, f[9 + i]);
+        f[9 + i] = fma(d, n0, f[9 + i]);
     }
 }
 
@@ -199,14 +206,13 @@ constexpr double kS3 = -0x1.5555555555555p-3, kS5 = 0x1.1111111111111p-7,
 constexpr double kC2 = -0x1.0000000000000p-1, kC4 = 0x1.5555555555555p-5,
                  kC6 = -0x1.6c16c16c16c17p-10, kC8 = 0x1.a01a01a01a01ap-16;
 
-// tab: the 128 doubles of kTabCS (in LDS).
+// tab: the 128 doubles of kTabCS (in LDS).  Branch-free.  The reduction is
+// exact for |k| < 2^20, i.e. |x| < 1e5 rad; beyond that it loses bits
+// gradually (2e-10 at |x| = 1e6).  The table index stays in [0, 63] for every
+// input, including NaN and inf (which give NaN).
 __device__ __forceinline__ void sincos_tab(double x, const double* tab, double* s_out,
                                            double* c_out) {
     const double k = rint(x * kTabInvStep);
-    if (!(fabs(k) < 1048576.0)) {  // also catches NaN/inf
-        sincos(x, s_out, c_out);
-        return;
-    }
     double d = fma(-k, kTabStep1, x);
     d = fma(-k, kTabStep2, d);
     d = fma(-k, kTabStep3, d);
@@ -489,6 +495,8 @@ __device__ __forceinline__ void lane_prefix(const double (*s_tot)[kF], int wave,
 // Pass 1 of both kernels: the lane composes its residues (local q0..q0+R-1)
 // from the identity and records each atom's position in that lane-local
 // frame: p[9*q + 3*k + c] for atom k (CA, C, N_next) of local residue q.
+// Atoms past the chain take masked identity steps; their slots get the
+// unchanged position and are never read.
 template <typename T, int R>
 __device__ __forceinline__ void lane_pass1(const T* s_ang, int TR, int r0, int n, int q0,
                                            const double* s_tab, T* s_p, double f[kF]) {
@@ -496,22 +504,18 @@ __device__ __forceinline__ void lane_pass1(const T* s_ang, int TR, int r0, int n
 #pragma unroll
     for (int q = 0; q < R; ++q) {
         const int r = r0 + q0 + q;
-        if (r < n) {
-            const int na = (r < n - 1) ? 3 : 2;
-            T* p = s_p + 9 * (q0 + q);
+        const bool v01 = r < n, v2 = r < n - 1;
+        T* p = s_p + 9 * (q0 + q);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                if (k < na) {
-                    double s, c;
-                    sincos_tab(static_cast<double>(s_ang[k * TR + q0 + q]), s_tab, &s, &c);
-                    if (k == 0) frame_step<0>(f, c, s);
-                    if (k == 1) frame_step<1>(f, c, s);
-                    if (k == 2) frame_step<2>(f, c, s);
-                    p[3 * k] = (T)f[9];
-                    p[3 * k + 1] = (T)f[10];
-                    p[3 * k + 2] = (T)f[11];
-                }
-            }
+        for (int k = 0; k < 3; ++k) {
+            double sn, cs;
+            sincos_tab(static_cast<double>(s_ang[k * TR + q0 + q]), s_tab, &sn, &cs);
+            if (k == 0) frame_step_masked<0>(f, cs, sn, v01);
+            if (k == 1) frame_step_masked<1>(f, cs, sn, v01);
+            if (k == 2) frame_step_masked<2>(f, cs, sn, v2);
+            p[3 * k] = (T)f[9];
+            p[3 * k + 1] = (T)f[10];
+            p[3 * k + 2] = (T)f[11];
         }
     }
 }
@@ -531,31 +535,30 @@ __device__ __forceinline__ void lane_pass1_moments(const T* s_ang, int TR, int r
 #pragma unroll
     for (int q = 0; q < R; ++q) {
         const int r = r0 + q0 + q;
-        if (r < n) {
-            const int na = (r < n - 1) ? 3 : 2;
-            T* p = s_p + 9 * (q0 + q);
-            const T* g = s_g + 9 * (q0 + q);
+        const bool v01 = r < n, v2 = r < n - 1;
+        T* p = s_p + 9 * (q0 + q);
+        const T* g = s_g + 9 * (q0 + q);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                if (k < na) {
-                    double s, c;
-                    sincos_tab(static_cast<double>(s_ang[k * TR + q0 + q]), s_tab, &s, &c);
-                    if (k == 0) frame_step<0>(f, c, s);
-                    if (k == 1) frame_step<1>(f, c, s);
-                    if (k == 2) frame_step<2>(f, c, s);
-                    const T p0 = (T)f[9], p1 = (T)f[10], p2 = (T)f[11];
-                    p[3 * k] = p0;
-                    p[3 * k + 1] = p1;
-                    p[3 * k + 2] = p2;
-                    const double pd[3] = {p0, p1, p2};
-                    const double gc[3] = {g[3 * k], g[3 * k + 1], g[3 * k + 2]};
+        for (int k = 0; k < 3; ++k) {
+            const bool valid = (k < 2) ? v01 : v2;
+            double sn, cs;
+            sincos_tab(static_cast<double>(s_ang[k * TR + q0 + q]), s_tab, &sn, &cs);
+            if (k == 0) frame_step_masked<0>(f, cs, sn, valid);
+            if (k == 1) frame_step_masked<1>(f, cs, sn, valid);
+            if (k == 2) frame_step_masked<2>(f, cs, sn, valid);
+            const T p0 = (T)f[9], p1 = (T)f[10], p2 = (T)f[11];
+            p[3 * k] = p0;
+            p[3 * k + 1] = p1;
+            p[3 * k + 2] = p2;
+            // grad_coords of atoms past the chain are not staged: use 0.
+            const double pd[3] = {p0, p1, p2};
+            const double gc[3] = {valid ? (double)g[3 * k] : 0.0, valid ? (double)g[3 * k + 1] : 0.0,
+                                  valid ? (double)g[3 * k + 2] : 0.0};
 #pragma unroll
-                    for (int d = 0; d < 3; ++d)
+            for (int d = 0; d < 3; ++d)
 #pragma unroll
-                        for (int cc = 0; cc < 3; ++cc) M[3 * d + cc] = fma(pd[d], gc[cc], M[3 * d + cc]);
-                    F[0] += gc[0]; F[1] += gc[1]; F[2] += gc[2];
-                }
-            }
+                for (int cc = 0; cc < 3; ++cc) M[3 * d + cc] = fma(pd[d], gc[cc], M[3 * d + cc]);
+            F[0] += gc[0]; F[1] += gc[1]; F[2] += gc[2];
         }
     }
 }
@@ -713,24 +716,16 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
                     for (int e = 0; e < kF; ++e) ck[e] = E[e];
                 }
                 // Pass 2: local positions -> global, in place.  Residue r (local
-                // j) holds CA_r, C_r, N_{r+1} = global floats 9r+3 .. 9r+11.
+                // j) holds CA_r, C_r, N_{r+1} = global floats 9r+3 .. 9r+11; slots
+                // past the chain are transformed too and never read.
 #pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    const int r = rl + q;
-                    if (r < n) {
-                        const int na = (r < n - 1) ? 3 : 2;
-                        T* p = s_xyz + 9 * (q0 + q);
-#pragma unroll
-                        for (int k = 0; k < 3; ++k) {
-                            if (k < na) {
-                                double x[3];
-                                affine(E, p[3 * k], p[3 * k + 1], p[3 * k + 2], x);
-                                p[3 * k] = (T)x[0];
-                                p[3 * k + 1] = (T)x[1];
-                                p[3 * k + 2] = (T)x[2];
-                            }
-                        }
-                    }
+                for (int a3 = 0; a3 < 3 * R; ++a3) {
+                    T* p = s_xyz + 9 * q0 + 3 * a3;
+                    double x[3];
+                    affine(E, p[0], p[1], p[2], x);
+                    p[0] = (T)x[0];
+                    p[1] = (T)x[1];
+                    p[2] = (T)x[2];
                 }
             }
             __syncthreads();
@@ -880,7 +875,9 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
             }
             if (wave_active) {
                 // Pass 3: S_m = sum over atoms after m; gradients into LDS
-                // (over the staged angles, which are no longer read).
+                // (over the staged angles, which are no longer read).  Atoms
+                // past the chain run masked (g = 0) and their results are
+                // dropped.
                 double S[6];
 #pragma unroll
                 for (int e = 0; e < 6; ++e) S[e] = C[e] + W[e];
@@ -888,43 +885,43 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
                     const int r = r0 + q0 + q;
-                    if (r < n) {
-                        double gr[3] = {0.0, 0.0, 0.0};
-                        const int na = (r < n - 1) ? 3 : 2;
-                        const T* p = s_p + 9 * (q0 + q);
-                        const T* g = s_g + 9 * (q0 + q);
+                    const bool v01 = r < n, v2 = r < n - 1;
+                    const T* p = s_p + 9 * (q0 + q);
+                    const T* g = s_g + 9 * (q0 + q);
+                    double gr[3];
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) {
-                            if (k < na) {
-                                constexpr double kInvD[3] = {1.0 / 1.460, 1.0 / 1.525, 1.0 / 1.330};
-                                const double p0 = p[3 * k], p1 = p[3 * k + 1], p2 = p[3 * k + 2];
-                                double x[3];
-                                affine(E, p0, p1, p2, x);
-                                const double d0 = (p0 - pp0) * kInvD[k];
-                                const double d1 = (p1 - pp1) * kInvD[k];
-                                const double d2 = (p2 - pp2) * kInvD[k];
-                                pp0 = p0; pp1 = p1; pp2 = p2;
-                                double u[3];
+                    for (int k = 0; k < 3; ++k) {
+                        constexpr double kInvD[3] = {1.0 / 1.460, 1.0 / 1.525, 1.0 / 1.330};
+                        const bool valid = (k < 2) ? v01 : v2;
+                        const double p0 = p[3 * k], p1 = p[3 * k + 1], p2 = p[3 * k + 2];
+                        double x[3];
+                        affine(E, p0, p1, p2, x);
+                        const double d0 = (p0 - pp0) * kInvD[k];
+                        const double d1 = (p1 - pp1) * kInvD[k];
+                        const double d2 = (p2 - pp2) * kInvD[k];
+                        pp0 = p0; pp1 = p1; pp2 = p2;
+                        double u[3];
 #pragma unroll
-                                for (int i = 0; i < 3; ++i)
-                                    u[i] = E[i] * d0 + E[3 + i] * d1 + E[6 + i] * d2;
-                                const double g0 = g[3 * k], g1 = g[3 * k + 1], g2 = g[3 * k + 2];
-                                S[0] -= x[1] * g2 - x[2] * g1;
-                                S[1] -= x[2] * g0 - x[0] * g2;
-                                S[2] -= x[0] * g1 - x[1] * g0;
-                                S[3] -= g0; S[4] -= g1; S[5] -= g2;
-                                const double w0 = x[1] * u[2] - x[2] * u[1];
-                                const double w1 = x[2] * u[0] - x[0] * u[2];
-                                const double w2 = x[0] * u[1] - x[1] * u[0];
-                                gr[k] = u[0] * S[0] + u[1] * S[1] + u[2] * S[2] + w0 * S[3] +
-                                        w1 * S[4] + w2 * S[5];
-                            }
-                        }
-                        if (r == n - 1) gr[1] = 0.0;  // psi of the last residue moves no atom
-                        s_ang[q0 + q] = (T)gr[0];
-                        s_ang[TR + q0 + q] = (T)gr[1];
-                        s_ang[2 * TR + q0 + q] = (T)gr[2];
+                        for (int i = 0; i < 3; ++i) u[i] = E[i] * d0 + E[3 + i] * d1 + E[6 + i] * d2;
+                        const double g0 = valid ? (double)g[3 * k] : 0.0;
+                        const double g1 = valid ? (double)g[3 * k + 1] : 0.0;
+                        const double g2 = valid ? (double)g[3 * k + 2] : 0.0;
+                        S[0] -= x[1] * g2 - x[2] * g1;
+                        S[1] -= x[2] * g0 - x[0] * g2;
+                        S[2] -= x[0] * g1 - x[1] * g0;
+                        S[3] -= g0; S[4] -= g1; S[5] -= g2;
+                        const double w0 = x[1] * u[2] - x[2] * u[1];
+                        const double w1 = x[2] * u[0] - x[0] * u[2];
+                        const double w2 = x[0] * u[1] - x[1] * u[0];
+                        const double gk = u[0] * S[0] + u[1] * S[1] + u[2] * S[2] + w0 * S[3] +
+                                          w1 * S[4] + w2 * S[5];
+                        gr[k] = valid ? gk : 0.0;
                     }
+                    if (r == n - 1) gr[1] = 0.0;  // psi of the last residue moves no atom
+                    // Slots past the chain (r >= n) are written too and never read.
+                    s_ang[q0 + q] = (T)gr[0];
+                    s_ang[TR + q0 + q] = (T)gr[1];
+                    s_ang[2 * TR + q0 + q] = (T)gr[2];
                 }
             }
             __syncthreads();

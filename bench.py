@@ -89,7 +89,7 @@ def time_kernels(sets, cfg, reps):
             if name == "forward":
                 ext.Angles2BackboneGPU_forward(A, X, N, st, sy, None, list(cfg))
             else:
-                ext.Angles2BackboneGPU_backward(GA, G, A, N, st, sy, None, list(cfg))
+                ext.Angles2BackboneGPU_backward(GA, G, X, N, st, None, list(cfg))
             if rec:
                 evs[i - 10][1].record(cur)
         torch.cuda.synchronize()

@@ -21,11 +21,14 @@
  * no atom.  The reference instead reads frames past the chain there.
  *
  * dtype is TPL_A2B_F32 or TPL_A2B_F64 for the angle/coordinate/gradient
- * arrays.  The scan kernels always compose transforms in FP64.
+ * arrays.  The scan kernels always compute in FP64.
  *
  * Streams: `stream` is a hipStream_t (NULL = the legacy default stream).  All
  * work is enqueued on it in stream order.  No call synchronises or allocates,
  * and every call is safe under hipStreamBeginCapture.
+ *
+ * `state` and `sync` buffers must start on a 256-byte boundary (hipMalloc and
+ * torch allocations do); a misaligned one returns TPL_A2B_ERR_ARG.
  *
  * Return value: TPL_A2B_OK (0) or an error code; tpl_a2b_strerror() names it.
  * Device-side conditions go to the optional `status` word as bits, because a
@@ -63,11 +66,11 @@ enum tpl_a2b_error {
 /* Launch shape of the scan kernels.  A forward tile is 64*waves*res_per_lane
  * consecutive residues of one chain; one workgroup scans one tile, and chains
  * longer than a tile are chained across workgroups by decoupled look-back.
- * Forward tiles hold at most 1024 residues for TPL_A2B_F32 and 512 for
- * TPL_A2B_F64, since their I/O is staged in LDS; larger tiles return
- * TPL_A2B_ERR_CONFIG.  The backward stages more per residue and uses tiles of
- * at most 512 (F32) / 256 (F64) residues: the forward's tile, halved when
- * needed. */
+ * Every lane keeps its residues' inputs and outputs in registers, which bounds
+ * the tile: at most 1024 residues for TPL_A2B_F32 and 512 for TPL_A2B_F64
+ * (larger tiles return TPL_A2B_ERR_CONFIG).  The backward holds more per
+ * residue and runs min(res_per_lane, 2) (F32) or 1 (F64) residues per lane
+ * with the same waves. */
 typedef struct tpl_a2b_config {
     int32_t waves;        /* 64-lane waves per workgroup: 1, 2 or 4 */
     int32_t res_per_lane; /* residues composed serially per lane: 1, 2, 4 or 8 */
@@ -85,13 +88,15 @@ int tpl_a2b_default_config(int batch, int angles_stride, tpl_a2b_config *cfg);
 /* Residues per tile of cfg, or -1 if cfg is invalid. */
 int64_t tpl_a2b_tile_residues(const tpl_a2b_config *cfg);
 
-/* Bytes of the forward->backward state for this dtype and config: one FP64
- * 3x4 frame checkpoint (the scan prefix) per backward-tile start, plus the
- * backward's look-back words.  The forward initialises all of it; the
- * backward leaves its look-back words as it found them, so one state can serve
- * repeated backward calls.  It replaces the reference's per-atom 4x4 frame
- * buffer A (Angles2Backbone.py:21) and is 0 when every chain fits in one
- * backward tile. */
+/* Bytes of the forward->backward state for this dtype and config.  For
+ * TPL_A2B_F32 it holds, per output coordinate, the rounding residual of the
+ * forward's FP64 value as a scaled half (2 bytes), so that the backward
+ * rebuilds FP64-grade coordinates from the FP32 ones; when chains span
+ * several backward tiles it also holds the backward's look-back words.  The
+ * forward writes all of it; the backward leaves its look-back words as it
+ * found them, so one state can serve repeated backward calls.  It replaces
+ * the reference's per-atom 4x4 frame buffer A (Angles2Backbone.py:21) and is
+ * 0 for TPL_A2B_F64 when every chain fits in one backward tile. */
 size_t tpl_a2b_state_bytes(int dtype, int batch, int angles_stride, const tpl_a2b_config *cfg);
 
 /* Bytes of the forward's look-back scratch for one call (0 when every chain
@@ -103,22 +108,23 @@ size_t tpl_a2b_sync_bytes(int dtype, int batch, int angles_stride, const tpl_a2b
  * (Layers/ReducedModel/Angles2Backbone/angles2backbone_interface.cpp:7-26) and
  * its kernel computeCoordinatesBackbone
  * (Layers/ReducedModel/cBackboneProteinCUDAKernels.cu:6-41,164-167).
- * Writes `state` (tpl_a2b_state_bytes) for the backward. */
+ * Writes `state` (tpl_a2b_state_bytes) for the backward; with state == NULL
+ * (a forward that no backward follows) it skips that work. */
 int tpl_a2b_forward(int dtype, const void *angles, void *coords, const int32_t *lengths,
                     void *state, void *sync, uint32_t *status, int batch, int angles_stride,
                     const tpl_a2b_config *cfg, tpl_stream_t stream);
 
 /* Backward: d(loss)/d(coords) -> d(loss)/d(angles), O(L) reverse-scan
- * adjoint.  Replaces Angles2BackboneGPU_backward
+ * adjoint over the forward's coordinates.  Replaces Angles2BackboneGPU_backward
  * (angles2backbone_interface.cpp:27-56) and its kernels
  * computeGradientsOptimizedBackbone + backwardFromCoordinatesBackbone
- * (cBackboneProteinCUDAKernels.cu:84-163,169-178).  `state` is what the forward
- * wrote for the same angles/lengths/cfg.  grad_angles is fully overwritten.
- * `sync` is not used by the backward (may be NULL). */
-int tpl_a2b_backward(int dtype, void *grad_angles, const void *grad_coords, const void *angles,
-                     const int32_t *lengths, const void *state, void *sync, uint32_t *status,
-                     int batch, int angles_stride, const tpl_a2b_config *cfg,
-                     tpl_stream_t stream);
+ * (cBackboneProteinCUDAKernels.cu:84-163,169-178).  Where the reference passes
+ * the input angles and the saved frames A, this takes the forward's output
+ * `coords` and `state` as the forward wrote them for the same lengths and cfg.
+ * grad_angles is fully overwritten. */
+int tpl_a2b_backward(int dtype, void *grad_angles, const void *grad_coords, const void *coords,
+                     const int32_t *lengths, void *state, uint32_t *status, int batch,
+                     int angles_stride, const tpl_a2b_config *cfg, tpl_stream_t stream);
 
 /* ---- plain form: the reference algorithm written as straightforward HIP ----
  * One thread per chain runs the serial recurrence and stores every atom's 3x4

@@ -33,7 +33,7 @@ def load():
     L.tpl_a2b_plain_frames_bytes.restype = ctypes.c_size_t
     L.tpl_a2b_plain_frames_bytes.argtypes = [i32, i32, i32]
     L.tpl_a2b_forward.argtypes = [i32, vp, vp, vp, vp, vp, vp, i32, i32, cfgp, vp]
-    L.tpl_a2b_backward.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, i32, i32, cfgp, vp]
+    L.tpl_a2b_backward.argtypes = [i32, vp, vp, vp, vp, vp, vp, i32, i32, cfgp, vp]
     L.tpl_a2b_forward_plain.argtypes = [i32, vp, vp, vp, vp, vp, i32, i32, vp]
     L.tpl_a2b_backward_plain.argtypes = [i32, vp, vp, vp, vp, vp, vp, i32, i32, vp]
     L.tpl_a2b_tune.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, i32, i32, cfgp, i32, i32,

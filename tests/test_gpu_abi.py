@@ -36,7 +36,7 @@ def test_forward_backward_via_c_abi(cfg):
     rc = lib.tpl_a2b_forward(0, _p(A), _p(X), _p(N), _p(st), _p(sy), _p(status), B, L,
                              ctypes.byref(c), stream)
     assert rc == 0, lib.tpl_a2b_strerror(rc)
-    rc = lib.tpl_a2b_backward(0, _p(GA), _p(G), _p(A), _p(N), _p(st), _p(sy), _p(status), B, L,
+    rc = lib.tpl_a2b_backward(0, _p(GA), _p(G), _p(X), _p(N), _p(st), _p(status), B, L,
                               ctypes.byref(c), stream)
     assert rc == 0, lib.tpl_a2b_strerror(rc)
     torch.cuda.synchronize()
@@ -70,8 +70,10 @@ def test_plain_via_c_abi_and_tuner():
     cands = (abi.Config * 3)(abi.Config(1, 1, 0, 0), abi.Config(4, 1, 0, 0), abi.Config(3, 1, 0, 0))
     best = abi.Config()
     ms = (ctypes.c_float * 3)()
-    st = torch.empty(4096, dtype=torch.uint8, device=dev)
-    sy = torch.empty(1 << 16, dtype=torch.uint8, device=dev)
+    st = torch.empty(max(lib.tpl_a2b_state_bytes(0, B, L, ctypes.byref(cands[i])) for i in range(2)),
+                     dtype=torch.uint8, device=dev)
+    sy = torch.empty(max(1, max(lib.tpl_a2b_sync_bytes(0, B, L, ctypes.byref(cands[i])) for i in range(2))),
+                     dtype=torch.uint8, device=dev)
     rc = lib.tpl_a2b_tune(0, _p(A), _p(N), _p(G), _p(X), _p(GA), _p(st), _p(sy), B, L, cands, 3, 3,
                           ctypes.byref(best), ms, stream)
     assert rc == 0

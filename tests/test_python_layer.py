@@ -32,9 +32,13 @@ def test_candidates_cover_the_length():
 
 
 def test_workspace_bytes_binding():
-    assert _native.ext.workspace_bytes(False, 512, 500, [2, 4, 0]) == (0, 0)
+    # float64, one 256-residue tile per chain both ways: nothing to carry
+    assert _native.ext.workspace_bytes(True, 512, 200, [4, 1, 0]) == (0, 0)
+    # float32: 2 bytes of residual per coordinate even with one tile
+    st, sy = _native.ext.workspace_bytes(False, 512, 200, [4, 1, 0])
+    assert st >= 512 * 9 * 200 * 2 and sy == 0
     st, sy = _native.ext.workspace_bytes(False, 64, 16384, [4, 4, 0])
-    assert st > 64 * 32 * 96 and sy > 64 * 16 * 24 * 8
+    assert st > 64 * 9 * 16384 * 2 + 64 * 32 * 96 and sy > 64 * 16 * 24 * 8
     with pytest.raises(RuntimeError):
         _native.ext.workspace_bytes(False, 4, 4, [3, 1, 0])
 

@@ -18,9 +18,11 @@ the error behaviour are the reference's:
 What differs, by design:
 
 * the forward is a parallel FP64 prefix scan over per-residue rigid transforms
-  and the backward an O(L) reverse-scan adjoint (no ``A``/``dr_dangle``
-  buffers: the reference's ``dr_dangle`` is O(B*L^2), 110 GB at B=1024,
-  L=1000);
+  and the backward an O(L) reverse-scan adjoint over the forward's output
+  coordinates (no ``A``/``dr_dangle`` buffers: the reference's ``dr_dangle``
+  is O(B*L^2), 110 GB at B=1024, L=1000); for float32 the forward also keeps
+  2 bytes per coordinate of FP64 rounding residual for the backward, and only
+  when the input requires grad;
 * gradients of psi[len-1] and omega[len-1] are exactly 0 (the reference reads
   frames past the chain there);
 * lengths outside [0, L_max] are clamped on the device and reported (the
@@ -93,30 +95,30 @@ class Angles2BackboneGPUFunction(Function):
         coords = torch.empty(B, 9 * L, dtype=input_angles.dtype, device=dev)
         st_bytes, sy_bytes = ext.workspace_bytes(input_angles.dtype == torch.float64, B, L,
                                                  list(cfg))
-        state = _empty_bytes(st_bytes, dev)
+        state = _empty_bytes(st_bytes, dev) if ctx.needs_input_grad[0] else None
         sync = _empty_bytes(sy_bytes, dev)
         status = _status_word(dev)
         ext.Angles2BackboneGPU_forward(input_angles, coords, angles_length, state, sync, status,
                                        list(cfg))
         _raise_on_status(status, "forward")
-        ctx.save_for_backward(input_angles, angles_length, state)
+        if state is not None:
+            ctx.save_for_backward(coords, angles_length, state)
         ctx.cfg = cfg
         return coords
 
     @staticmethod
     @once_differentiable
     def backward(ctx, gradOutput):
-        input_angles, angles_length, state = ctx.saved_tensors
+        coords, angles_length, state = ctx.saved_tensors
         gradOutput = gradOutput.contiguous()
-        if gradOutput.dtype != input_angles.dtype:
-            gradOutput = gradOutput.to(input_angles.dtype)
-        B, _, L = input_angles.shape
-        dev = input_angles.device
-        gradInput = torch.empty(B, 3, L, dtype=input_angles.dtype, device=dev)
-        sync = _empty_bytes(0, dev)  # the backward keeps its look-back words in `state`
+        if gradOutput.dtype != coords.dtype:
+            gradOutput = gradOutput.to(coords.dtype)
+        B, L = coords.shape[0], coords.shape[1] // 9
+        dev = coords.device
+        gradInput = torch.empty(B, 3, L, dtype=coords.dtype, device=dev)
         status = _status_word(dev)
-        ext.Angles2BackboneGPU_backward(gradInput, gradOutput, input_angles, angles_length, state,
-                                        sync, status, list(ctx.cfg))
+        ext.Angles2BackboneGPU_backward(gradInput, gradOutput, coords, angles_length, state,
+                                        status, list(ctx.cfg))
         _raise_on_status(status, "backward")
         return gradInput, None
 

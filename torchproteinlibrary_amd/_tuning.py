@@ -27,7 +27,7 @@ WAVES = (1, 2, 4)
 RES_PER_LANE = (1, 2, 4, 8)
 
 
-MAX_TILE = {torch.float32: 1024, torch.float64: 512}  # LDS staging limit per dtype
+MAX_TILE = {torch.float32: 1024, torch.float64: 512}  # register-tile limit per dtype
 
 
 def candidates(max_length: int, dtype=torch.float32):

@@ -10,22 +10,24 @@
 // N_{r+1}, so every lane reads its own angles and writes its own gradients.
 //
 // Scan kernels (a2b_fwd_scan / a2b_bwd_scan):
-//   * a tile is 64*W*R consecutive residues of one chain; its angles, coordinates
-//     and coordinate gradients pass through LDS so that every global access is
-//     a coalesced row segment;
-//   * lane l composes its R residues serially in FP64 (the structured 3x4 step
-//     below), the 64 lane aggregates are scanned with DPP (row_shr 1/2/4/8,
-//     row_bcast 15/31), the W wave aggregates through LDS, and tiles of one
-//     chain through a decoupled look-back on flags in global memory (tickets
-//     taken in chain order, so every tile waits only on tiles already being
-//     processed);
-//   * the backward recomputes frames from the forward's per-tile FP64 prefix
-//     checkpoints and runs the O(L) adjoint
-//       dL/dtheta_m = u_m . T_m + (x_m x u_m) . F_m,
-//       F_m = sum_{i>m} g_i,  T_m = sum_{i>m} x_i x g_i,
-//     a reverse (suffix) sum of 6-vector wrenches (x_i x g_i, g_i) over lanes,
-//     waves and tiles; u_m is column 0 of frame m (the bond axis rotated by
-//     theta_m).
+//   * a tile is 64*W*R consecutive residues of one chain; lane l owns R of
+//     them and moves its own angles, coordinates and gradients between HBM
+//     and registers (whole runs in 16-byte accesses where aligned); lanes past
+//     the chain write their zeros first;
+//   * forward: lane l composes its R residues serially in FP64 (the structured
+//     3x4 step below), the 64 lane aggregates are scanned with DPP (row_shr
+//     1/2/4/8, row_bcast 15/31), the W wave aggregates through LDS, and tiles
+//     of one chain through a decoupled look-back on flags in global memory
+//     (tickets taken in chain order, so every tile waits only on tiles already
+//     being processed); FP32 outputs also keep each coordinate's rounding
+//     residual as a scaled half;
+//   * backward: the O(L) adjoint from the forward's coordinates (rebuilt to
+//     FP64 grade with the residuals),
+//       dL/dtheta_j = u_j . (T_j - x_j x F_j),
+//       F_j = sum_{i>j} g_i,  T_j = sum_{i>j} x_i x g_i,
+//     with u_j the unit bond vector ending at atom j: a reverse (suffix) sum of
+//     the 6-vectors (x_i x g_i, g_i) over lanes, waves and tiles.  No frame is
+//     recomputed.
 // Plain kernels (a2b_fwd_plain / a2b_bwd_plain) restate the reference GPU
 // algorithm without the O(L^2) derivative tensor: they are the baseline.
 #include <hip/hip_runtime.h>
@@ -65,34 +67,25 @@ __device__ __forceinline__ void frame_identity(double f[kF]) {
     for (int e = 0; e < kF; ++e) f[e] = ident(e);
 }
 
-// f <- f * B(theta, beta_K, d_K) given (cos theta, sin theta), if `valid`;
-// else f unchanged.
+// f <- f * B(theta, beta_K, d_K) given (cos theta, sin theta).
 //   f.R Ry(beta): c0' = cb c0 - sb c2, c1' = c1, c2' = sb c0 + cb c2
 //   * Rx(theta):  c1'' = c c1' + s c2', c2'' = c c2' - s c1'
 //   t' = t + d c0'
-// An invalid step runs with beta = 0, d = 0, theta = 0 (the identity), so
-// every atom of a lane executes the same straight-line code and the scheduler
-// can overlap one atom's sincos with the previous atom's step.
 template <int K>
-__device__ __forceinline__ void frame_step_masked(double f[kF], double c, double s, bool valid) {
+__device__ __forceinline__ void frame_step(double f[kF], double c, double s) {
     constexpr Bond bd = bond(K);
-    const double cb = valid ? bd.cb : 1.0, sb = valid ? bd.sb : 0.0, d = valid ? bd.d : 0.0;
-    c = valid ? c : 1.0;
-    s = valid ? s : 0.0;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const double a0 = f[i], a1 = f[3 + i], a2 = f[6 + i];
-        const double n0 = cb * a0 - sb * a2;
-        const double p2 = sb * a0 + cb * a2;
+        const double n0 = bd.cb * a0 - bd.sb * a2;
+        const double p2 = bd.sb * a0 + bd.cb * a2;
         f[i] = n0;
         f[3 + i] = c * a1 + s * p2;
         f[6 + i] = c * p2 - s * a1;
-        f[9 + i] = fma(d, n0, f[9 + i]);
+        f[9 + i] = fma(bd.d, n0, f[9 + i]);
     }
 }
 
-// f <- M * f, in place one column at a time (M applied after f).  M may live
-// in registers, LDS or global memory.
 template <typename P>
 __device__ __forceinline__ void frame_lmul(const P& M, double f[kF]) {
 #pragma unroll
@@ -128,8 +121,7 @@ __device__ __forceinline__ void frame_rmul_to(const double f[kF], const double* 
 // with (c_j, s_j) = (cos, sin)(2 pi j / 64), j = k mod 64, from a table, and
 // Taylor series for sin d (to d^7) and cos d (to d^8), truncation < 5e-18.
 // Table entries are the exact values rounded to double.  Agreement with libm
-// is within 2.2e-16 for |x| < 1e4.  |k| >= 2^20 (|x| > 1e5) falls back to the
-// library's sincos.
+// is within 2.2e-16 for |x| < 1e4; the reduction stays exact to |x| < 1e5.
 constexpr double kTabInvStep = 0x1.45f306dc9c883p+3;  // 32/pi
 constexpr double kTabStep1 = 0x1.921fb54400000p-4;
 constexpr double kTabStep2 = 0x1.0b4611a600000p-38;
@@ -362,84 +354,102 @@ __device__ __forceinline__ uint32_t wait_flag(const uint32_t* p, uint32_t* statu
     return 0;
 }
 
-// 16-byte vector of T.
-template <typename T>
-struct Vec16;
+// ------------------------------------------------------------------ lane I/O
+// A lane moves a run of N consecutive elements of T between global memory
+// and registers.  Elements at or past `cnt` read as 0 and are not written.  A
+// run that is whole (cnt >= N) and U-byte aligned moves in U-byte accesses;
+// U = the widest of 16/8/4 bytes dividing the run, fixed per kernel, and every
+// lane of one row shares the alignment, so the branch is uniform there.
+template <int U>
+struct Unit;
 template <>
-struct Vec16<float> {
-    using type = float4;
-    static constexpr int n = 4;
+struct Unit<4> {
+    using type = uint32_t;
 };
 template <>
-struct Vec16<double> {
-    using type = double2;
-    static constexpr int n = 2;
+struct Unit<8> {
+    using type = uint2;
 };
-
-// Workgroup-wide store of dst[0, count): element i is src[i] (LDS) for
-// i < nsrc and 0 otherwise.  Aligned body in 16-byte stores.
-template <typename T>
-__device__ __forceinline__ bool store_span(T* dst, int count, const T* src, int nsrc) {
-    using V = typename Vec16<T>::type;
-    constexpr int E = Vec16<T>::n;
-    const int mis = (int)((reinterpret_cast<uintptr_t>(dst) / sizeof(T)) % E);
-    const int head = min((E - mis) % E, count);
-    const int nvec = (count - head) / E;
-    const int tail0 = head + nvec * E;
-    bool bad = false;
-    auto elem = [&](int i) -> T {
-        if (i < nsrc) {
-            const T v = src[i];
-            bad |= (v != v);
-            return v;
-        }
-        return (T)0;
-    };
-    for (int i = threadIdx.x; i < head; i += blockDim.x) dst[i] = elem(i);
-    for (int v = threadIdx.x; v < nvec; v += blockDim.x) {
-        const int i0 = head + v * E;
-        V w;
-        T* wp = reinterpret_cast<T*>(&w);
-#pragma unroll
-        for (int e = 0; e < E; ++e) wp[e] = elem(i0 + e);
-        *reinterpret_cast<V*>(dst + i0) = w;
-    }
-    for (int i = tail0 + threadIdx.x; i < count; i += blockDim.x) dst[i] = elem(i);
-    return bad;
+template <>
+struct Unit<16> {
+    using type = uint4;
+};
+template <typename T, int N>
+constexpr int unit_bytes() {
+    return (N * sizeof(T)) % 16 == 0 ? 16 : ((N * sizeof(T)) % 8 == 0 ? 8 : 4);
 }
 
-// Workgroup-wide copy of src[0, count) (global) into LDS dst; aligned body in
-// 16-byte loads.
-template <typename T>
-__device__ __forceinline__ void load_span(const T* src, int count, T* dst) {
-    using V = typename Vec16<T>::type;
-    constexpr int E = Vec16<T>::n;
-    const int mis = (int)((reinterpret_cast<uintptr_t>(src) / sizeof(T)) % E);
-    const int head = min((E - mis) % E, count);
-    const int nvec = (count - head) / E;
-    const int tail0 = head + nvec * E;
-    for (int i = threadIdx.x; i < head; i += blockDim.x) dst[i] = src[i];
-    for (int v = threadIdx.x; v < nvec; v += blockDim.x) {
-        const int i0 = head + v * E;
-        const V w = *reinterpret_cast<const V*>(src + i0);
-        const T* wp = reinterpret_cast<const T*>(&w);
+template <typename T, int N>
+__device__ __forceinline__ void lane_load(const T* p, int cnt, T (&v)[N]) {
+    constexpr int U = unit_bytes<T, N>();
+    using V = typename Unit<U>::type;
+    if (cnt >= N && (reinterpret_cast<uintptr_t>(p) % U) == 0) {
+        V w[N * sizeof(T) / U];
 #pragma unroll
-        for (int e = 0; e < E; ++e) dst[i0 + e] = wp[e];
+        for (int i = 0; i < (int)(N * sizeof(T) / U); ++i) w[i] = reinterpret_cast<const V*>(p)[i];
+        __builtin_memcpy(v, w, sizeof(v));
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = i < cnt ? p[i] : (T)0;
     }
-    for (int i = tail0 + threadIdx.x; i < count; i += blockDim.x) dst[i] = src[i];
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void lane_store(T* p, int cnt, const T (&v)[N]) {
+    constexpr int U = unit_bytes<T, N>();
+    using V = typename Unit<U>::type;
+    if (cnt >= N && (reinterpret_cast<uintptr_t>(p) % U) == 0) {
+        V w[N * sizeof(T) / U];
+        __builtin_memcpy(w, v, sizeof(v));
+#pragma unroll
+        for (int i = 0; i < (int)(N * sizeof(T) / U); ++i) reinterpret_cast<V*>(p)[i] = w[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (i < cnt) p[i] = v[i];
+    }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void lane_store_zero(T* p, int cnt) {
+    T z[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) z[i] = (T)0;
+    lane_store<T, N>(p, cnt, z);
+}
+
+// ------------------------------------------------------- FP32 output residual
+// With FP32 coordinates the forward also keeps, per coordinate, the rounding
+// residual of its FP64 value, r = x - (double)(float)x, as the half
+// h = RTZ_f16(r * 2^24).  The backward rebuilds x = (double)x32 + h * 2^-24,
+// within |x| * 2^-36 of the FP64 value (|x| < 2^16: h stays finite), so the
+// bond directions it takes from coordinate differences are FP64-grade; from
+// the FP32 coordinates alone they would carry ~1e-5 relative error.
+constexpr float kResScale = 0x1.0p24f, kResInv = 0x1.0p-24f;
+
+__device__ __forceinline__ uint32_t res_pack(double x0, float f0, double x1, float f1) {
+    const float r0 = (float)(x0 - (double)f0) * kResScale;
+    const float r1 = (float)(x1 - (double)f1) * kResScale;
+    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(r0, r1));
+}
+__device__ __forceinline__ double res_apply(float x32, _Float16 h) {
+    return fma((double)((float)h), (double)kResInv, (double)x32);
 }
 
 // Look-back payload per (chain, tile): aggregate and inclusive value.  The
-// forward uses 12 doubles each (frames), the backward 6 (wrenches).
+// forward uses 12 doubles each (frames), the backward 6 (sums).
 constexpr int kFwdPayload = 2 * kF;
 constexpr int kBwdPayload = 12;
 
 struct ScanArgs {
-    const void* angles;
+    const void* angles;    // forward input
+    const void* coords;    // backward input: the forward's output
     void* out;             // coords (forward) / grad_angles (backward)
     const void* grad;      // grad_coords (backward)
     const int32_t* lengths;
     uint32_t* status;
+    _Float16* res;         // [B][res_stride] residuals of FP32 coordinates, or null
+    int res_stride;        // halves per chain: 9L rounded up to a multiple of 8
     int B, L;
     int ntf, trf;          // forward tiles per chain, residues per forward tile
     int ntb, trb;          // backward tiles per chain, residues per backward tile
@@ -447,9 +457,9 @@ struct ScanArgs {
     uint32_t* fflags;      // [B][ntf]
     uint32_t* ftickets;    // [B]
     double* fpay;          // [B][ntf][kFwdPayload]
-    // forward -> backward state (ntb > 1)
-    double* ckpt;          // [B][ntb][12]: frame prefix at each backward tile start
-    uint32_t* bflags;      // [B][ntb]   zeroed by the forward, cleaned by the backward
+    // backward look-back (ntb > 1), in the state: zeroed by the forward,
+    // returned to zero by the backward
+    uint32_t* bflags;      // [B][ntb]
     uint32_t* btickets;    // [B]
     uint32_t* bdone;       // [B]
     double* bpay;          // [B][ntb][kBwdPayload]
@@ -492,126 +502,122 @@ __device__ __forceinline__ void lane_prefix(const double (*s_tot)[kF], int wave,
     if (tile_pfx) frame_lmul(tile_pfx, E);
 }
 
-// Pass 1 of both kernels: the lane composes its residues (local q0..q0+R-1)
-// from the identity and records each atom's position in that lane-local
-// frame: p[9*q + 3*k + c] for atom k (CA, C, N_next) of local residue q.
-// Atoms past the chain take masked identity steps; their slots get the
-// unchanged position and are never read.
-template <typename T, int R>
-__device__ __forceinline__ void lane_pass1(const T* s_ang, int TR, int r0, int n, int q0,
-                                           const double* s_tab, T* s_p, double f[kF]) {
-    frame_identity(f);
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-        const int r = r0 + q0 + q;
-        const bool v01 = r < n, v2 = r < n - 1;
-        T* p = s_p + 9 * (q0 + q);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            double sn, cs;
-            sincos_tab(static_cast<double>(s_ang[k * TR + q0 + q]), s_tab, &sn, &cs);
-            if (k == 0) frame_step_masked<0>(f, cs, sn, v01);
-            if (k == 1) frame_step_masked<1>(f, cs, sn, v01);
-            if (k == 2) frame_step_masked<2>(f, cs, sn, v2);
-            p[3 * k] = (T)f[9];
-            p[3 * k + 1] = (T)f[10];
-            p[3 * k + 2] = (T)f[11];
-        }
-    }
-}
-
-// Backward pass 1: as lane_pass1, and also the lane's moments of the
-// gradients in its local frame, M = sum_i p_i g_i^T (M[3d+c] = sum p_d g_c) and
-// F = sum_i g_i.  With the lane's global frame (R, t), its wrench total is
-// sum_i x_i x g_i = eps:(R M) + t x F (see wrench_total).
-template <typename T, int R>
-__device__ __forceinline__ void lane_pass1_moments(const T* s_ang, int TR, int r0, int n, int q0,
-                                                   const double* s_tab, T* s_p, const T* s_g,
-                                                   double f[kF], double M[9], double F[3]) {
-    frame_identity(f);
-#pragma unroll
-    for (int e = 0; e < 9; ++e) M[e] = 0.0;
-    F[0] = F[1] = F[2] = 0.0;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-        const int r = r0 + q0 + q;
-        const bool v01 = r < n, v2 = r < n - 1;
-        T* p = s_p + 9 * (q0 + q);
-        const T* g = s_g + 9 * (q0 + q);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const bool valid = (k < 2) ? v01 : v2;
-            double sn, cs;
-            sincos_tab(static_cast<double>(s_ang[k * TR + q0 + q]), s_tab, &sn, &cs);
-            if (k == 0) frame_step_masked<0>(f, cs, sn, valid);
-            if (k == 1) frame_step_masked<1>(f, cs, sn, valid);
-            if (k == 2) frame_step_masked<2>(f, cs, sn, valid);
-            const T p0 = (T)f[9], p1 = (T)f[10], p2 = (T)f[11];
-            p[3 * k] = p0;
-            p[3 * k + 1] = p1;
-            p[3 * k + 2] = p2;
-            // grad_coords of atoms past the chain are not staged: use 0.
-            const double pd[3] = {p0, p1, p2};
-            const double gc[3] = {valid ? (double)g[3 * k] : 0.0, valid ? (double)g[3 * k + 1] : 0.0,
-                                  valid ? (double)g[3 * k + 2] : 0.0};
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-#pragma unroll
-                for (int cc = 0; cc < 3; ++cc) M[3 * d + cc] = fma(pd[d], gc[cc], M[3 * d + cc]);
-            F[0] += gc[0]; F[1] += gc[1]; F[2] += gc[2];
-        }
-    }
-}
-
-// W = (sum_i x_i x g_i, sum_i g_i) of a lane from its moments and global frame
-// E = (R, t): sum (R p_i + t) x g_i = eps:(R M) + t x F.
-__device__ __forceinline__ void wrench_total(const double E[kF], const double M[9],
-                                             const double F[3], double W[6]) {
-    auto rm = [&](int b, int c) {  // (R M)_{bc} = sum_d R_{bd} M_{dc}, R_{bd} = E[3d+b]
-        return E[b] * M[c] + E[3 + b] * M[3 + c] + E[6 + b] * M[6 + c];
-    };
-    W[0] = rm(1, 2) - rm(2, 1) + (E[10] * F[2] - E[11] * F[1]);
-    W[1] = rm(2, 0) - rm(0, 2) + (E[11] * F[0] - E[9] * F[2]);
-    W[2] = rm(0, 1) - rm(1, 0) + (E[9] * F[1] - E[10] * F[0]);
-    W[3] = F[0];
-    W[4] = F[1];
-    W[5] = F[2];
-}
-
 __device__ __forceinline__ void affine(const double E[kF], double p0, double p1, double p2,
                                        double x[3]) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) x[i] = E[i] * p0 + E[3 + i] * p1 + E[6 + i] * p2 + E[9 + i];
 }
 
-// Dynamic LDS bytes (the sincos table and small arrays are static).
-template <typename T>
-__host__ __device__ constexpr size_t fwd_lds_bytes(int tr) {
-    return sizeof(T) * (size_t)tr * 12;  // angles [3][TR], positions [9*TR]
+__device__ __forceinline__ void cross(const double a[3], const double b[3], double c[3]) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
 }
-template <typename T>
-__host__ __device__ constexpr size_t bwd_lds_bytes(int tr) {
-    return sizeof(T) * (size_t)tr * 21;  // angles/grads [3][TR], grad_coords [9*TR], positions [9*TR]
+
+// Streams a lane's run of N elements of T to p: element i arrives by
+// put(i, v) in order 0 .. N-1.  VEC: whole run, U-byte aligned, U-byte
+// stores as soon as a unit is complete; else element stores below `cnt`.
+template <typename T, int N, bool VEC>
+struct RunWriter {
+    static constexpr int kE = unit_bytes<T, N>() / (int)sizeof(T);  // elements per unit
+    T* p;
+    int cnt;
+    T buf[kE];
+    __device__ __forceinline__ void put(int i, T v) {
+        if constexpr (VEC) {
+            buf[i % kE] = v;
+            if (i % kE == kE - 1) {
+                using V = typename Unit<unit_bytes<T, N>()>::type;
+                V w;
+                __builtin_memcpy(&w, buf, sizeof(w));
+                reinterpret_cast<V*>(p)[i / kE] = w;
+            }
+        } else {
+            if (i < cnt) p[i] = v;
+        }
+    }
+};
+
+// Pass 2 of the forward for one lane: atom 0 is the translation of E, atom
+// a >= 1 is E times local position a-1 (s_p); atoms at or past 3*cnt are 0.
+// Coordinates stream to x_l, FP32 residuals (when res_l) to res_l.  Returns
+// the sum of the coordinates (NaN if any is NaN).
+template <typename T, int R, bool VEC>
+__device__ __forceinline__ double fwd_pass2(const double E[kF], const T* s_p, int stride, int cnt,
+                                            int crow, T* x_l, _Float16* res_l) {
+    RunWriter<T, 9 * R, VEC> xw{x_l, 9 * crow, {}};
+    constexpr int KD = (9 * R + 1) / 2;
+    RunWriter<uint32_t, KD, VEC && (R % 2 == 0)> rw{reinterpret_cast<uint32_t*>(res_l), KD, {}};
+    [[maybe_unused]] double pend_x = 0.0;
+    [[maybe_unused]] float pend_f = 0.f;
+    double chk = 0.0;
+#pragma unroll
+    for (int at = 0; at < 3 * R; ++at) {
+        double x[3];
+        if (at == 0) {
+            x[0] = E[9];
+            x[1] = E[10];
+            x[2] = E[11];
+        } else {
+            const int m = 3 * (at - 1);
+            affine(E, (double)s_p[m * stride], (double)s_p[(m + 1) * stride],
+                   (double)s_p[(m + 2) * stride], x);
+        }
+        const bool ok = VEC || at < 3 * cnt;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int i = 3 * at + c;
+            const double v = ok ? x[c] : 0.0;
+            const T o = (T)v;
+            chk += v;
+            xw.put(i, o);
+            if constexpr (sizeof(T) == 4) {
+                if (i % 2 == 0) {
+                    pend_x = v;
+                    pend_f = o;
+                } else if (res_l) {
+                    const uint32_t w = res_pack(pend_x, pend_f, v, o);
+                    if constexpr (VEC && R % 2 == 0) {
+                        rw.put(i / 2, w);
+                    } else {
+                        if (i - 1 < 9 * crow) res_l[i - 1] = __builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+                        if (i < 9 * crow) res_l[i] = __builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
+                    }
+                }
+            }
+        }
+    }
+    if constexpr (sizeof(T) == 4 && (9 * R) % 2 == 1) {
+        if (res_l && 9 * R - 1 < 9 * crow) {
+            const uint32_t w = res_pack(pend_x, pend_f, pend_x, pend_f);
+            res_l[9 * R - 1] = __builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+        }
+    }
+    return chk;
 }
 
 // ------------------------------------------------------------ forward scan
-// One tile = TR = 64*W*R residues of chain b.  Steps:
-//   1. stage the tile's angles in LDS (coalesced row loads);
-//   2. pass 1: per lane, FP64 sincos and the serial product of its R residues,
-//      atom positions in the lane's local frame into LDS; waves whose first
-//      residue is past the chain skip passes 1-2;
+// One tile = TR = 64*W*R residues of chain b; lane l owns residues
+// rl = r0 + l*R .. rl+R-1 and writes their 9R output floats (atoms N_rl,
+// CA_rl, C_rl, ..., C_{rl+R-1}).  Steps:
+//   1. lanes wholly past the chain write their zeros at once; the others load
+//      their R angles per row straight into registers;
+//   2. pass 1: per lane, FP64 sincos and the serial product of its R residues;
+//      atom positions in the lane's local frame stay in registers; waves
+//      whose first residue is past the chain skip passes 1-2;
 //   3. DPP scan over the wave, LDS over the waves, look-back over tiles;
-//   4. pass 2: x = E p with E the lane's global prefix frame, in place in LDS;
-//      the prefix frames at backward-tile starts go to the checkpoints;
-//   5. coalesced store of the tile's coordinates, zeros past the chain.
+//   4. pass 2: N_rl is the translation of the lane's global prefix frame E,
+//      the other atoms x = E p; zeros past the chain; FP32 outputs also write
+//      their residuals.
 template <typename T, int R, int MAXT>
 __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nw = blockDim.x >> 6;
     const int TR = blockDim.x * R;
-    T* s_ang = reinterpret_cast<T*>(s_dyn);
-    T* s_xyz = s_ang + 3 * TR;
+    // Local positions of the lane's atoms 1 .. 3R-1, coordinate m of the lane
+    // at s_p[m * blockDim.x + tid] (conflict-free; read back by the same lane).
+    T* s_p = reinterpret_cast<T*>(s_dyn);
     __shared__ double s_tab[128];
     __shared__ double s_tot[4][kF];
     __shared__ double s_pfx[kF];
@@ -623,17 +629,19 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
     for (int i = tid; i < 128; i += blockDim.x) s_tab[i] = kTabCS[i];
 
     for (int item = blockIdx.x; item < a.B * a.ntf; item += gridDim.x) {
-        __syncthreads();  // LDS of the previous tile is no longer read
+        __syncthreads();  // s_tab written; LDS of the previous tile no longer read
         int b, t;
         item_tile(item, a.B, a.ntf, a.ftickets, false, &s_tile, &b, &t);
         const int n = clamp_length(a, b, t == 0 && tid == 0);
         const int r0 = t * TR;
-        const int nvalid = min(max(n - r0, 0), TR);
-        T* x_b = coords + (size_t)b * 9 * L;
+        const int rl = r0 + tid * R;
+        const int cnt = min(max(n - rl, 0), R);   // residues of the chain in this lane
+        const int crow = min(max(L - rl, 0), R);  // residues of the row in this lane
+        T* x_l = coords + (size_t)b * 9 * L + (size_t)9 * rl;
 
-        // Initialise the backward's look-back words of the backward tiles
-        // inside this forward tile (the backward cleans them after use).
-        if (a.ntb > 1) {
+        // Zero the backward's look-back words of the backward tiles inside
+        // this forward tile (the backward returns them to zero after use).
+        if (a.bflags) {
             const int per = TR / a.trb;
             for (int j = tid; j < per; j += blockDim.x) {
                 const int tb = t * per + j;
@@ -644,135 +652,174 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
                 word_store(a.bdone + b, 0u);
             }
         }
+        if (cnt == 0 && crow > 0) lane_store_zero<T, 9 * R>(x_l, 9 * crow);
 
-        if (nvalid > 0) {  // uniform: the tile holds residues of the chain
-            for (int k = 0; k < 3; ++k)
-                load_span(angles + ((size_t)b * 3 + k) * L + r0, nvalid, s_ang + k * TR);
-            __syncthreads();
-            const int q0 = tid * R;
-            const bool wave_active = wave * 64 * R < nvalid;  // uniform per wave
-            double E[kF];
-            if (wave_active) {
-                lane_pass1<T, R>(s_ang, TR, r0, n, q0, s_tab, s_xyz, E);
-                wave_scan_frames(E);
-                if (lane == 63) {
+        const bool wave_active = r0 + wave * 64 * R < n;  // uniform per wave
+        double E[kF];
+        if (wave_active) {
+            T ang[3][R];
 #pragma unroll
-                    for (int e = 0; e < kF; ++e) s_tot[wave][e] = E[e];
-                }
-                wave_shift_frames(E);
-            } else if (lane == 63) {
+            for (int k = 0; k < 3; ++k) lane_load<T, R>(angles + ((size_t)b * 3 + k) * L + rl, cnt, ang[k]);
+            frame_identity(E);
 #pragma unroll
-                for (int e = 0; e < kF; ++e) s_tot[wave][e] = ident(e);
-            }
-            __syncthreads();
-            if (a.ntf > 1) {
-                if (tid == 0) {
-                    const size_t s0 = (size_t)b * a.ntf;
-                    uint32_t* fl = a.fflags + s0;
-                    double* pay = a.fpay + s0 * kFwdPayload;
-                    double* my = pay + (size_t)t * kFwdPayload;
-                    double acc[kF];
-                    {   // tile aggregate = product of the wave aggregates
+            for (int q = 0; q < R; ++q) {
 #pragma unroll
-                        for (int e = 0; e < kF; ++e) acc[e] = s_tot[nw - 1][e];
-                        for (int w = nw - 2; w >= 0; --w) frame_lmul(s_tot[w], acc);
-#pragma unroll
-                        for (int e = 0; e < kF; ++e) {
-                            s_agg[e] = acc[e];
-                            payload_store(my + (t == 0 ? kF : 0) + e, acc[e]);
-                        }
+                for (int k = 0; k < 3; ++k) {
+                    double sn, cs;
+                    sincos_tab((double)ang[k][q], s_tab, &sn, &cs);
+                    if (k == 0) frame_step<0>(E, cs, sn);
+                    if (k == 1) frame_step<1>(E, cs, sn);
+                    if (k == 2) frame_step<2>(E, cs, sn);
+                    if (3 * q + k < 3 * R - 1) {
+                        const int m = 3 * (3 * q + k);
+                        s_p[m * blockDim.x + tid] = (T)E[9];
+                        s_p[(m + 1) * blockDim.x + tid] = (T)E[10];
+                        s_p[(m + 2) * blockDim.x + tid] = (T)E[11];
                     }
-                    flag_store(fl + t, t == 0 ? kFlagInclusive : kFlagAggregate);
-                    frame_identity(acc);  // becomes this tile's exclusive prefix
-                    if (t > 0) {
-                        for (int j = t - 1; j >= 0; --j) {
-                            const uint32_t fj = wait_flag(fl + j, a.status);
-                            if (fj == 0) continue;
-                            const double* pj = pay + (size_t)j * kFwdPayload;
-                            const bool incl = (fj == kFlagInclusive);
-                            double v[kF];
-#pragma unroll
-                            for (int e = 0; e < kF; ++e) v[e] = payload_load(pj + (incl ? kF : 0) + e);
-                            frame_lmul(v, acc);
-                            if (incl) break;
-                        }
-                        double incl_v[kF];  // inclusive = prefix * aggregate
-                        frame_rmul_to(acc, s_agg, incl_v);
-#pragma unroll
-                        for (int e = 0; e < kF; ++e) payload_store(my + kF + e, incl_v[e]);
-                        flag_store(fl + t, kFlagInclusive);
-                    }
-#pragma unroll
-                    for (int e = 0; e < kF; ++e) s_pfx[e] = acc[e];
                 }
-                __syncthreads();
             }
-            if (wave_active) {
-                lane_prefix(s_tot, wave, a.ntf > 1 ? s_pfx : nullptr, E);
-                const int rl = r0 + q0;  // first residue of this lane
-                if (a.ntb > 1 && rl < n && rl % a.trb == 0) {
-                    double* ck = a.ckpt + ((size_t)b * a.ntb + rl / a.trb) * kF;
+            wave_scan_frames(E);
+            if (lane == 63) {
 #pragma unroll
-                    for (int e = 0; e < kF; ++e) ck[e] = E[e];
-                }
-                // Pass 2: local positions -> global, in place.  Residue r (local
-                // j) holds CA_r, C_r, N_{r+1} = global floats 9r+3 .. 9r+11; slots
-                // past the chain are transformed too and never read.
+                for (int e = 0; e < kF; ++e) s_tot[wave][e] = E[e];
+            }
+            wave_shift_frames(E);
+        } else if (lane == 63) {
 #pragma unroll
-                for (int a3 = 0; a3 < 3 * R; ++a3) {
-                    T* p = s_xyz + 9 * q0 + 3 * a3;
-                    double x[3];
-                    affine(E, p[0], p[1], p[2], x);
-                    p[0] = (T)x[0];
-                    p[1] = (T)x[1];
-                    p[2] = (T)x[2];
+            for (int e = 0; e < kF; ++e) s_tot[wave][e] = ident(e);
+        }
+        __syncthreads();
+        if (a.ntf > 1) {
+            if (tid == 0) {
+                const size_t s0 = (size_t)b * a.ntf;
+                uint32_t* fl = a.fflags + s0;
+                double* pay = a.fpay + s0 * kFwdPayload;
+                double* my = pay + (size_t)t * kFwdPayload;
+                double acc[kF];
+                {   // tile aggregate = product of the wave aggregates
+#pragma unroll
+                    for (int e = 0; e < kF; ++e) acc[e] = s_tot[nw - 1][e];
+                    for (int w = nw - 2; w >= 0; --w) frame_lmul(s_tot[w], acc);
+#pragma unroll
+                    for (int e = 0; e < kF; ++e) {
+                        s_agg[e] = acc[e];
+                        payload_store(my + (t == 0 ? kF : 0) + e, acc[e]);
+                    }
                 }
+                flag_store(fl + t, t == 0 ? kFlagInclusive : kFlagAggregate);
+                frame_identity(acc);  // becomes this tile's exclusive prefix
+                if (t > 0) {
+                    for (int j = t - 1; j >= 0; --j) {
+                        const uint32_t fj = wait_flag(fl + j, a.status);
+                        if (fj == 0) continue;
+                        const double* pj = pay + (size_t)j * kFwdPayload;
+                        const bool incl = (fj == kFlagInclusive);
+                        double v[kF];
+#pragma unroll
+                        for (int e = 0; e < kF; ++e) v[e] = payload_load(pj + (incl ? kF : 0) + e);
+                        frame_lmul(v, acc);
+                        if (incl) break;
+                    }
+                    double incl_v[kF];  // inclusive = prefix * aggregate
+                    frame_rmul_to(acc, s_agg, incl_v);
+#pragma unroll
+                    for (int e = 0; e < kF; ++e) payload_store(my + kF + e, incl_v[e]);
+                    flag_store(fl + t, kFlagInclusive);
+                }
+#pragma unroll
+                for (int e = 0; e < kF; ++e) s_pfx[e] = acc[e];
             }
             __syncthreads();
         }
-
-        // Store global floats [9*r0+3, 9*r0+3+9*TR) within [0, 9L): atoms < 3n
-        // from LDS, the rest 0.  Tile 0 also writes N_0 = 0.
-        if (t == 0 && tid < 3) x_b[tid] = (T)0;
-        const int g0 = 9 * r0 + 3;
-        const int cnt = min(9 * TR, 9 * L - g0);
-        const int nsrc = min(max(9 * n - g0, 0), cnt);
-        if (store_span(x_b + g0, cnt, s_xyz, nsrc)) set_status(a.status, TPL_A2B_STATUS_NAN);
+        if (wave_active && cnt > 0) {
+            lane_prefix(s_tot, wave, a.ntf > 1 ? s_pfx : nullptr, E);
+            // Pass 2: the lane's 9R coordinates (chain floats 9rl ..), zeros
+            // past the chain; 16-byte stores unless the lane ends the chain or
+            // the row is misaligned.
+            _Float16* res_l = a.res ? a.res + (size_t)b * a.res_stride + (size_t)9 * rl : nullptr;
+            constexpr int U = unit_bytes<T, 9 * R>();
+            const bool vec = cnt == R && (reinterpret_cast<uintptr_t>(x_l) % U) == 0;
+            double chk;
+            if (vec)
+                chk = fwd_pass2<T, R, true>(E, s_p + tid, blockDim.x, cnt, crow, x_l, res_l);
+            else
+                chk = fwd_pass2<T, R, false>(E, s_p + tid, blockDim.x, cnt, crow, x_l, res_l);
+            const bool bad = chk != chk;
+            if (bad) set_status(a.status, TPL_A2B_STATUS_NAN);
+        }
     }
 }
 
+// The lane's 9R coordinates as loaded: values (xr) and, for FP32, their
+// residual halves (hr, from res + ro; zero past the chain).
+template <typename T, int R>
+struct LaneCoords {
+    T xr[9 * R];
+    _Float16 hr[sizeof(T) == 4 ? 9 * R : 1];
+
+    __device__ __forceinline__ void load(const T* x, const _Float16* res, int cnt) {
+        lane_load<T, 9 * R>(x, 9 * cnt, xr);
+        if constexpr (sizeof(T) == 4) {
+            constexpr int KD = (9 * R + 1) / 2;
+            bool done = false;
+            if constexpr (R % 2 == 0) {
+                if (cnt == R) {  // 9*rl halves: 4-byte aligned (rl even)
+                    uint32_t hw[KD];
+                    lane_load<uint32_t, KD>(reinterpret_cast<const uint32_t*>(res), KD, hw);
+                    __builtin_memcpy(hr, hw, sizeof(hr));
+                    done = true;
+                }
+            }
+            if (!done) {
+#pragma unroll
+                for (int i = 0; i < 9 * R; ++i) hr[i] = i < 9 * cnt ? res[i] : (_Float16)0;
+            }
+        }
+    }
+    // FP64 coordinates of the lane's atom `at` (0 .. 3R-1).
+    __device__ __forceinline__ void atom(int at, double out[3]) const {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if constexpr (sizeof(T) == 4)
+                out[c] = res_apply(xr[3 * at + c], hr[3 * at + c]);
+            else
+                out[c] = (double)xr[3 * at + c];
+        }
+    }
+};
+
 // ----------------------------------------------------------- backward scan
-// Backward tiles of TR = 64*W*R residues, taken from the last tile of each
-// chain to the first.  Steps:
-//   1. stage angles and the tile's grad_coords in LDS (coalesced);
-//   2. pass 1 as in the forward (local positions p), DPP/LDS scan; the tile
-//      prefix is the forward's checkpoint;
-//   3. the lane's wrench sum W = (sum x x g, sum g) from moments gathered in
-//      pass 1: sum (R p + t) x g = eps:(R sum p g^T) + t x sum g;
-//   4. suffix sums of wrenches over lanes (DPP), waves (LDS) and tiles
-//      (reverse look-back);
-//   5. pass 3: per atom m, S_m = sum over atoms after m, u_m = E.R (p_m -
-//      p_{m-1}) / d_m, gradient u_m . S_T + (x_m x u_m) . S_F, into LDS;
-//   6. coalesced store of the tile's gradients, zeros past the chain.
+// The adjoint from the forward's coordinates.  Chain atom j >= 1 (atoms
+// N_0, CA_0, C_0, N_1, ...) is placed by the transform whose angle theta_j is
+// phi_r, psi_r, omega_r for j = 3r+1, 3r+2, 3r+3.  theta_j turns every later
+// atom about the bond axis u_j = (x_j - x_{j-1}) / d_j through x_j, so
+//   dL/dtheta_j = u_j . sum_{i>j} (x_i - x_j) x g_i = u_j . (T_j - x_j x F_j),
+//   T_j = sum_{i>j} x_i x g_i,  F_j = sum_{i>j} g_i,
+// a suffix sum of the 6-vectors (x_i x g_i, g_i).  Tiles of TR = 64*W*R
+// residues are taken from the last tile of each chain to the first; lane l
+// sums its own atoms 3rl .. 3rl+3R-1 and writes the gradients of its residues
+// (atoms 3rl+1 .. 3rl+3R, the last one being the next lane's first).  Steps:
+//   1. lanes wholly past the chain write their zero gradients at once; the
+//      others load their coordinates (FP64, with the residuals for FP32) and
+//      coordinate gradients into registers, zero past the chain;
+//   2. per lane, W = sum over its atoms of (x x g, g);
+//   3. suffix sums of W over lanes (DPP), waves (LDS) and tiles (reverse
+//      look-back) give C = the sum over all later atoms;
+//   4. per atom, from the lane's last down: the suffix (T_j, F_j) and the
+//      gradient; psi and omega of the last residue are 0.
 template <typename T, int R, int MAXT>
 __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
-    extern __shared__ __align__(16) unsigned char s_dyn[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nw = blockDim.x >> 6;
     const int TR = blockDim.x * R;
-    T* s_ang = reinterpret_cast<T*>(s_dyn);  // angles, then gradients
-    T* s_g = s_ang + 3 * TR;
-    T* s_p = s_g + 9 * TR;
-    __shared__ double s_tab[128];
-    __shared__ double s_tot[4][kF];
     __shared__ double s_wsum[4][6];
     __shared__ double s_carry[6];
     __shared__ int s_tile;
-    const T* angles = static_cast<const T*>(a.angles);
-    const T* gcoords = static_cast<const T*>(a.grad);
-    T* gangles = static_cast<T*>(a.out);
+    const T* xin = static_cast<const T*>(a.coords);
+    const T* gin = static_cast<const T*>(a.grad);
+    T* gang = static_cast<T*>(a.out);
     const int L = a.L;
-    for (int i = tid; i < 128; i += blockDim.x) s_tab[i] = kTabCS[i];
+    constexpr double kInvD[3] = {1.0 / 1.460, 1.0 / 1.525, 1.0 / 1.330};
 
     for (int item = blockIdx.x; item < a.B * a.ntb; item += gridDim.x) {
         __syncthreads();
@@ -780,67 +827,66 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
         item_tile(item, a.B, a.ntb, a.btickets, true, &s_tile, &b, &t);
         const int n = clamp_length(a, b, t == 0 && tid == 0);
         const int r0 = t * TR;
-        const int nvalid = min(max(n - r0, 0), TR);
-        T* ga_b = gangles + (size_t)b * 3 * L;
+        const int rl = r0 + tid * R;
+        const int cnt = min(max(n - rl, 0), R);
+        const int crow = min(max(L - rl, 0), R);
+        T* ga_l = gang + (size_t)b * 3 * L + rl;
+        if (cnt == 0 && crow > 0) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) lane_store_zero<T, R>(ga_l + (size_t)k * L, crow);
+        }
+        const bool wave_active = r0 + wave * 64 * R < n;  // uniform per wave
 
-        if (nvalid > 0) {  // uniform
-            for (int k = 0; k < 3; ++k)
-                load_span(angles + ((size_t)b * 3 + k) * L + r0, nvalid, s_ang + k * TR);
-            {
-                const int g0 = 9 * r0 + 3;
-                load_span(gcoords + (size_t)b * 9 * L + g0, min(9 * TR, 9 * n - g0), s_g);
-            }
-            __syncthreads();
-            const int q0 = tid * R;
-            const bool wave_active = wave * 64 * R < nvalid;
-            double E[kF];
-            double M[9], F[3];
-            if (wave_active) {
-                lane_pass1_moments<T, R>(s_ang, TR, r0, n, q0, s_tab, s_p, s_g, E, M, F);
-                wave_scan_frames(E);
-                if (lane == 63) {
+        // Pass A: W = sum over the lane's atoms 0 .. 3R-1 (chain atoms
+        // 3rl .. 3rl+3R-1) of (x x g, g); zero past the chain.
+        const size_t o = (size_t)b * 9 * L + (size_t)9 * rl;
+        const size_t ro = (size_t)b * a.res_stride + (size_t)9 * rl;
+        double W[6] = {0, 0, 0, 0, 0, 0};
+        if (wave_active) {
+            LaneCoords<T, R> xc;
+            T gr[9 * R];
+            xc.load(xin + o, a.res ? a.res + ro : nullptr, cnt);
+            lane_load<T, 9 * R>(gin + o, 9 * cnt, gr);
 #pragma unroll
-                    for (int e = 0; e < kF; ++e) s_tot[wave][e] = E[e];
+            for (int at = 0; at < 3 * R; ++at) {
+                double x[3], xg[3];
+                xc.atom(at, x);
+                const double g[3] = {(double)gr[3 * at], (double)gr[3 * at + 1], (double)gr[3 * at + 2]};
+                cross(x, g, xg);
+                W[0] += xg[0]; W[1] += xg[1]; W[2] += xg[2];
+                W[3] += g[0]; W[4] += g[1]; W[5] += g[2];
+            }
+        }
+        // C = sum over later lanes, waves and tiles.
+        double C[6];
+#pragma unroll
+        for (int e = 0; e < 6; ++e) C[e] = W[e];
+        if (wave_active) {
+            wave_scan_sum<6>(C);  // inclusive prefix over lanes
+#pragma unroll
+            for (int e = 0; e < 6; ++e) {
+                const double tot = read_lane(C[e], 63);
+                if (lane == 0) s_wsum[wave][e] = tot;
+                C[e] = tot - C[e];
+            }
+        } else if (lane == 0) {
+#pragma unroll
+            for (int e = 0; e < 6; ++e) s_wsum[wave][e] = 0.0;
+        }
+        __syncthreads();
+        for (int w = wave + 1; w < nw; ++w) {
+#pragma unroll
+            for (int e = 0; e < 6; ++e) C[e] += s_wsum[w][e];
+        }
+        if (a.ntb > 1) {  // then bflags, bpay, btickets, bdone are set (state)
+            if (tid == 0) {
+                double tot[6] = {0, 0, 0, 0, 0, 0}, carry[6] = {0, 0, 0, 0, 0, 0};
+                for (int w = 0; w < nw; ++w) {
+#pragma unroll
+                    for (int e = 0; e < 6; ++e) tot[e] += s_wsum[w][e];
                 }
-                wave_shift_frames(E);
-            }
-            __syncthreads();
-            double W[6] = {0, 0, 0, 0, 0, 0};  // (sum x x g, sum g) of this lane
-            if (wave_active) {
-                lane_prefix(s_tot, wave,
-                            (a.ntb > 1 && t > 0) ? a.ckpt + ((size_t)b * a.ntb + t) * kF : nullptr,
-                            E);
-                wrench_total(E, M, F, W);
-            }
-            // Suffix sums of wrenches: lanes, waves and tiles after this one.
-            double C[6];
-#pragma unroll
-            for (int e = 0; e < 6; ++e) C[e] = W[e];
-            if (wave_active) {
-                wave_scan_sum<6>(C);  // inclusive prefix over lanes
-#pragma unroll
-                for (int e = 0; e < 6; ++e) {
-                    const double tot = read_lane(C[e], 63);
-                    if (lane == 0) s_wsum[wave][e] = tot;
-                    C[e] = tot - C[e];  // sum over later lanes
-                }
-            } else if (lane == 0) {
-#pragma unroll
-                for (int e = 0; e < 6; ++e) s_wsum[wave][e] = 0.0;
-            }
-            __syncthreads();
-            for (int w = wave + 1; w < nw; ++w) {
-#pragma unroll
-                for (int e = 0; e < 6; ++e) C[e] += s_wsum[w][e];
-            }
-            if (a.ntb > 1) {
-                if (tid == 0) {
-                    double tot[6] = {0, 0, 0, 0, 0, 0}, carry[6] = {0, 0, 0, 0, 0, 0};
-                    for (int w = 0; w < nw; ++w) {
-#pragma unroll
-                        for (int e = 0; e < 6; ++e) tot[e] += s_wsum[w][e];
-                    }
-                    const int ntv = (n + TR - 1) / TR;  // tiles holding residues
+                const int ntv = (n + TR - 1) / TR;  // tiles holding residues
+                if (t < ntv) {
                     const size_t s0 = (size_t)b * a.ntb;
                     uint32_t* fl = a.bflags + s0;
                     double* pay = a.bpay + s0 * kBwdPayload;
@@ -866,73 +912,85 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
                         for (int e = 0; e < 6; ++e) payload_store(my + 6 + e, carry[e] + tot[e]);
                         flag_store(fl + t, kFlagInclusive);
                     }
-#pragma unroll
-                    for (int e = 0; e < 6; ++e) s_carry[e] = carry[e];
                 }
-                __syncthreads();
 #pragma unroll
-                for (int e = 0; e < 6; ++e) C[e] += s_carry[e];
-            }
-            if (wave_active) {
-                // Pass 3: S_m = sum over atoms after m; gradients into LDS
-                // (over the staged angles, which are no longer read).  Atoms
-                // past the chain run masked (g = 0) and their results are
-                // dropped.
-                double S[6];
-#pragma unroll
-                for (int e = 0; e < 6; ++e) S[e] = C[e] + W[e];
-                double pp0 = 0.0, pp1 = 0.0, pp2 = 0.0;  // previous atom (the lane's N)
-#pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    const int r = r0 + q0 + q;
-                    const bool v01 = r < n, v2 = r < n - 1;
-                    const T* p = s_p + 9 * (q0 + q);
-                    const T* g = s_g + 9 * (q0 + q);
-                    double gr[3];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        constexpr double kInvD[3] = {1.0 / 1.460, 1.0 / 1.525, 1.0 / 1.330};
-                        const bool valid = (k < 2) ? v01 : v2;
-                        const double p0 = p[3 * k], p1 = p[3 * k + 1], p2 = p[3 * k + 2];
-                        double x[3];
-                        affine(E, p0, p1, p2, x);
-                        const double d0 = (p0 - pp0) * kInvD[k];
-                        const double d1 = (p1 - pp1) * kInvD[k];
-                        const double d2 = (p2 - pp2) * kInvD[k];
-                        pp0 = p0; pp1 = p1; pp2 = p2;
-                        double u[3];
-#pragma unroll
-                        for (int i = 0; i < 3; ++i) u[i] = E[i] * d0 + E[3 + i] * d1 + E[6 + i] * d2;
-                        const double g0 = valid ? (double)g[3 * k] : 0.0;
-                        const double g1 = valid ? (double)g[3 * k + 1] : 0.0;
-                        const double g2 = valid ? (double)g[3 * k + 2] : 0.0;
-                        S[0] -= x[1] * g2 - x[2] * g1;
-                        S[1] -= x[2] * g0 - x[0] * g2;
-                        S[2] -= x[0] * g1 - x[1] * g0;
-                        S[3] -= g0; S[4] -= g1; S[5] -= g2;
-                        const double w0 = x[1] * u[2] - x[2] * u[1];
-                        const double w1 = x[2] * u[0] - x[0] * u[2];
-                        const double w2 = x[0] * u[1] - x[1] * u[0];
-                        const double gk = u[0] * S[0] + u[1] * S[1] + u[2] * S[2] + w0 * S[3] +
-                                          w1 * S[4] + w2 * S[5];
-                        gr[k] = valid ? gk : 0.0;
-                    }
-                    if (r == n - 1) gr[1] = 0.0;  // psi of the last residue moves no atom
-                    // Slots past the chain (r >= n) are written too and never read.
-                    s_ang[q0 + q] = (T)gr[0];
-                    s_ang[TR + q0 + q] = (T)gr[1];
-                    s_ang[2 * TR + q0 + q] = (T)gr[2];
-                }
+                for (int e = 0; e < 6; ++e) s_carry[e] = carry[e];
             }
             __syncthreads();
+#pragma unroll
+            for (int e = 0; e < 6; ++e) C[e] += s_carry[e];
         }
-
-        // Store the tile's gradients, zeros past the chain.
-        bool bad = false;
-        const int cnt = min(TR, L - r0);
-        for (int k = 0; k < 3; ++k)
-            bad |= store_span(ga_b + (size_t)k * L + r0, cnt, s_ang + k * TR, nvalid);
-        if (bad) set_status(a.status, TPL_A2B_STATUS_NAN);
+        if (wave_active && cnt > 0) {
+            // Pass B: the lane's atoms again (from L2), plus chain atom
+            // 3(rl+R), the next lane's first.
+            LaneCoords<T, R> xc;
+            T gr[9 * R];
+            xc.load(xin + o, a.res ? a.res + ro : nullptr, cnt);
+            lane_load<T, 9 * R>(gin + o, 9 * cnt, gr);
+            const bool has_next = rl + R < n;
+            double xn[3], gn[3];  // chain atom 3(rl+R)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T xv = has_next ? xin[o + 9 * R + c] : (T)0;
+                gn[c] = has_next ? (double)gin[o + 9 * R + c] : 0.0;
+                if constexpr (sizeof(T) == 4)
+                    xn[c] = has_next ? res_apply((float)xv, a.res[ro + 9 * R + c]) : 0.0;
+                else
+                    xn[c] = (double)xv;
+            }
+            // From the lane's atom 3R down: Xc = atom j, Xp = atom j-1, Xa =
+            // atom j+1, S = sum over atoms after j of (x x g, g).  Atom 3R
+            // belongs to the next lane, whose sum is inside C.
+            double S[6], Xc[3], Xp[3], Xa[3];
+            {
+                double xg[3];
+                cross(xn, gn, xg);
+                S[0] = C[0] - xg[0]; S[1] = C[1] - xg[1]; S[2] = C[2] - xg[2];
+                S[3] = C[3] - gn[0]; S[4] = C[4] - gn[1]; S[5] = C[5] - gn[2];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) Xc[c] = xn[c];
+            xc.atom(3 * R - 1, Xp);
+            T out[3][R];
+#pragma unroll
+            for (int j = 3 * R; j >= 1; --j) {
+                if (j < 3 * R) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        Xa[c] = Xc[c];
+                        Xc[c] = Xp[c];
+                    }
+                    xc.atom(j - 1, Xp);
+                }
+                if (j == 3 * R - 1) {
+#pragma unroll
+                    for (int e = 0; e < 6; ++e) S[e] = C[e];
+                } else if (j < 3 * R - 1) {
+                    const double g[3] = {(double)gr[3 * j + 3], (double)gr[3 * j + 4], (double)gr[3 * j + 5]};
+                    double xg[3];
+                    cross(Xa, g, xg);
+                    S[0] += xg[0]; S[1] += xg[1]; S[2] += xg[2];
+                    S[3] += g[0]; S[4] += g[1]; S[5] += g[2];
+                }
+                const int k = (j - 1) % 3, q = (j - 1) / 3;
+                double u[3], xf[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) u[c] = (Xc[c] - Xp[c]) * kInvD[k];
+                cross(Xc, S + 3, xf);
+                const double v = u[0] * (S[0] - xf[0]) + u[1] * (S[1] - xf[1]) + u[2] * (S[2] - xf[2]);
+                const int r = rl + q;
+                const bool zero = (r >= n) || (r == n - 1 && k > 0);
+                out[k][q] = zero ? (T)0 : (T)v;
+            }
+            bool bad = false;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+#pragma unroll
+                for (int q = 0; q < R; ++q) bad |= (out[k][q] != out[k][q]);
+                lane_store<T, R>(ga_l + (size_t)k * L, crow, out[k]);
+            }
+            if (bad) set_status(a.status, TPL_A2B_STATUS_NAN);
+        }
 
         // Last tile of the chain to finish returns its look-back words to 0.
         if (a.ntb > 1 && tid == 0) {
@@ -1098,18 +1156,18 @@ bool valid_config(const tpl_a2b_config* c) {
     return wok && rok && c->max_wgs >= 0 && c->flags == 0;
 }
 
-// Largest forward / backward tiles whose LDS staging (fwd_lds_bytes,
-// bwd_lds_bytes) stays within 48 KiB, so three workgroups fit one CU.
-constexpr int kMaxFwdTileF32 = 1024, kMaxFwdTileF64 = 512;
-constexpr int kMaxBwdTileF32 = 512, kMaxBwdTileF64 = 256;
+// Largest tiles, and the backward's residues per lane: bounds that keep
+// each lane's runs in registers (no spills) at the shapes the tuner tries.
+constexpr int kMaxTileF32 = 1024, kMaxTileF64 = 512;
+constexpr int kMaxBwdRF32 = 2, kMaxBwdRF64 = 1;
 
 bool fits_dtype(int dtype, const tpl_a2b_config* c) {
     const int tr = 64 * c->waves * c->res_per_lane;
-    return tr <= (dtype == TPL_A2B_F32 ? kMaxFwdTileF32 : kMaxFwdTileF64);
+    return tr <= (dtype == TPL_A2B_F32 ? kMaxTileF32 : kMaxTileF64);
 }
 
-// Tiling of one call: forward tiles of trf residues, backward tiles of trb
-// (= trf, or a divisor of it when the backward's LDS needs smaller tiles).
+// Tiling of one call: forward tiles of trf = 64*W*R residues; backward tiles
+// of trb = 64*W*rb with rb = min(R, the backward's bound), a divisor of trf.
 struct Tiling {
     int trf, ntf, trb, ntb, rb;
 };
@@ -1117,14 +1175,18 @@ struct Tiling {
 Tiling tiling(int dtype, int L, const tpl_a2b_config* c) {
     Tiling t{};
     t.trf = 64 * c->waves * c->res_per_lane;
-    t.trb = std::min(t.trf, dtype == TPL_A2B_F32 ? kMaxBwdTileF32 : kMaxBwdTileF64);
-    t.rb = t.trb / (64 * c->waves);
+    t.rb = std::min(c->res_per_lane, dtype == TPL_A2B_F32 ? kMaxBwdRF32 : kMaxBwdRF64);
+    t.trb = 64 * c->waves * t.rb;
     t.ntf = L <= 0 ? 0 : (L + t.trf - 1) / t.trf;
     t.ntb = L <= 0 ? 0 : (L + t.trb - 1) / t.trb;
     return t;
 }
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// state and sync regions start on 256-byte boundaries (the kernels move the
+// residual runs in aligned 4- to 16-byte units).
+bool misaligned(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 255) != 0; }
 
 // Per-call scratch of the forward's own look-back (ntf > 1): flags [B][ntf]
 // and tickets [B] (zeroed by a2b_zero_words before the forward), payload.
@@ -1143,17 +1205,21 @@ SyncLayout sync_layout(int B, int ntf) {
     return s;
 }
 
-// Forward -> backward state (ntb > 1): checkpoints, backward look-back payload,
-// flags, tickets and done counters.
+// Forward -> backward state: FP32 coordinate residuals [B][res_stride] halves
+// (FP32 only), then the backward's look-back words and payload (ntb > 1).
+int res_stride(int L) { return (9 * L + 7) & ~7; }
+
 struct StateLayout {
-    size_t ckpt_off, pay_off, flags_off, tickets_off, done_off, total;
+    size_t res_off, pay_off, flags_off, tickets_off, done_off, total;
 };
-StateLayout state_layout(int B, int ntb) {
+StateLayout state_layout(int dtype, int B, int L, int ntb) {
     StateLayout s{};
+    s.res_off = 0;
+    const size_t res = dtype == TPL_A2B_F32 ? (size_t)B * res_stride(L) * 2 : 0;
+    s.total = align256(res);
     if (ntb <= 1) return s;
     const size_t tiles = (size_t)B * ntb;
-    s.ckpt_off = 0;
-    s.pay_off = tiles * kF * sizeof(double);
+    s.pay_off = s.total;
     s.flags_off = s.pay_off + tiles * kBwdPayload * sizeof(double);
     s.tickets_off = s.flags_off + tiles * 4;
     s.done_off = s.tickets_off + (size_t)B * 4;
@@ -1187,6 +1253,7 @@ int check_common(int dtype, int B, int L) {
     return TPL_A2B_OK;
 }
 
+// state == nullptr (forward only): no residuals, no backward words.
 int launch_scan(bool fwd, int dtype, ScanArgs args, void* state, void* sync,
                 const tpl_a2b_config* cfg, hipStream_t stream) {
     const Tiling tl = tiling(dtype, args.L, cfg);
@@ -1194,14 +1261,17 @@ int launch_scan(bool fwd, int dtype, ScanArgs args, void* state, void* sync,
     args.ntf = tl.ntf;
     args.trb = tl.trb;
     args.ntb = tl.ntb;
-    if (tl.ntb > 1) {
-        const StateLayout st = state_layout(args.B, tl.ntb);
+    args.res_stride = res_stride(args.L);
+    if (state) {
+        const StateLayout st = state_layout(dtype, args.B, args.L, tl.ntb);
         char* base = static_cast<char*>(state);
-        args.ckpt = reinterpret_cast<double*>(base + st.ckpt_off);
-        args.bpay = reinterpret_cast<double*>(base + st.pay_off);
-        args.bflags = reinterpret_cast<uint32_t*>(base + st.flags_off);
-        args.btickets = reinterpret_cast<uint32_t*>(base + st.tickets_off);
-        args.bdone = reinterpret_cast<uint32_t*>(base + st.done_off);
+        if (dtype == TPL_A2B_F32) args.res = reinterpret_cast<_Float16*>(base + st.res_off);
+        if (tl.ntb > 1) {
+            args.bpay = reinterpret_cast<double*>(base + st.pay_off);
+            args.bflags = reinterpret_cast<uint32_t*>(base + st.flags_off);
+            args.btickets = reinterpret_cast<uint32_t*>(base + st.tickets_off);
+            args.bdone = reinterpret_cast<uint32_t*>(base + st.done_off);
+        }
     }
     if (fwd && tl.ntf > 1) {
         const SyncLayout sl = sync_layout(args.B, tl.ntf);
@@ -1221,9 +1291,8 @@ int launch_scan(bool fwd, int dtype, ScanArgs args, void* state, void* sync,
     const int r = fwd ? cfg->res_per_lane : tl.rb;
     const bool f32 = dtype == TPL_A2B_F32;
     KernelFn k = f32 ? pick_kernel<float>(fwd, r) : pick_kernel<double>(fwd, r);
-    const int tr = fwd ? tl.trf : tl.trb;
-    const size_t lds = fwd ? (f32 ? fwd_lds_bytes<float>(tr) : fwd_lds_bytes<double>(tr))
-                           : (f32 ? bwd_lds_bytes<float>(tr) : bwd_lds_bytes<double>(tr));
+    // The forward parks 9R-3 local coordinates per lane in LDS.
+    const size_t lds = fwd ? (size_t)(9 * r - 3) * (f32 ? 4 : 8) * threads : 0;
     hipLaunchKernelGGL(k, dim3(grid), dim3(threads), lds, stream, args);
     return hipGetLastError() == hipSuccess ? TPL_A2B_OK : TPL_A2B_ERR_LAUNCH;
 }
@@ -1264,7 +1333,7 @@ int64_t tpl_a2b_tile_residues(const tpl_a2b_config* cfg) {
 size_t tpl_a2b_state_bytes(int dtype, int batch, int angles_stride, const tpl_a2b_config* cfg) {
     if (check_common(dtype, batch, angles_stride) || !valid_config(cfg)) return 0;
     if (batch == 0 || angles_stride == 0) return 0;
-    return state_layout(batch, tiling(dtype, angles_stride, cfg).ntb).total;
+    return state_layout(dtype, batch, angles_stride, tiling(dtype, angles_stride, cfg).ntb).total;
 }
 
 size_t tpl_a2b_sync_bytes(int dtype, int batch, int angles_stride, const tpl_a2b_config* cfg) {
@@ -1281,8 +1350,8 @@ int tpl_a2b_forward(int dtype, const void* angles, void* coords, const int32_t* 
     if (!valid_config(cfg) || !fits_dtype(dtype, cfg)) return TPL_A2B_ERR_CONFIG;
     if (batch == 0 || angles_stride == 0) return TPL_A2B_OK;
     if (!angles || !coords || !lengths) return TPL_A2B_ERR_ARG;
-    if (tpl_a2b_state_bytes(dtype, batch, angles_stride, cfg) && !state) return TPL_A2B_ERR_ARG;
     if (tpl_a2b_sync_bytes(dtype, batch, angles_stride, cfg) && !sync) return TPL_A2B_ERR_ARG;
+    if (misaligned(state) || misaligned(sync)) return TPL_A2B_ERR_ARG;
     ScanArgs a{};
     a.angles = angles;
     a.out = coords;
@@ -1290,28 +1359,30 @@ int tpl_a2b_forward(int dtype, const void* angles, void* coords, const int32_t* 
     a.status = status;
     a.B = batch;
     a.L = angles_stride;
-    return launch_scan(true, dtype, a, state, sync, cfg, static_cast<hipStream_t>(stream));
+    void* st = tpl_a2b_state_bytes(dtype, batch, angles_stride, cfg) ? state : nullptr;
+    return launch_scan(true, dtype, a, st, sync, cfg, static_cast<hipStream_t>(stream));
 }
 
-int tpl_a2b_backward(int dtype, void* grad_angles, const void* grad_coords, const void* angles,
-                     const int32_t* lengths, const void* state, void* sync, uint32_t* status,
-                     int batch, int angles_stride, const tpl_a2b_config* cfg,
-                     tpl_stream_t stream) {
+int tpl_a2b_backward(int dtype, void* grad_angles, const void* grad_coords, const void* coords,
+                     const int32_t* lengths, void* state, uint32_t* status, int batch,
+                     int angles_stride, const tpl_a2b_config* cfg, tpl_stream_t stream) {
     int rc = check_common(dtype, batch, angles_stride);
     if (rc) return rc;
     if (!valid_config(cfg) || !fits_dtype(dtype, cfg)) return TPL_A2B_ERR_CONFIG;
     if (batch == 0 || angles_stride == 0) return TPL_A2B_OK;
-    if (!grad_angles || !grad_coords || !angles || !lengths) return TPL_A2B_ERR_ARG;
-    if (tpl_a2b_state_bytes(dtype, batch, angles_stride, cfg) && !state) return TPL_A2B_ERR_ARG;
+    if (!grad_angles || !grad_coords || !coords || !lengths) return TPL_A2B_ERR_ARG;
+    const bool need_state = tpl_a2b_state_bytes(dtype, batch, angles_stride, cfg) != 0;
+    if (need_state && !state) return TPL_A2B_ERR_ARG;
+    if (need_state && misaligned(state)) return TPL_A2B_ERR_ARG;
     ScanArgs a{};
-    a.angles = angles;
+    a.coords = coords;
     a.out = grad_angles;
     a.grad = grad_coords;
     a.lengths = lengths;
     a.status = status;
     a.B = batch;
     a.L = angles_stride;
-    return launch_scan(false, dtype, a, const_cast<void*>(state), sync, cfg,
+    return launch_scan(false, dtype, a, need_state ? state : nullptr, nullptr, cfg,
                        static_cast<hipStream_t>(stream));
 }
 
@@ -1395,8 +1466,8 @@ int tpl_a2b_tune(int dtype, const void* angles, const int32_t* lengths,
                 rc = tpl_a2b_forward(dtype, angles, coords, lengths, state, sync, nullptr, batch,
                                      angles_stride, cf, stream);
                 if (rc) break;
-                rc = tpl_a2b_backward(dtype, grad_angles, grad_coords, angles, lengths, state,
-                                      sync, nullptr, batch, angles_stride, cf, stream);
+                rc = tpl_a2b_backward(dtype, grad_angles, grad_coords, coords, lengths, state,
+                                      nullptr, batch, angles_stride, cf, stream);
                 if (rc) break;
                 if (hipEventRecord(e1, s) != hipSuccess) { rc = TPL_A2B_ERR_HIP; break; }
                 if (hipEventSynchronize(e1) != hipSuccess) { rc = TPL_A2B_ERR_HIP; break; }

@@ -96,16 +96,20 @@ hipStream_t stream_of(const torch::Tensor& t) {
 // ----------------------------------------------------------------- bindings
 
 // Angles2BackboneGPU_forward (reference angles2backbone_interface.cpp:7-26):
-// (input_angles, output_coords, angles_length, A) with A replaced by the scan
-// checkpoint state, plus the look-back scratch, optional status and config.
+// (input_angles, output_coords, angles_length, A) with A replaced by the
+// forward->backward state (None: no backward follows), plus the look-back
+// scratch, optional status and config.
 void forward(torch::Tensor input_angles, torch::Tensor output_coords,
-             torch::Tensor angles_length, torch::Tensor state, torch::Tensor sync,
+             torch::Tensor angles_length, c10::optional<torch::Tensor> state, torch::Tensor sync,
              c10::optional<torch::Tensor> status, std::vector<int64_t> config) {
     auto [B, L] = check_inputs(input_angles, angles_length);
     check_out(output_coords, input_angles, {B, 9 * (int64_t)L}, "output_coords");
     const tpl_a2b_config cfg = to_config(config);
     c10::DeviceGuard guard(input_angles.device());
-    void* st = opt_ptr(state, tpl_a2b_state_bytes(dtype_code(input_angles), B, L, &cfg), "state", input_angles);
+    void* st = nullptr;
+    if (state.has_value() && state->defined())
+        st = opt_ptr(*state, tpl_a2b_state_bytes(dtype_code(input_angles), B, L, &cfg), "state",
+                     input_angles);
     void* sy = opt_ptr(sync, tpl_a2b_sync_bytes(dtype_code(input_angles), B, L, &cfg), "sync", input_angles);
     check_rc(tpl_a2b_forward(dtype_code(input_angles), input_angles.data_ptr(),
                              output_coords.data_ptr(), angles_length.data_ptr<int32_t>(), st, sy,
@@ -115,23 +119,22 @@ void forward(torch::Tensor input_angles, torch::Tensor output_coords,
 }
 
 // Angles2BackboneGPU_backward (reference angles2backbone_interface.cpp:27-56):
-// (gradInput, gradOutput, input_angles, angles_length, A, dr_dangle) with A ->
-// state and dr_dangle -> look-back scratch.
-void backward(torch::Tensor grad_input, torch::Tensor grad_output, torch::Tensor input_angles,
-              torch::Tensor angles_length, torch::Tensor state, torch::Tensor sync,
+// (gradInput, gradOutput, input_angles, angles_length, A, dr_dangle) with
+// input_angles -> the forward's output_coords, A -> the forward's state, and
+// no dr_dangle.
+void backward(torch::Tensor grad_input, torch::Tensor grad_output, torch::Tensor output_coords,
+              torch::Tensor angles_length, torch::Tensor state,
               c10::optional<torch::Tensor> status, std::vector<int64_t> config) {
-    auto [B, L] = check_inputs(input_angles, angles_length);
-    check_out(grad_input, input_angles, {B, 3, L}, "gradInput");
-    check_out(grad_output, input_angles, {B, 9 * (int64_t)L}, "gradOutput");
+    auto [B, L] = check_inputs(grad_input, angles_length);
+    check_out(grad_output, grad_input, {B, 9 * (int64_t)L}, "gradOutput");
+    check_out(output_coords, grad_input, {B, 9 * (int64_t)L}, "output_coords");
     const tpl_a2b_config cfg = to_config(config);
-    c10::DeviceGuard guard(input_angles.device());
-    void* st = opt_ptr(state, tpl_a2b_state_bytes(dtype_code(input_angles), B, L, &cfg), "state", input_angles);
-    (void)sync;  // the backward keeps its look-back words in `state`
-    check_rc(tpl_a2b_backward(dtype_code(input_angles), grad_input.data_ptr(),
-                              grad_output.data_ptr(), input_angles.data_ptr(),
-                              angles_length.data_ptr<int32_t>(), st, nullptr,
-                              status_ptr(status, input_angles), B, L, &cfg,
-                              stream_of(input_angles)),
+    c10::DeviceGuard guard(grad_input.device());
+    void* st = opt_ptr(state, tpl_a2b_state_bytes(dtype_code(grad_input), B, L, &cfg), "state", grad_input);
+    check_rc(tpl_a2b_backward(dtype_code(grad_input), grad_input.data_ptr(),
+                              grad_output.data_ptr(), output_coords.data_ptr(),
+                              angles_length.data_ptr<int32_t>(), st,
+                              status_ptr(status, grad_input), B, L, &cfg, stream_of(grad_input)),
              "Angles2BackboneGPU_backward");
 }
 
@@ -242,9 +245,9 @@ std::tuple<double, double> bench_steps(std::vector<std::vector<torch::Tensor>> s
             rc = tpl_a2b_forward(dt, s[0].data_ptr(), s[3].data_ptr(), s[1].data_ptr<int32_t>(),
                                  s[5].data_ptr(), s[6].data_ptr(), nullptr, B, L, &cfg, stream);
             if (rc == TPL_A2B_OK)
-                rc = tpl_a2b_backward(dt, s[4].data_ptr(), s[2].data_ptr(), s[0].data_ptr(),
-                                      s[1].data_ptr<int32_t>(), s[5].data_ptr(), s[6].data_ptr(),
-                                      nullptr, B, L, &cfg, stream);
+                rc = tpl_a2b_backward(dt, s[4].data_ptr(), s[2].data_ptr(), s[3].data_ptr(),
+                                      s[1].data_ptr<int32_t>(), s[5].data_ptr(), nullptr, B, L,
+                                      &cfg, stream);
         }
         check_rc(rc, "bench_steps");
     };

@@ -37,9 +37,9 @@ def test_config_validation_and_sizes(lib):
         return (x + 255) // 256 * 256
 
     def state(f32, B, L, ntb):
-        # FP32 residual halves [B][9L rounded up to 8], then (ntb > 1) the
+        # FP32 residual halves [B][9][L rounded up to 8], then (ntb > 1) the
         # backward payload (12 doubles per tile) + flags + tickets + done
-        res = align(B * ((9 * L + 7) // 8 * 8) * 2) if f32 else 0
+        res = align(B * 9 * ((L + 7) // 8 * 8) * 2) if f32 else 0
         return res if ntb <= 1 else align(res + B * ntb * (96 + 4) + 8 * B)
 
     def sync(B, ntf):  # flags + tickets (zeroed), then 24-double payload per tile

@@ -360,8 +360,13 @@ __device__ __forceinline__ uint32_t wait_flag(const uint32_t* p, uint32_t* statu
 // run that is whole (cnt >= N) and U-byte aligned moves in U-byte accesses;
 // U = the widest of 16/8/4 bytes dividing the run, fixed per kernel, and every
 // lane of one row shares the alignment, so the branch is uniform there.
+// (Runs here are whole multiples of 2 bytes.)
 template <int U>
 struct Unit;
+template <>
+struct Unit<2> {
+    using type = uint16_t;
+};
 template <>
 struct Unit<4> {
     using type = uint32_t;
@@ -376,7 +381,8 @@ struct Unit<16> {
 };
 template <typename T, int N>
 constexpr int unit_bytes() {
-    return (N * sizeof(T)) % 16 == 0 ? 16 : ((N * sizeof(T)) % 8 == 0 ? 8 : 4);
+    constexpr int b = N * (int)sizeof(T);
+    return b % 16 == 0 ? 16 : (b % 8 == 0 ? 8 : (b % 4 == 0 ? 4 : 2));
 }
 
 template <typename T, int N>
@@ -421,15 +427,20 @@ __device__ __forceinline__ void lane_store_zero(T* p, int cnt) {
 // ------------------------------------------------------- FP32 output residual
 // With FP32 coordinates the forward also keeps, per coordinate, the rounding
 // residual of its FP64 value, r = x - (double)(float)x, as the half
-// h = RTZ_f16(r * 2^24).  The backward rebuilds x = (double)x32 + h * 2^-24,
+// h = RTZ_f16(r * 2^24), in nine planes per chain: coordinate m (0..8) of
+// residue r at res[(b*9 + m)*Lp + r], Lp = L rounded up to 8, so that the R
+// halves of a lane in one plane are contiguous and consecutive lanes are
+// adjacent.  The backward rebuilds x = (double)x32 + h * 2^-24,
 // within |x| * 2^-36 of the FP64 value (|x| < 2^16: h stays finite), so the
 // bond directions it takes from coordinate differences are FP64-grade; from
 // the FP32 coordinates alone they would carry ~1e-5 relative error.
 constexpr float kResScale = 0x1.0p24f, kResInv = 0x1.0p-24f;
 
-__device__ __forceinline__ uint32_t res_pack(double x0, float f0, double x1, float f1) {
-    const float r0 = (float)(x0 - (double)f0) * kResScale;
-    const float r1 = (float)(x1 - (double)f1) * kResScale;
+// Scaled residual of x against its FP32 rounding f.
+__device__ __forceinline__ float res_of(double x, float f) {
+    return (float)(x - (double)f) * kResScale;
+}
+__device__ __forceinline__ uint32_t res_pack(float r0, float r1) {
     return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(r0, r1));
 }
 __device__ __forceinline__ double res_apply(float x32, _Float16 h) {
@@ -448,8 +459,8 @@ struct ScanArgs {
     const void* grad;      // grad_coords (backward)
     const int32_t* lengths;
     uint32_t* status;
-    _Float16* res;         // [B][res_stride] residuals of FP32 coordinates, or null
-    int res_stride;        // halves per chain: 9L rounded up to a multiple of 8
+    _Float16* res;         // [B][9][res_lp] residuals of FP32 coordinates, or null
+    int res_lp;            // halves per residual plane: L rounded up to 8
     int B, L;
     int ntf, trf;          // forward tiles per chain, residues per forward tile
     int ntb, trb;          // backward tiles per chain, residues per backward tile
@@ -465,8 +476,8 @@ struct ScanArgs {
     double* bpay;          // [B][ntb][kBwdPayload]
 };
 
-__device__ __forceinline__ int clamp_length(const ScanArgs& a, int b, bool report) {
-    int n = a.lengths[b];
+// The chain length as used: lengths outside [0, L] are clamped and reported.
+__device__ __forceinline__ int clamp_length(const ScanArgs& a, int n, bool report) {
     if (n < 0 || n > a.L) {
         if (report) set_status(a.status, TPL_A2B_STATUS_LENGTH);
         n = n < 0 ? 0 : a.L;
@@ -474,14 +485,12 @@ __device__ __forceinline__ int clamp_length(const ScanArgs& a, int b, bool repor
     return n;
 }
 
-// Work item -> (chain, tile).  Item i belongs to chain i % B.  With several
-// tiles per chain, the tile comes from the chain's ticket counter, so a tile
-// is taken only after every earlier ticket of its chain was taken by a
-// running workgroup (forward: ascending tiles; backward: descending).
-__device__ __forceinline__ void item_tile(int item, int B, int nt, uint32_t* tickets,
-                                          bool descending, int* s_tile, int* b_out,
-                                          int* t_out) {
-    const int b = item % B;
+// Work item -> tile of chain b = item % B.  With several tiles per chain, the
+// tile comes from the chain's ticket counter, so a tile is taken only after
+// every earlier ticket of its chain was taken by a running workgroup
+// (forward: ascending tiles; backward: descending).
+__device__ __forceinline__ void item_tile(int b, int nt, uint32_t* tickets, bool descending,
+                                          int* s_tile, int* t_out) {
     int t = 0;
     if (nt > 1) {
         if (threadIdx.x == 0) *s_tile = (int)atomic_inc(tickets + b);
@@ -489,7 +498,6 @@ __device__ __forceinline__ void item_tile(int item, int B, int nt, uint32_t* tic
         t = __builtin_amdgcn_readfirstlane(*s_tile);
         if (descending) t = nt - 1 - t;
     }
-    *b_out = b;
     *t_out = t;
 }
 
@@ -514,110 +522,64 @@ __device__ __forceinline__ void cross(const double a[3], const double b[3], doub
     c[2] = a[0] * b[1] - a[1] * b[0];
 }
 
-// Streams a lane's run of N elements of T to p: element i arrives by
-// put(i, v) in order 0 .. N-1.  VEC: whole run, U-byte aligned, U-byte
-// stores as soon as a unit is complete; else element stores below `cnt`.
-template <typename T, int N, bool VEC>
-struct RunWriter {
-    static constexpr int kE = unit_bytes<T, N>() / (int)sizeof(T);  // elements per unit
-    T* p;
-    int cnt;
-    T buf[kE];
-    __device__ __forceinline__ void put(int i, T v) {
-        if constexpr (VEC) {
-            buf[i % kE] = v;
-            if (i % kE == kE - 1) {
-                using V = typename Unit<unit_bytes<T, N>()>::type;
-                V w;
-                __builtin_memcpy(&w, buf, sizeof(w));
-                reinterpret_cast<V*>(p)[i / kE] = w;
-            }
-        } else {
-            if (i < cnt) p[i] = v;
-        }
+// Wave-wide store of the run dst[0, count) in global memory: element f <
+// nsrc comes from the wave's LDS staging (lane f / E, element f % E, lane
+// stride E + 1), the rest is 0.  The aligned body moves in 16-byte stores,
+// lane-consecutive, so every store instruction of the wave writes 1 KiB.
+template <typename T, int E>
+__device__ __forceinline__ void wave_store_run(T* dst, int count, int nsrc, const T* s_w, int lane) {
+    constexpr int V = 16 / (int)sizeof(T);
+    auto elem = [&](int f) -> T { return f < nsrc ? s_w[(f / E) * (E + 1) + f % E] : (T)0; };
+    const int mis = (int)((reinterpret_cast<uintptr_t>(dst) / sizeof(T)) % V);
+    const int head = min((V - mis) % V, count);
+    const int nvec = (count - head) / V;
+    const int tail0 = head + nvec * V;
+    if (lane < head) dst[lane] = elem(lane);
+    for (int v = lane; v < nvec; v += 64) {
+        const int f0 = head + v * V;
+        T w[V];
+#pragma unroll
+        for (int e = 0; e < V; ++e) w[e] = elem(f0 + e);
+        uint4 u;
+        __builtin_memcpy(&u, w, sizeof(u));
+        *reinterpret_cast<uint4*>(dst + f0) = u;
     }
-};
+    if (tail0 + lane < count) dst[tail0 + lane] = elem(tail0 + lane);
+}
 
-// Pass 2 of the forward for one lane: atom 0 is the translation of E, atom
-// a >= 1 is E times local position a-1 (s_p); atoms at or past 3*cnt are 0.
-// Coordinates stream to x_l, FP32 residuals (when res_l) to res_l.  Returns
-// the sum of the coordinates (NaN if any is NaN).
-template <typename T, int R, bool VEC>
-__device__ __forceinline__ double fwd_pass2(const double E[kF], const T* s_p, int stride, int cnt,
-                                            int crow, T* x_l, _Float16* res_l) {
-    RunWriter<T, 9 * R, VEC> xw{x_l, 9 * crow, {}};
-    constexpr int KD = (9 * R + 1) / 2;
-    RunWriter<uint32_t, KD, VEC && (R % 2 == 0)> rw{reinterpret_cast<uint32_t*>(res_l), KD, {}};
-    [[maybe_unused]] double pend_x = 0.0;
-    [[maybe_unused]] float pend_f = 0.f;
-    double chk = 0.0;
-#pragma unroll
-    for (int at = 0; at < 3 * R; ++at) {
-        double x[3];
-        if (at == 0) {
-            x[0] = E[9];
-            x[1] = E[10];
-            x[2] = E[11];
-        } else {
-            const int m = 3 * (at - 1);
-            affine(E, (double)s_p[m * stride], (double)s_p[(m + 1) * stride],
-                   (double)s_p[(m + 2) * stride], x);
-        }
-        const bool ok = VEC || at < 3 * cnt;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const int i = 3 * at + c;
-            const double v = ok ? x[c] : 0.0;
-            const T o = (T)v;
-            chk += v;
-            xw.put(i, o);
-            if constexpr (sizeof(T) == 4) {
-                if (i % 2 == 0) {
-                    pend_x = v;
-                    pend_f = o;
-                } else if (res_l) {
-                    const uint32_t w = res_pack(pend_x, pend_f, v, o);
-                    if constexpr (VEC && R % 2 == 0) {
-                        rw.put(i / 2, w);
-                    } else {
-                        if (i - 1 < 9 * crow) res_l[i - 1] = __builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
-                        if (i < 9 * crow) res_l[i] = __builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
-                    }
-                }
-            }
-        }
-    }
-    if constexpr (sizeof(T) == 4 && (9 * R) % 2 == 1) {
-        if (res_l && 9 * R - 1 < 9 * crow) {
-            const uint32_t w = res_pack(pend_x, pend_f, pend_x, pend_f);
-            res_l[9 * R - 1] = __builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
-        }
-    }
-    return chk;
+// Wave-scope LDS ordering: the wave's earlier LDS writes are visible to its
+// later LDS reads (the LDS serves one wave's instructions in order; this
+// keeps the compiler from reordering across the point).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // ------------------------------------------------------------ forward scan
 // One tile = TR = 64*W*R residues of chain b; lane l owns residues
-// rl = r0 + l*R .. rl+R-1 and writes their 9R output floats (atoms N_rl,
-// CA_rl, C_rl, ..., C_{rl+R-1}).  Steps:
-//   1. lanes wholly past the chain write their zeros at once; the others load
-//      their R angles per row straight into registers;
+// rl = r0 + l*R .. rl+R-1, i.e. chain atoms 3rl .. 3rl+3R-1 and output
+// floats 9rl .. 9rl+9R-1.  Steps:
+//   1. the lanes load their R angles per row straight into registers; waves
+//      wholly past the chain skip to step 5;
 //   2. pass 1: per lane, FP64 sincos and the serial product of its R residues;
-//      atom positions in the lane's local frame stay in registers; waves
-//      whose first residue is past the chain skip passes 1-2;
+//      local positions of the lane's atoms 1 .. 3R-1 go to the lane's LDS slot;
 //   3. DPP scan over the wave, LDS over the waves, look-back over tiles;
-//   4. pass 2: N_rl is the translation of the lane's global prefix frame E,
-//      the other atoms x = E p; zeros past the chain; FP32 outputs also write
-//      their residuals.
+//   4. pass 2: atom 0 is the translation of the lane's global prefix frame E,
+//      atom a >= 1 is E p_a, written over p_a in LDS; FP32 outputs also write
+//      their residual planes;
+//   5. each wave stores its 9*64*R-float output span with 16-byte stores from
+//      LDS, zeros past the chain.
 template <typename T, int R, int MAXT>
 __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
+    constexpr int E9 = 9 * R;  // output floats per lane; LDS lane stride E9 + 1
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nw = blockDim.x >> 6;
     const int TR = blockDim.x * R;
-    // Local positions of the lane's atoms 1 .. 3R-1, coordinate m of the lane
-    // at s_p[m * blockDim.x + tid] (conflict-free; read back by the same lane).
-    T* s_p = reinterpret_cast<T*>(s_dyn);
+    T* s_x = reinterpret_cast<T*>(s_dyn);
+    T* s_l = s_x + tid * (E9 + 1);            // this lane's slot
+    const T* s_w = s_x + wave * 64 * (E9 + 1);  // this wave's staging
     __shared__ double s_tab[128];
     __shared__ double s_tot[4][kF];
     __shared__ double s_pfx[kF];
@@ -630,14 +592,14 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
 
     for (int item = blockIdx.x; item < a.B * a.ntf; item += gridDim.x) {
         __syncthreads();  // s_tab written; LDS of the previous tile no longer read
-        int b, t;
-        item_tile(item, a.B, a.ntf, a.ftickets, false, &s_tile, &b, &t);
-        const int n = clamp_length(a, b, t == 0 && tid == 0);
+        const int b = item % a.B;
+        const int n_raw = a.lengths[b];  // issued before the ticket round trip
+        int t;
+        item_tile(b, a.ntf, a.ftickets, false, &s_tile, &t);
+        const int n = clamp_length(a, n_raw, t == 0 && tid == 0);
         const int r0 = t * TR;
         const int rl = r0 + tid * R;
-        const int cnt = min(max(n - rl, 0), R);   // residues of the chain in this lane
-        const int crow = min(max(L - rl, 0), R);  // residues of the row in this lane
-        T* x_l = coords + (size_t)b * 9 * L + (size_t)9 * rl;
+        const int cnt = min(max(n - rl, 0), R);  // residues of the chain in this lane
 
         // Zero the backward's look-back words of the backward tiles inside
         // this forward tile (the backward returns them to zero after use).
@@ -652,9 +614,9 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
                 word_store(a.bdone + b, 0u);
             }
         }
-        if (cnt == 0 && crow > 0) lane_store_zero<T, 9 * R>(x_l, 9 * crow);
 
-        const bool wave_active = r0 + wave * 64 * R < n;  // uniform per wave
+        const int rw0 = r0 + wave * 64 * R;  // first residue of the wave
+        const bool wave_active = rw0 < n;    // uniform per wave
         double E[kF];
         if (wave_active) {
             T ang[3][R];
@@ -670,11 +632,11 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
                     if (k == 0) frame_step<0>(E, cs, sn);
                     if (k == 1) frame_step<1>(E, cs, sn);
                     if (k == 2) frame_step<2>(E, cs, sn);
-                    if (3 * q + k < 3 * R - 1) {
-                        const int m = 3 * (3 * q + k);
-                        s_p[m * blockDim.x + tid] = (T)E[9];
-                        s_p[(m + 1) * blockDim.x + tid] = (T)E[10];
-                        s_p[(m + 2) * blockDim.x + tid] = (T)E[11];
+                    const int at = 3 * q + k + 1;  // lane atom placed by this step
+                    if (at < 3 * R) {
+                        s_l[3 * at] = (T)E[9];
+                        s_l[3 * at + 1] = (T)E[10];
+                        s_l[3 * at + 2] = (T)E[11];
                     }
                 }
             }
@@ -731,48 +693,81 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
             }
             __syncthreads();
         }
-        if (wave_active && cnt > 0) {
-            lane_prefix(s_tot, wave, a.ntf > 1 ? s_pfx : nullptr, E);
-            // Pass 2: the lane's 9R coordinates (chain floats 9rl ..), zeros
-            // past the chain; 16-byte stores unless the lane ends the chain or
-            // the row is misaligned.
-            _Float16* res_l = a.res ? a.res + (size_t)b * a.res_stride + (size_t)9 * rl : nullptr;
-            constexpr int U = unit_bytes<T, 9 * R>();
-            const bool vec = cnt == R && (reinterpret_cast<uintptr_t>(x_l) % U) == 0;
-            double chk;
-            if (vec)
-                chk = fwd_pass2<T, R, true>(E, s_p + tid, blockDim.x, cnt, crow, x_l, res_l);
-            else
-                chk = fwd_pass2<T, R, false>(E, s_p + tid, blockDim.x, cnt, crow, x_l, res_l);
-            const bool bad = chk != chk;
-            if (bad) set_status(a.status, TPL_A2B_STATUS_NAN);
+        T* x_b = coords + (size_t)b * 9 * L;
+        const int g0 = 9 * rw0;  // the wave's output span [g0, g0 + 9*64*R) within [0, 9L)
+        const int wcount = min(9 * 64 * R, 9 * L - g0);
+        if (wave_active) {
+            if (cnt > 0) {
+                lane_prefix(s_tot, wave, a.ntf > 1 ? s_pfx : nullptr, E);
+                // Pass 2, in place in the lane's slot.  Residuals go to plane
+                // m at residue rl + q as soon as a pair (q even, q + 1) is
+                // complete (one half when R = 1): the lane's R halves per
+                // plane are a whole run (rl + R <= Lp).
+                [[maybe_unused]] float rpend[9];
+                [[maybe_unused]] _Float16* rp =
+                    a.res ? a.res + (size_t)b * 9 * a.res_lp + rl : nullptr;
+                double chk = 0.0;  // NaN anywhere makes the sum NaN
+#pragma unroll
+                for (int at = 0; at < 3 * R; ++at) {
+                    double x[3];
+                    if (at == 0) {
+                        x[0] = E[9];
+                        x[1] = E[10];
+                        x[2] = E[11];
+                    } else {
+                        affine(E, (double)s_l[3 * at], (double)s_l[3 * at + 1], (double)s_l[3 * at + 2], x);
+                    }
+                    const bool ok = at < 3 * cnt;
+                    const int q = at / 3;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const T o = (T)x[c];
+                        s_l[3 * at + c] = o;
+                        chk += ok ? x[c] : 0.0;
+                        if constexpr (sizeof(T) == 4) {
+                            const int m = 3 * (at % 3) + c;  // plane
+                            const float r = res_of(x[c], o);
+                            if (rp) {
+                                _Float16* d = rp + (size_t)m * a.res_lp + q;
+                                if constexpr (R == 1) {
+                                    *reinterpret_cast<uint16_t*>(d) = (uint16_t)(res_pack(r, 0.f) & 0xffffu);
+                                } else if (q % 2 == 0) {
+                                    rpend[m] = r;
+                                } else {
+                                    *reinterpret_cast<uint32_t*>(d - 1) = res_pack(rpend[m], r);
+                                }
+                            }
+                        }
+                    }
+                }
+                if (chk != chk) set_status(a.status, TPL_A2B_STATUS_NAN);
+            }
+            wave_lds_sync();
+            wave_store_run<T, E9>(x_b + g0, wcount, min(max(9 * n - g0, 0), wcount), s_w, lane);
+        } else if (wcount > 0) {
+            wave_store_run<T, E9>(x_b + g0, wcount, 0, s_w, lane);
         }
     }
 }
 
 // The lane's 9R coordinates as loaded: values (xr) and, for FP32, their
-// residual halves (hr, from res + ro; zero past the chain).
+// residual halves (hr[9q + m] from plane m at residue rl + q; zero past the
+// chain).
 template <typename T, int R>
 struct LaneCoords {
     T xr[9 * R];
     _Float16 hr[sizeof(T) == 4 ? 9 * R : 1];
 
-    __device__ __forceinline__ void load(const T* x, const _Float16* res, int cnt) {
+    // x: the lane's first coordinate; rp: plane 0 at the lane's first residue.
+    __device__ __forceinline__ void load(const T* x, const _Float16* rp, int lp, int cnt) {
         lane_load<T, 9 * R>(x, 9 * cnt, xr);
         if constexpr (sizeof(T) == 4) {
-            constexpr int KD = (9 * R + 1) / 2;
-            bool done = false;
-            if constexpr (R % 2 == 0) {
-                if (cnt == R) {  // 9*rl halves: 4-byte aligned (rl even)
-                    uint32_t hw[KD];
-                    lane_load<uint32_t, KD>(reinterpret_cast<const uint32_t*>(res), KD, hw);
-                    __builtin_memcpy(hr, hw, sizeof(hr));
-                    done = true;
-                }
-            }
-            if (!done) {
 #pragma unroll
-                for (int i = 0; i < 9 * R; ++i) hr[i] = i < 9 * cnt ? res[i] : (_Float16)0;
+            for (int m = 0; m < 9; ++m) {
+                _Float16 h[R];
+                lane_load<_Float16, R>(rp + (size_t)m * lp, cnt, h);
+#pragma unroll
+                for (int q = 0; q < R; ++q) hr[9 * q + m] = h[q];
             }
         }
     }
@@ -823,9 +818,11 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
 
     for (int item = blockIdx.x; item < a.B * a.ntb; item += gridDim.x) {
         __syncthreads();
-        int b, t;
-        item_tile(item, a.B, a.ntb, a.btickets, true, &s_tile, &b, &t);
-        const int n = clamp_length(a, b, t == 0 && tid == 0);
+        const int b = item % a.B;
+        const int n_raw = a.lengths[b];  // issued before the ticket round trip
+        int t;
+        item_tile(b, a.ntb, a.btickets, true, &s_tile, &t);
+        const int n = clamp_length(a, n_raw, t == 0 && tid == 0);
         const int r0 = t * TR;
         const int rl = r0 + tid * R;
         const int cnt = min(max(n - rl, 0), R);
@@ -840,12 +837,12 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
         // Pass A: W = sum over the lane's atoms 0 .. 3R-1 (chain atoms
         // 3rl .. 3rl+3R-1) of (x x g, g); zero past the chain.
         const size_t o = (size_t)b * 9 * L + (size_t)9 * rl;
-        const size_t ro = (size_t)b * a.res_stride + (size_t)9 * rl;
+        const size_t ro = (size_t)b * 9 * a.res_lp + (size_t)rl;  // plane 0, residue rl
         double W[6] = {0, 0, 0, 0, 0, 0};
         if (wave_active) {
             LaneCoords<T, R> xc;
             T gr[9 * R];
-            xc.load(xin + o, a.res ? a.res + ro : nullptr, cnt);
+            xc.load(xin + o, a.res ? a.res + ro : nullptr, a.res_lp, cnt);
             lane_load<T, 9 * R>(gin + o, 9 * cnt, gr);
 #pragma unroll
             for (int at = 0; at < 3 * R; ++at) {
@@ -925,7 +922,7 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
             // 3(rl+R), the next lane's first.
             LaneCoords<T, R> xc;
             T gr[9 * R];
-            xc.load(xin + o, a.res ? a.res + ro : nullptr, cnt);
+            xc.load(xin + o, a.res ? a.res + ro : nullptr, a.res_lp, cnt);
             lane_load<T, 9 * R>(gin + o, 9 * cnt, gr);
             const bool has_next = rl + R < n;
             double xn[3], gn[3];  // chain atom 3(rl+R)
@@ -934,7 +931,7 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
                 const T xv = has_next ? xin[o + 9 * R + c] : (T)0;
                 gn[c] = has_next ? (double)gin[o + 9 * R + c] : 0.0;
                 if constexpr (sizeof(T) == 4)
-                    xn[c] = has_next ? res_apply((float)xv, a.res[ro + 9 * R + c]) : 0.0;
+                    xn[c] = has_next ? res_apply((float)xv, a.res[ro + (size_t)c * a.res_lp + R]) : 0.0;
                 else
                     xn[c] = (double)xv;
             }
@@ -1205,9 +1202,10 @@ SyncLayout sync_layout(int B, int ntf) {
     return s;
 }
 
-// Forward -> backward state: FP32 coordinate residuals [B][res_stride] halves
-// (FP32 only), then the backward's look-back words and payload (ntb > 1).
-int res_stride(int L) { return (9 * L + 7) & ~7; }
+// Forward -> backward state: FP32 coordinate residuals [B][9][Lp] halves,
+// Lp = L rounded up to 8 (FP32 only), then the backward's look-back words
+// and payload (ntb > 1).
+int res_lp(int L) { return (L + 7) & ~7; }
 
 struct StateLayout {
     size_t res_off, pay_off, flags_off, tickets_off, done_off, total;
@@ -1215,7 +1213,7 @@ struct StateLayout {
 StateLayout state_layout(int dtype, int B, int L, int ntb) {
     StateLayout s{};
     s.res_off = 0;
-    const size_t res = dtype == TPL_A2B_F32 ? (size_t)B * res_stride(L) * 2 : 0;
+    const size_t res = dtype == TPL_A2B_F32 ? (size_t)B * 9 * res_lp(L) * 2 : 0;
     s.total = align256(res);
     if (ntb <= 1) return s;
     const size_t tiles = (size_t)B * ntb;
@@ -1261,7 +1259,7 @@ int launch_scan(bool fwd, int dtype, ScanArgs args, void* state, void* sync,
     args.ntf = tl.ntf;
     args.trb = tl.trb;
     args.ntb = tl.ntb;
-    args.res_stride = res_stride(args.L);
+    args.res_lp = res_lp(args.L);
     if (state) {
         const StateLayout st = state_layout(dtype, args.B, args.L, tl.ntb);
         char* base = static_cast<char*>(state);
@@ -1291,8 +1289,8 @@ int launch_scan(bool fwd, int dtype, ScanArgs args, void* state, void* sync,
     const int r = fwd ? cfg->res_per_lane : tl.rb;
     const bool f32 = dtype == TPL_A2B_F32;
     KernelFn k = f32 ? pick_kernel<float>(fwd, r) : pick_kernel<double>(fwd, r);
-    // The forward parks 9R-3 local coordinates per lane in LDS.
-    const size_t lds = fwd ? (size_t)(9 * r - 3) * (f32 ? 4 : 8) * threads : 0;
+    // The forward stages 9R coordinates per lane in LDS (lane stride 9R + 1).
+    const size_t lds = fwd ? (size_t)(9 * r + 1) * (f32 ? 4 : 8) * threads : 0;
     hipLaunchKernelGGL(k, dim3(grid), dim3(threads), lds, stream, args);
     return hipGetLastError() == hipSuccess ? TPL_A2B_OK : TPL_A2B_ERR_LAUNCH;
 }

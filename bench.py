@@ -12,8 +12,9 @@ every step streams from HBM.
 Prints one JSON line.  ``value`` is residues/s (sum of chain lengths per
 second).  The line also carries:
 * ``roofline``: the slower of the two kernels, timed alone with HIP events on
-  the stream it runs on; algorithmic bytes per launch / average duration
-  against the MI355X 8 TB/s HBM peak;
+  the stream it runs on; algorithmic bytes per launch (``alg_bytes``) / average
+  duration against the MI355X 8 TB/s HBM peak, and the HBM bytes per launch the
+  counters measured (``profiles/traffic_<workload>.json``);
 * ``plain``: the same step with the plain kernels (serial forward, O(L^2)
   backward) and the speedup of the scan path over it;
 * ``cpu_baseline``: the reference CPU algorithm on a bounded sample of the same
@@ -38,15 +39,18 @@ MALL_BYTES = 256 << 20
 def alg_bytes(n, L, io_bytes=4):
     """Compulsory HBM bytes of one forward and one backward launch.
 
-    forward:  read angles (3 per residue), write coords (9 per slot incl. padding),
-              read lengths;
-    backward: read grad coords (9 per residue) and angles (3 per residue), write
-              grad angles (3 per slot incl. padding), read lengths."""
+    forward:  read angles (3 per residue), write coords (9 per slot incl.
+              padding), write the FP32 rounding residual of every coordinate
+              (9 halves per residue, the forward->backward state), read lengths;
+    backward: read coords and grad coords (9 + 9 per residue) and the residuals
+              (9 halves per residue), write grad angles (3 per slot incl.
+              padding), read lengths."""
     import numpy as np
     B = len(n)
     S = int(np.sum(n))
-    fwd = io_bytes * (3 * S + 9 * B * L) + 4 * B
-    bwd = io_bytes * (9 * S + 3 * S + 3 * B * L) + 4 * B
+    res = 2 * 9 * S if io_bytes == 4 else 0
+    fwd = io_bytes * (3 * S + 9 * B * L) + res + 4 * B
+    bwd = io_bytes * (9 * S + 9 * S + 3 * B * L) + res + 4 * B
     return fwd, bwd
 
 

@@ -74,7 +74,10 @@ enum tpl_a2b_error {
 typedef struct tpl_a2b_config {
     int32_t waves;        /* 64-lane waves per workgroup: 1, 2 or 4 */
     int32_t res_per_lane; /* residues composed serially per lane: 1, 2, 4 or 8 */
-    int32_t max_wgs;      /* persistent grid cap; 0 = one workgroup per tile */
+    int32_t max_wgs;      /* forward grid cap (persistent workgroups walk the
+                             tiles; with one tile per chain each loads its next
+                             chain during the current one); 0 = one workgroup
+                             per tile.  The backward runs one workgroup per tile. */
     int32_t flags;        /* reserved, must be 0 */
 } tpl_a2b_config;
 

@@ -71,6 +71,8 @@ CONFIGS = [
     (2, 8, 0),   # 1024-residue tiles, the float32 maximum
     (1, 1, 3),   # persistent: 3 workgroups walk 45 tiles with look-back
     (2, 1, 7),
+    (4, 2, 3),   # one tile per chain, 3 workgroups load each next chain ahead
+    (2, 4, 2),
 ]
 
 
@@ -86,7 +88,7 @@ def test_ragged_batch_all_configs(cfg):
 
 def test_stride_not_multiple_of_anything():
     a, n, g = synthetic.custom([37, 1, 36, 13], 37, seed=5)
-    for cfg in [(1, 1, 0), (1, 2, 0), (2, 8, 0)]:
+    for cfg in [(1, 1, 0), (1, 2, 0), (2, 8, 0), (1, 1, 2)]:
         x, ga = run(a, n, g, cfg)
         x64, g64 = oracle(a, n, g)
         assert np.abs(x - x64).max() <= coord_tol(x64)

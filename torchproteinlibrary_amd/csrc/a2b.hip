@@ -590,13 +590,39 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
     const int L = a.L;
     for (int i = tid; i < 128; i += blockDim.x) s_tab[i] = kTabCS[i];
 
-    for (int item = blockIdx.x; item < a.B * a.ntf; item += gridDim.x) {
+    // With one tile per chain the items are independent, and a workgroup that
+    // walks several of them (a grid smaller than B) loads the next chain's
+    // length and angles while it works on the current one.  With several
+    // tiles per chain the tile comes from the chain's ticket at the start of
+    // the item, so nothing is loaded ahead.
+    const int total = a.B * a.ntf;
+    const bool ahead = a.ntf == 1;
+    T ang[3][R];  // this lane's angles of the current item (rows phi, psi, omega)
+    int n_ahead = 0;
+    auto load_ahead = [&](int it, T (&dst)[3][R]) {
+        const int nb = clamp_length(a, a.lengths[it % a.B], false);
+        const int c = min(max(nb - tid * R, 0), R);
+        if (wave * 64 * R < nb) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                lane_load<T, R>(angles + ((size_t)(it % a.B) * 3 + k) * L + tid * R, c, dst[k]);
+        }
+        return nb;
+    };
+    if (ahead && (int)blockIdx.x < total) n_ahead = load_ahead(blockIdx.x, ang);
+
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
         __syncthreads();  // s_tab written; LDS of the previous tile no longer read
         const int b = item % a.B;
-        const int n_raw = a.lengths[b];  // issued before the ticket round trip
-        int t;
-        item_tile(b, a.ntf, a.ftickets, false, &s_tile, &t);
-        const int n = clamp_length(a, n_raw, t == 0 && tid == 0);
+        int t = 0, n;
+        if (ahead) {
+            n = n_ahead;
+            if (tid == 0) clamp_length(a, a.lengths[b], true);  // reports a bad length
+        } else {
+            const int n_raw = a.lengths[b];  // issued before the ticket round trip
+            item_tile(b, a.ntf, a.ftickets, false, &s_tile, &t);
+            n = clamp_length(a, n_raw, t == 0 && tid == 0);
+        }
         const int r0 = t * TR;
         const int rl = r0 + tid * R;
         const int cnt = min(max(n - rl, 0), R);  // residues of the chain in this lane
@@ -617,11 +643,14 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
 
         const int rw0 = r0 + wave * 64 * R;  // first residue of the wave
         const bool wave_active = rw0 < n;    // uniform per wave
+        const int next = item + (int)gridDim.x;
+        T ang_next[3][R];
         double E[kF];
         if (wave_active) {
-            T ang[3][R];
+            if (!ahead) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) lane_load<T, R>(angles + ((size_t)b * 3 + k) * L + rl, cnt, ang[k]);
+                for (int k = 0; k < 3; ++k) lane_load<T, R>(angles + ((size_t)b * 3 + k) * L + rl, cnt, ang[k]);
+            }
             frame_identity(E);
 #pragma unroll
             for (int q = 0; q < R; ++q) {
@@ -640,6 +669,10 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
                     }
                 }
             }
+        }
+        // The next item's inputs load during this item's scan and stores.
+        if (ahead && next < total) n_ahead = load_ahead(next, ang_next);
+        if (wave_active) {
             wave_scan_frames(E);
             if (lane == 63) {
 #pragma unroll
@@ -704,9 +737,11 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
                 // complete (one half when R = 1): the lane's R halves per
                 // plane are a whole run (rl + R <= Lp).
                 [[maybe_unused]] float rpend[9];
-                [[maybe_unused]] _Float16* rp =
-                    a.res ? a.res + (size_t)b * 9 * a.res_lp + rl : nullptr;
-                double chk = 0.0;  // NaN anywhere makes the sum NaN
+                [[maybe_unused]] _Float16* rp = a.res + (size_t)b * 9 * a.res_lp + rl;
+                const bool has_res = a.res != nullptr;  // uniform
+                // Sum of the lane's coordinates: NaN or inf anywhere shows.
+                // Coordinates past the chain come from zero angles: finite.
+                float chk = 0.f;
 #pragma unroll
                 for (int at = 0; at < 3 * R; ++at) {
                     double x[3];
@@ -717,17 +752,16 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
                     } else {
                         affine(E, (double)s_l[3 * at], (double)s_l[3 * at + 1], (double)s_l[3 * at + 2], x);
                     }
-                    const bool ok = at < 3 * cnt;
                     const int q = at / 3;
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const T o = (T)x[c];
                         s_l[3 * at + c] = o;
-                        chk += ok ? x[c] : 0.0;
+                        chk += (float)o;
                         if constexpr (sizeof(T) == 4) {
                             const int m = 3 * (at % 3) + c;  // plane
                             const float r = res_of(x[c], o);
-                            if (rp) {
+                            if (has_res) {
                                 _Float16* d = rp + (size_t)m * a.res_lp + q;
                                 if constexpr (R == 1) {
                                     *reinterpret_cast<uint16_t*>(d) = (uint16_t)(res_pack(r, 0.f) & 0xffffu);
@@ -740,12 +774,18 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
                         }
                     }
                 }
-                if (chk != chk) set_status(a.status, TPL_A2B_STATUS_NAN);
+                if (!__builtin_isfinite(chk)) set_status(a.status, TPL_A2B_STATUS_NAN);
             }
             wave_lds_sync();
             wave_store_run<T, E9>(x_b + g0, wcount, min(max(9 * n - g0, 0), wcount), s_w, lane);
         } else if (wcount > 0) {
             wave_store_run<T, E9>(x_b + g0, wcount, 0, s_w, lane);
+        }
+        if (ahead) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int q = 0; q < R; ++q) ang[k][q] = ang_next[k][q];
         }
     }
 }
@@ -1284,7 +1324,7 @@ int launch_scan(bool fwd, int dtype, ScanArgs args, void* state, void* sync,
     }
     const int nt = fwd ? tl.ntf : tl.ntb;
     int grid = args.B * nt;
-    if (cfg->max_wgs > 0) grid = std::min(grid, cfg->max_wgs);
+    if (fwd && cfg->max_wgs > 0) grid = std::min(grid, cfg->max_wgs);
     const int threads = 64 * cfg->waves;
     const int r = fwd ? cfg->res_per_lane : tl.rb;
     const bool f32 = dtype == TPL_A2B_F32;

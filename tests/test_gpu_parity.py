@@ -253,7 +253,8 @@ def test_tuner_picks_a_candidate():
     A = torch.tensor(a, device=DEV)
     N = torch.tensor(n, device=DEV)
     cfg = _tuning.tune_now(A, N)
-    assert cfg in [tuple(c) for c in _tuning.candidates(500, torch.float32)]
+    cus = torch.cuda.get_device_properties(A.device).multi_processor_count
+    assert cfg in [tuple(c) for c in _tuning.candidates(500, torch.float32, cus, 128)]
     t = _tuning.timings(A)
     assert t and all(v > 0 for v in t.values())
     print("fit500 B=128 tuner timings (ms):", t)

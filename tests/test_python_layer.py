@@ -26,14 +26,15 @@ def test_bad_form():
 def test_candidates_cover_the_length():
     for L in (1, 30, 64, 200, 700, 1000, 16384):
         for dt, cap in ((torch.float32, 1024), (torch.float64, 512)):
-            c = _tuning.candidates(L, dt, cus=256)
+            c = _tuning.candidates(L, dt, cus=256, batch=1024)
             assert c and all(w in (1, 2, 4) and r in (1, 2, 4, 8) for w, r, _ in c)
             assert all(64 * w * r <= cap for w, r, _ in c)
             # persistent grids only where one tile holds a chain
             assert all(g in (0, 512, 768, 1024) for _, _, g in c)
             assert all(g == 0 or 64 * w * r >= L for w, r, g in c)
-    assert (4, 4, 768) in _tuning.candidates(1000, torch.float32, cus=256)
+    assert (4, 4, 768) in _tuning.candidates(1000, torch.float32, cus=256, batch=1024)
     assert all(g == 0 for _, _, g in _tuning.candidates(1000, torch.float32))
+    assert all(g == 0 for _, _, g in _tuning.candidates(1000, torch.float32, cus=256, batch=512))
 
 
 def test_workspace_bytes_binding():

@@ -30,13 +30,14 @@ RES_PER_LANE = (1, 2, 4, 8)
 MAX_TILE = {torch.float32: 1024, torch.float64: 512}  # register-tile limit per dtype
 
 
-def candidates(max_length: int, dtype=torch.float32, cus: int = 0):
+def candidates(max_length: int, dtype=torch.float32, cus: int = 0, batch: int = 0):
     """Launch shapes worth timing for chains padded to max_length: every
     (waves, res_per_lane) whose tile is at most the next power of two >= L
     (larger tiles only add idle lanes) and within the dtype's tile limit, with
     one workgroup per tile; and, where one tile holds a whole chain, the same
-    shape on a persistent forward grid of 2, 3 or 4 workgroups per CU (the
-    forward then loads the next chain while it works on the current one)."""
+    shape on a persistent forward grid of 2, 3 or 4 workgroups per CU, when
+    that is fewer workgroups than chains (each then loads its next chain while
+    it works on the current one)."""
     cap = 64
     while cap < max_length:
         cap *= 2
@@ -48,7 +49,7 @@ def candidates(max_length: int, dtype=torch.float32, cus: int = 0):
             if tile <= max(cap, 64):
                 out.append((w, r, 0))
                 if cus > 0 and tile >= max_length:
-                    out.extend((w, r, k * cus) for k in (2, 3, 4))
+                    out.extend((w, r, k * cus) for k in (2, 3, 4) if k * cus < batch)
     if not out:
         out.append((1, 1, 0))
     return out
@@ -116,7 +117,7 @@ def _tune(angles, lengths, reps: int = 5):
     if B == 0 or L == 0:
         return default_config(angles)
     cus = torch.cuda.get_device_properties(angles.device).multi_processor_count
-    cands = candidates(int(L), angles.dtype, cus)
+    cands = candidates(int(L), angles.dtype, cus, int(B))
     st = sy = 0
     for c in cands:
         s, y = ext.workspace_bytes(angles.dtype == torch.float64, int(B), int(L), list(c))

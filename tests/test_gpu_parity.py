@@ -86,6 +86,22 @@ def test_ragged_batch_all_configs(cfg):
     assert_exact_zeros(x, ga, n, 300)
 
 
+def test_backward_whole_chain_walk():
+    """B >= 512 chains of at most 4 backward tiles: one workgroup walks each
+    chain's tiles from the last to the first (no look-back); lengths cover
+    every tile count and the boundaries of tiles and lanes."""
+    rng = np.random.default_rng(7)
+    n = rng.integers(0, 1001, size=512)
+    n[:8] = [1000, 999, 768, 767, 513, 256, 255, 1]
+    a, n, g = synthetic.custom(n, 1000, seed=8)
+    x64, g64 = oracle(a, n, g)
+    for cfg in [(2, 2, 0), (4, 2, 0), (4, 4, 0)]:  # 4, 2, 2 backward tiles per chain
+        x, ga = run(a, n, g, cfg)
+        assert np.abs(x - x64).max() <= coord_tol(x64)
+        assert grad_rel_err(ga, g64, n) <= 1e-5
+        assert_exact_zeros(x, ga, n, 1000)
+
+
 def test_stride_not_multiple_of_anything():
     a, n, g = synthetic.custom([37, 1, 36, 13], 37, seed=5)
     for cfg in [(1, 1, 0), (1, 2, 0), (2, 8, 0), (1, 1, 2)]:

@@ -385,18 +385,83 @@ constexpr int unit_bytes() {
     return b % 16 == 0 ? 16 : (b % 8 == 0 ? 8 : (b % 4 == 0 ? 4 : 2));
 }
 
+// avail: elements at p that may be read (the run's room in its row); the
+// whole run is read when avail >= N, and elements at or past cnt are then
+// zeroed with selects, so a lane that ends its chain takes no branch.
 template <typename T, int N>
-__device__ __forceinline__ void lane_load(const T* p, int cnt, T (&v)[N]) {
+__device__ __forceinline__ void lane_load(const T* p, int cnt, T (&v)[N], int avail = -1) {
     constexpr int U = unit_bytes<T, N>();
     using V = typename Unit<U>::type;
-    if (cnt >= N && (reinterpret_cast<uintptr_t>(p) % U) == 0) {
+    if (avail < 0) avail = cnt;
+    if (avail >= N && (reinterpret_cast<uintptr_t>(p) % U) == 0) {
         V w[N * sizeof(T) / U];
 #pragma unroll
         for (int i = 0; i < (int)(N * sizeof(T) / U); ++i) w[i] = reinterpret_cast<const V*>(p)[i];
         __builtin_memcpy(v, w, sizeof(v));
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = i < cnt ? v[i] : (T)0;
     } else {
 #pragma unroll
         for (int i = 0; i < N; ++i) v[i] = i < cnt ? p[i] : (T)0;
+    }
+}
+
+// Two runs of the same shape (e.g. coordinates and their gradients) under
+// one alignment test, so their loads are in flight together.
+template <typename T, int N>
+__device__ __forceinline__ void lane_load2(const T* p, const T* q, int cnt, int avail, T (&v)[N],
+                                           T (&w)[N]) {
+    constexpr int U = unit_bytes<T, N>();
+    using V = typename Unit<U>::type;
+    const bool vec = avail >= N && ((reinterpret_cast<uintptr_t>(p) | reinterpret_cast<uintptr_t>(q)) % U) == 0;
+    if (vec) {
+        V a[N * sizeof(T) / U], c[N * sizeof(T) / U];
+#pragma unroll
+        for (int i = 0; i < (int)(N * sizeof(T) / U); ++i) {
+            a[i] = reinterpret_cast<const V*>(p)[i];
+            c[i] = reinterpret_cast<const V*>(q)[i];
+        }
+        __builtin_memcpy(v, a, sizeof(v));
+        __builtin_memcpy(w, c, sizeof(w));
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            v[i] = i < cnt ? v[i] : (T)0;
+            w[i] = i < cnt ? w[i] : (T)0;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            v[i] = i < cnt ? p[i] : (T)0;
+            w[i] = i < cnt ? q[i] : (T)0;
+        }
+    }
+}
+
+// The lane's R angles of the three rows (stride L elements) under one
+// alignment test, so the three loads are in flight together.
+template <typename T, int R>
+__device__ __forceinline__ void lane_load_rows(const T* p, int L, int cnt, int avail, T (&v)[3][R]) {
+    constexpr int U = unit_bytes<T, R>();
+    using V = typename Unit<U>::type;
+    const bool vec = avail >= R && (reinterpret_cast<uintptr_t>(p) % U) == 0 &&
+                     ((size_t)L * sizeof(T)) % U == 0;
+    if (vec) {
+        V a[3][R * sizeof(T) / U];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int i = 0; i < (int)(R * sizeof(T) / U); ++i) a[k][i] = reinterpret_cast<const V*>(p + (size_t)k * L)[i];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            __builtin_memcpy(v[k], a[k], sizeof(v[k]));
+#pragma unroll
+            for (int i = 0; i < R; ++i) v[k][i] = i < cnt ? v[k][i] : (T)0;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int i = 0; i < R; ++i) v[k][i] = i < cnt ? p[(size_t)k * L + i] : (T)0;
     }
 }
 
@@ -464,6 +529,7 @@ struct ScanArgs {
     int B, L;
     int ntf, trf;          // forward tiles per chain, residues per forward tile
     int ntb, trb;          // backward tiles per chain, residues per backward tile
+    int bwhole;            // backward: one workgroup walks all tiles of a chain
     // forward look-back (ntf > 1), in the per-call scratch, zeroed before launch
     uint32_t* fflags;      // [B][ntf]
     uint32_t* ftickets;    // [B]
@@ -602,11 +668,8 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
     auto load_ahead = [&](int it, T (&dst)[3][R]) {
         const int nb = clamp_length(a, a.lengths[it % a.B], false);
         const int c = min(max(nb - tid * R, 0), R);
-        if (wave * 64 * R < nb) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-                lane_load<T, R>(angles + ((size_t)(it % a.B) * 3 + k) * L + tid * R, c, dst[k]);
-        }
+        if (wave * 64 * R < nb)
+            lane_load_rows<T, R>(angles + (size_t)(it % a.B) * 3 * L + tid * R, L, c, L - tid * R, dst);
         return nb;
     };
     if (ahead && (int)blockIdx.x < total) n_ahead = load_ahead(blockIdx.x, ang);
@@ -647,10 +710,7 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
         T ang_next[3][R];
         double E[kF];
         if (wave_active) {
-            if (!ahead) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) lane_load<T, R>(angles + ((size_t)b * 3 + k) * L + rl, cnt, ang[k]);
-            }
+            if (!ahead) lane_load_rows<T, R>(angles + (size_t)b * 3 * L + rl, L, cnt, L - rl, ang);
             frame_identity(E);
 #pragma unroll
             for (int q = 0; q < R; ++q) {
@@ -798,16 +858,26 @@ struct LaneCoords {
     T xr[9 * R];
     _Float16 hr[sizeof(T) == 4 ? 9 * R : 1];
 
-    // x: the lane's first coordinate; rp: plane 0 at the lane's first residue.
-    __device__ __forceinline__ void load(const T* x, const _Float16* rp, int lp, int cnt) {
-        lane_load<T, 9 * R>(x, 9 * cnt, xr);
+    // x, g: the lane's first coordinate and coordinate gradient (into gr);
+    // rp: plane 0 of the chain; rl: the lane's first residue; room: residues
+    // of the row from rl.
+    __device__ __forceinline__ void load(const T* x, const T* g, T (&gr)[9 * R], const _Float16* rp,
+                                         int lp, int rl, int cnt, int room) {
+        lane_load2<T, 9 * R>(x, g, 9 * cnt, 9 * room, xr, gr);
         if constexpr (sizeof(T) == 4) {
+            // One 2R-byte load per plane, no branch: the run at residue rl is
+            // 2R-aligned and inside the plane row for every lane of the row
+            // (rl + R <= Lp); lanes past the row read the row's last run.
+            // Halves past the chain are zeroed.
+            const _Float16* p0 = rp + min(rl, lp - R);
 #pragma unroll
             for (int m = 0; m < 9; ++m) {
+                using V = typename Unit<2 * R>::type;
+                const V u = *reinterpret_cast<const V*>(p0 + (size_t)m * lp);
                 _Float16 h[R];
-                lane_load<_Float16, R>(rp + (size_t)m * lp, cnt, h);
+                __builtin_memcpy(h, &u, sizeof(h));
 #pragma unroll
-                for (int q = 0; q < R; ++q) hr[9 * q + m] = h[q];
+                for (int q = 0; q < R; ++q) hr[9 * q + m] = q < cnt ? h[q] : (_Float16)0;
             }
         }
     }
@@ -856,181 +926,205 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
     const int L = a.L;
     constexpr double kInvD[3] = {1.0 / 1.460, 1.0 / 1.525, 1.0 / 1.330};
 
-    for (int item = blockIdx.x; item < a.B * a.ntb; item += gridDim.x) {
+    // Work items: with bwhole, a chain (its tiles walked from the last to the
+    // first, the sum over later tiles carried in registers); else a tile,
+    // chained to the later tiles of its chain by the reverse look-back.
+    const bool whole = a.bwhole != 0;
+    const int nitems = whole ? a.B : a.B * a.ntb;
+    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
         __syncthreads();
         const int b = item % a.B;
         const int n_raw = a.lengths[b];  // issued before the ticket round trip
-        int t;
-        item_tile(b, a.ntb, a.btickets, true, &s_tile, &t);
-        const int n = clamp_length(a, n_raw, t == 0 && tid == 0);
-        const int r0 = t * TR;
-        const int rl = r0 + tid * R;
-        const int cnt = min(max(n - rl, 0), R);
-        const int crow = min(max(L - rl, 0), R);
-        T* ga_l = gang + (size_t)b * 3 * L + rl;
-        if (cnt == 0 && crow > 0) {
+        int t0 = 0;
+        if (!whole) item_tile(b, a.ntb, a.btickets, true, &s_tile, &t0);
+        const int n = clamp_length(a, n_raw, (whole || t0 == 0) && tid == 0);
+        double carry[6] = {0, 0, 0, 0, 0, 0};  // whole: sum over the later tiles
+        for (int step = 0; step < (whole ? a.ntb : 1); ++step) {
+            const int t = whole ? a.ntb - 1 - step : t0;
+            if (whole) __syncthreads();  // s_wsum of the previous tile no longer read
+            const int r0 = t * TR;
+            const int rl = r0 + tid * R;
+            const int cnt = min(max(n - rl, 0), R);
+            const int crow = min(max(L - rl, 0), R);
+            T* ga_l = gang + (size_t)b * 3 * L + rl;
+            if (cnt == 0 && crow > 0) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) lane_store_zero<T, R>(ga_l + (size_t)k * L, crow);
-        }
-        const bool wave_active = r0 + wave * 64 * R < n;  // uniform per wave
+                for (int k = 0; k < 3; ++k) lane_store_zero<T, R>(ga_l + (size_t)k * L, crow);
+            }
+            const bool wave_active = r0 + wave * 64 * R < n;  // uniform per wave
 
-        // Pass A: W = sum over the lane's atoms 0 .. 3R-1 (chain atoms
-        // 3rl .. 3rl+3R-1) of (x x g, g); zero past the chain.
-        const size_t o = (size_t)b * 9 * L + (size_t)9 * rl;
-        const size_t ro = (size_t)b * 9 * a.res_lp + (size_t)rl;  // plane 0, residue rl
-        double W[6] = {0, 0, 0, 0, 0, 0};
-        if (wave_active) {
-            LaneCoords<T, R> xc;
-            T gr[9 * R];
-            xc.load(xin + o, a.res ? a.res + ro : nullptr, a.res_lp, cnt);
-            lane_load<T, 9 * R>(gin + o, 9 * cnt, gr);
+            // Pass A: W = sum over the lane's atoms 0 .. 3R-1 (chain atoms
+            // 3rl .. 3rl+3R-1) of (x x g, g); zero past the chain.
+            const size_t o = (size_t)b * 9 * L + (size_t)9 * rl;
+            const size_t ro = (size_t)b * 9 * a.res_lp + (size_t)rl;  // plane 0, residue rl
+            double W[6] = {0, 0, 0, 0, 0, 0};
+            if (wave_active) {
+                LaneCoords<T, R> xc;
+                T gr[9 * R];
+                xc.load(xin + o, gin + o, gr, a.res ? a.res + (size_t)b * 9 * a.res_lp : nullptr,
+                        a.res_lp, rl, cnt, L - rl);
 #pragma unroll
-            for (int at = 0; at < 3 * R; ++at) {
-                double x[3], xg[3];
-                xc.atom(at, x);
-                const double g[3] = {(double)gr[3 * at], (double)gr[3 * at + 1], (double)gr[3 * at + 2]};
-                cross(x, g, xg);
-                W[0] += xg[0]; W[1] += xg[1]; W[2] += xg[2];
-                W[3] += g[0]; W[4] += g[1]; W[5] += g[2];
-            }
-        }
-        // C = sum over later lanes, waves and tiles.
-        double C[6];
-#pragma unroll
-        for (int e = 0; e < 6; ++e) C[e] = W[e];
-        if (wave_active) {
-            wave_scan_sum<6>(C);  // inclusive prefix over lanes
-#pragma unroll
-            for (int e = 0; e < 6; ++e) {
-                const double tot = read_lane(C[e], 63);
-                if (lane == 0) s_wsum[wave][e] = tot;
-                C[e] = tot - C[e];
-            }
-        } else if (lane == 0) {
-#pragma unroll
-            for (int e = 0; e < 6; ++e) s_wsum[wave][e] = 0.0;
-        }
-        __syncthreads();
-        for (int w = wave + 1; w < nw; ++w) {
-#pragma unroll
-            for (int e = 0; e < 6; ++e) C[e] += s_wsum[w][e];
-        }
-        if (a.ntb > 1) {  // then bflags, bpay, btickets, bdone are set (state)
-            if (tid == 0) {
-                double tot[6] = {0, 0, 0, 0, 0, 0}, carry[6] = {0, 0, 0, 0, 0, 0};
-                for (int w = 0; w < nw; ++w) {
-#pragma unroll
-                    for (int e = 0; e < 6; ++e) tot[e] += s_wsum[w][e];
+                for (int at = 0; at < 3 * R; ++at) {
+                    double x[3], xg[3];
+                    xc.atom(at, x);
+                    const double g[3] = {(double)gr[3 * at], (double)gr[3 * at + 1], (double)gr[3 * at + 2]};
+                    cross(x, g, xg);
+                    W[0] += xg[0]; W[1] += xg[1]; W[2] += xg[2];
+                    W[3] += g[0]; W[4] += g[1]; W[5] += g[2];
                 }
-                const int ntv = (n + TR - 1) / TR;  // tiles holding residues
-                if (t < ntv) {
-                    const size_t s0 = (size_t)b * a.ntb;
-                    uint32_t* fl = a.bflags + s0;
-                    double* pay = a.bpay + s0 * kBwdPayload;
-                    double* my = pay + (size_t)t * kBwdPayload;
-                    if (t == ntv - 1) {
+            }
+            // C = sum over later lanes, waves and tiles.
+            double C[6];
 #pragma unroll
-                        for (int e = 0; e < 6; ++e) payload_store(my + 6 + e, tot[e]);
-                        flag_store(fl + t, kFlagInclusive);
-                    } else {
+            for (int e = 0; e < 6; ++e) C[e] = W[e];
+            if (wave_active) {
+                wave_scan_sum<6>(C);  // inclusive prefix over lanes
 #pragma unroll
-                        for (int e = 0; e < 6; ++e) payload_store(my + e, tot[e]);
-                        flag_store(fl + t, kFlagAggregate);
-                        for (int j = t + 1; j < ntv; ++j) {
-                            const uint32_t fj = wait_flag(fl + j, a.status);
-                            if (fj == 0) continue;
-                            const double* pj = pay + (size_t)j * kBwdPayload;
-                            const bool incl = (fj == kFlagInclusive);
-#pragma unroll
-                            for (int e = 0; e < 6; ++e) carry[e] += payload_load(pj + (incl ? 6 : 0) + e);
-                            if (incl) break;
-                        }
-#pragma unroll
-                        for (int e = 0; e < 6; ++e) payload_store(my + 6 + e, carry[e] + tot[e]);
-                        flag_store(fl + t, kFlagInclusive);
-                    }
+                for (int e = 0; e < 6; ++e) {
+                    const double tot = read_lane(C[e], 63);
+                    if (lane == 0) s_wsum[wave][e] = tot;
+                    C[e] = tot - C[e];
                 }
+            } else if (lane == 0) {
 #pragma unroll
-                for (int e = 0; e < 6; ++e) s_carry[e] = carry[e];
+                for (int e = 0; e < 6; ++e) s_wsum[wave][e] = 0.0;
             }
             __syncthreads();
+            for (int w = wave + 1; w < nw; ++w) {
 #pragma unroll
-            for (int e = 0; e < 6; ++e) C[e] += s_carry[e];
-        }
-        if (wave_active && cnt > 0) {
-            // Pass B: the lane's atoms again (from L2), plus chain atom
-            // 3(rl+R), the next lane's first.
-            LaneCoords<T, R> xc;
-            T gr[9 * R];
-            xc.load(xin + o, a.res ? a.res + ro : nullptr, a.res_lp, cnt);
-            lane_load<T, 9 * R>(gin + o, 9 * cnt, gr);
-            const bool has_next = rl + R < n;
-            double xn[3], gn[3];  // chain atom 3(rl+R)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const T xv = has_next ? xin[o + 9 * R + c] : (T)0;
-                gn[c] = has_next ? (double)gin[o + 9 * R + c] : 0.0;
-                if constexpr (sizeof(T) == 4)
-                    xn[c] = has_next ? res_apply((float)xv, a.res[ro + (size_t)c * a.res_lp + R]) : 0.0;
-                else
-                    xn[c] = (double)xv;
+                for (int e = 0; e < 6; ++e) C[e] += s_wsum[w][e];
             }
-            // From the lane's atom 3R down: Xc = atom j, Xp = atom j-1, Xa =
-            // atom j+1, S = sum over atoms after j of (x x g, g).  Atom 3R
-            // belongs to the next lane, whose sum is inside C.
-            double S[6], Xc[3], Xp[3], Xa[3];
-            {
-                double xg[3];
-                cross(xn, gn, xg);
-                S[0] = C[0] - xg[0]; S[1] = C[1] - xg[1]; S[2] = C[2] - xg[2];
-                S[3] = C[3] - gn[0]; S[4] = C[4] - gn[1]; S[5] = C[5] - gn[2];
-            }
+            if (whole) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) Xc[c] = xn[c];
-            xc.atom(3 * R - 1, Xp);
-            T out[3][R];
+                for (int e = 0; e < 6; ++e) C[e] += carry[e];
+                for (int w = 0; w < nw; ++w) {
 #pragma unroll
-            for (int j = 3 * R; j >= 1; --j) {
-                if (j < 3 * R) {
+                    for (int e = 0; e < 6; ++e) carry[e] += s_wsum[w][e];
+                }
+            } else if (a.ntb > 1) {  // then bflags, bpay, btickets, bdone are set (state)
+                if (tid == 0) {
+                    double tot[6] = {0, 0, 0, 0, 0, 0}, carry[6] = {0, 0, 0, 0, 0, 0};
+                    for (int w = 0; w < nw; ++w) {
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        Xa[c] = Xc[c];
-                        Xc[c] = Xp[c];
+                        for (int e = 0; e < 6; ++e) tot[e] += s_wsum[w][e];
                     }
-                    xc.atom(j - 1, Xp);
-                }
-                if (j == 3 * R - 1) {
+                    const int ntv = (n + TR - 1) / TR;  // tiles holding residues
+                    if (t < ntv) {
+                        const size_t s0 = (size_t)b * a.ntb;
+                        uint32_t* fl = a.bflags + s0;
+                        double* pay = a.bpay + s0 * kBwdPayload;
+                        double* my = pay + (size_t)t * kBwdPayload;
+                        if (t == ntv - 1) {
 #pragma unroll
-                    for (int e = 0; e < 6; ++e) S[e] = C[e];
-                } else if (j < 3 * R - 1) {
-                    const double g[3] = {(double)gr[3 * j + 3], (double)gr[3 * j + 4], (double)gr[3 * j + 5]};
+                            for (int e = 0; e < 6; ++e) payload_store(my + 6 + e, tot[e]);
+                            flag_store(fl + t, kFlagInclusive);
+                        } else {
+#pragma unroll
+                            for (int e = 0; e < 6; ++e) payload_store(my + e, tot[e]);
+                            flag_store(fl + t, kFlagAggregate);
+                            for (int j = t + 1; j < ntv; ++j) {
+                                const uint32_t fj = wait_flag(fl + j, a.status);
+                                if (fj == 0) continue;
+                                const double* pj = pay + (size_t)j * kBwdPayload;
+                                const bool incl = (fj == kFlagInclusive);
+#pragma unroll
+                                for (int e = 0; e < 6; ++e) carry[e] += payload_load(pj + (incl ? 6 : 0) + e);
+                                if (incl) break;
+                            }
+#pragma unroll
+                            for (int e = 0; e < 6; ++e) payload_store(my + 6 + e, carry[e] + tot[e]);
+                            flag_store(fl + t, kFlagInclusive);
+                        }
+                    }
+#pragma unroll
+                    for (int e = 0; e < 6; ++e) s_carry[e] = carry[e];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int e = 0; e < 6; ++e) C[e] += s_carry[e];
+            }
+            if (wave_active && cnt > 0) {
+                // Pass B: the lane's atoms again (from L2), plus chain atom
+                // 3(rl+R), the next lane's first.
+                LaneCoords<T, R> xc;
+                T gr[9 * R];
+                xc.load(xin + o, gin + o, gr, a.res ? a.res + (size_t)b * 9 * a.res_lp : nullptr,
+                        a.res_lp, rl, cnt, L - rl);
+                const bool has_next = rl + R < n;
+                double xn[3], gn[3];  // chain atom 3(rl+R)
+                // Without a next atom the loads read the lane's own first atom
+                // (valid here, cnt > 0) and are discarded: no branch.
+                const size_t on = has_next ? o + 9 * R : o;
+                [[maybe_unused]] const size_t rn = has_next ? ro + R : ro;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const T xv = xin[on + c];
+                    const T gv = gin[on + c];
+                    double xd;
+                    if constexpr (sizeof(T) == 4)
+                        xd = res_apply((float)xv, a.res[rn + (size_t)c * a.res_lp]);
+                    else
+                        xd = (double)xv;
+                    xn[c] = has_next ? xd : 0.0;
+                    gn[c] = has_next ? (double)gv : 0.0;
+                }
+                // From the lane's atom 3R down: Xc = atom j, Xp = atom j-1, Xa =
+                // atom j+1, S = sum over atoms after j of (x x g, g).  Atom 3R
+                // belongs to the next lane, whose sum is inside C.
+                double S[6], Xc[3], Xp[3], Xa[3];
+                {
                     double xg[3];
-                    cross(Xa, g, xg);
-                    S[0] += xg[0]; S[1] += xg[1]; S[2] += xg[2];
-                    S[3] += g[0]; S[4] += g[1]; S[5] += g[2];
+                    cross(xn, gn, xg);
+                    S[0] = C[0] - xg[0]; S[1] = C[1] - xg[1]; S[2] = C[2] - xg[2];
+                    S[3] = C[3] - gn[0]; S[4] = C[4] - gn[1]; S[5] = C[5] - gn[2];
                 }
-                const int k = (j - 1) % 3, q = (j - 1) / 3;
-                double u[3], xf[3];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) u[c] = (Xc[c] - Xp[c]) * kInvD[k];
-                cross(Xc, S + 3, xf);
-                const double v = u[0] * (S[0] - xf[0]) + u[1] * (S[1] - xf[1]) + u[2] * (S[2] - xf[2]);
-                const int r = rl + q;
-                const bool zero = (r >= n) || (r == n - 1 && k > 0);
-                out[k][q] = zero ? (T)0 : (T)v;
+                for (int c = 0; c < 3; ++c) Xc[c] = xn[c];
+                xc.atom(3 * R - 1, Xp);
+                T out[3][R];
+#pragma unroll
+                for (int j = 3 * R; j >= 1; --j) {
+                    if (j < 3 * R) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            Xa[c] = Xc[c];
+                            Xc[c] = Xp[c];
+                        }
+                        xc.atom(j - 1, Xp);
+                    }
+                    if (j == 3 * R - 1) {
+#pragma unroll
+                        for (int e = 0; e < 6; ++e) S[e] = C[e];
+                    } else if (j < 3 * R - 1) {
+                        const double g[3] = {(double)gr[3 * j + 3], (double)gr[3 * j + 4], (double)gr[3 * j + 5]};
+                        double xg[3];
+                        cross(Xa, g, xg);
+                        S[0] += xg[0]; S[1] += xg[1]; S[2] += xg[2];
+                        S[3] += g[0]; S[4] += g[1]; S[5] += g[2];
+                    }
+                    const int k = (j - 1) % 3, q = (j - 1) / 3;
+                    double u[3], xf[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) u[c] = (Xc[c] - Xp[c]) * kInvD[k];
+                    cross(Xc, S + 3, xf);
+                    const double v = u[0] * (S[0] - xf[0]) + u[1] * (S[1] - xf[1]) + u[2] * (S[2] - xf[2]);
+                    const int r = rl + q;
+                    const bool zero = (r >= n) || (r == n - 1 && k > 0);
+                    out[k][q] = zero ? (T)0 : (T)v;
+                }
+                bool bad = false;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+#pragma unroll
+                    for (int q = 0; q < R; ++q) bad |= (out[k][q] != out[k][q]);
+                    lane_store<T, R>(ga_l + (size_t)k * L, crow, out[k]);
+                }
+                if (bad) set_status(a.status, TPL_A2B_STATUS_NAN);
             }
-            bool bad = false;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-#pragma unroll
-                for (int q = 0; q < R; ++q) bad |= (out[k][q] != out[k][q]);
-                lane_store<T, R>(ga_l + (size_t)k * L, crow, out[k]);
-            }
-            if (bad) set_status(a.status, TPL_A2B_STATUS_NAN);
-        }
 
+        }  // tiles of a whole chain
         // Last tile of the chain to finish returns its look-back words to 0.
-        if (a.ntb > 1 && tid == 0) {
+        if (!whole && a.ntb > 1 && tid == 0) {
             if (atomic_inc(a.bdone + b) == (uint32_t)(a.ntb - 1)) {
                 for (int j = 0; j < a.ntb; ++j) word_store(a.bflags + (size_t)b * a.ntb + j, 0u);
                 word_store(a.btickets + b, 0u);
@@ -1268,6 +1362,7 @@ StateLayout state_layout(int dtype, int B, int L, int ntb) {
 using KernelFn = void (*)(ScanArgs);
 
 constexpr int kMaxThreads = 256;  // up to 4 waves per workgroup
+constexpr int kWholeMaxTiles = 4, kWholeMinChains = 512;
 
 template <typename T, int R>
 KernelFn pick_kernel(bool fwd) {
@@ -1322,7 +1417,11 @@ int launch_scan(bool fwd, int dtype, ScanArgs args, void* state, void* sync,
                            (int)sl.words);
         if (hipGetLastError() != hipSuccess) return TPL_A2B_ERR_LAUNCH;
     }
-    const int nt = fwd ? tl.ntf : tl.ntb;
+    // Backward: with few tiles per chain and enough chains to fill the device,
+    // one workgroup takes a whole chain (no tickets, no look-back waits);
+    // otherwise one workgroup per tile.
+    args.bwhole = (!fwd && tl.ntb > 1 && tl.ntb <= kWholeMaxTiles && args.B >= kWholeMinChains) ? 1 : 0;
+    const int nt = fwd ? tl.ntf : (args.bwhole ? 1 : tl.ntb);
     int grid = args.B * nt;
     if (fwd && cfg->max_wgs > 0) grid = std::min(grid, cfg->max_wgs);
     const int threads = 64 * cfg->waves;

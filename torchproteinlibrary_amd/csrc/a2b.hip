@@ -589,28 +589,41 @@ __device__ __forceinline__ void cross(const double a[3], const double b[3], doub
 }
 
 // Wave-wide store of the run dst[0, count) in global memory: element f <
-// nsrc comes from the wave's LDS staging (lane f / E, element f % E, lane
-// stride E + 1), the rest is 0.  The aligned body moves in 16-byte stores,
-// lane-consecutive, so every store instruction of the wave writes 1 KiB.
-template <typename T, int E>
+// nsrc is s_w[f], the wave's LDS staging in output order; the rest is 0.  The
+// aligned body moves in 16-byte LDS reads and 16-byte global stores,
+// lane-consecutive, so every store instruction of the wave writes 1 KiB.  A
+// wave with nothing staged (nsrc == 0, uniform) reads nothing.
+template <typename T>
 __device__ __forceinline__ void wave_store_run(T* dst, int count, int nsrc, const T* s_w, int lane) {
     constexpr int V = 16 / (int)sizeof(T);
-    auto elem = [&](int f) -> T { return f < nsrc ? s_w[(f / E) * (E + 1) + f % E] : (T)0; };
+    const bool staged = nsrc > 0;
     const int mis = (int)((reinterpret_cast<uintptr_t>(dst) / sizeof(T)) % V);
     const int head = min((V - mis) % V, count);
     const int nvec = (count - head) / V;
     const int tail0 = head + nvec * V;
-    if (lane < head) dst[lane] = elem(lane);
+    if (lane < head) dst[lane] = (staged && lane < nsrc) ? s_w[lane] : (T)0;
     for (int v = lane; v < nvec; v += 64) {
         const int f0 = head + v * V;
         T w[V];
+        if (staged) {
+            if (head == 0) {  // uniform: 16-byte aligned in LDS too
+                const uint4 u = *reinterpret_cast<const uint4*>(s_w + f0);
+                __builtin_memcpy(w, &u, sizeof(u));
+            } else {
 #pragma unroll
-        for (int e = 0; e < V; ++e) w[e] = elem(f0 + e);
+                for (int e = 0; e < V; ++e) w[e] = s_w[f0 + e];
+            }
+#pragma unroll
+            for (int e = 0; e < V; ++e) w[e] = f0 + e < nsrc ? w[e] : (T)0;
+        } else {
+#pragma unroll
+            for (int e = 0; e < V; ++e) w[e] = (T)0;
+        }
         uint4 u;
         __builtin_memcpy(&u, w, sizeof(u));
         *reinterpret_cast<uint4*>(dst + f0) = u;
     }
-    if (tail0 + lane < count) dst[tail0 + lane] = elem(tail0 + lane);
+    if (tail0 + lane < count) dst[tail0 + lane] = (staged && tail0 + lane < nsrc) ? s_w[tail0 + lane] : (T)0;
 }
 
 // Wave-scope LDS ordering: the wave's earlier LDS writes are visible to its
@@ -639,13 +652,13 @@ __device__ __forceinline__ void wave_lds_sync() {
 template <typename T, int R, int MAXT>
 __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
-    constexpr int E9 = 9 * R;  // output floats per lane; LDS lane stride E9 + 1
+    constexpr int E9 = 9 * R;  // output floats per lane = the lane's LDS stride
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nw = blockDim.x >> 6;
     const int TR = blockDim.x * R;
     T* s_x = reinterpret_cast<T*>(s_dyn);
-    T* s_l = s_x + tid * (E9 + 1);            // this lane's slot
-    const T* s_w = s_x + wave * 64 * (E9 + 1);  // this wave's staging
+    T* s_l = s_x + tid * E9;              // this lane's slot
+    const T* s_w = s_x + wave * 64 * E9;  // this wave's staging, in output order
     __shared__ double s_tab[128];
     __shared__ double s_tot[4][kF];
     __shared__ double s_pfx[kF];
@@ -837,9 +850,9 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
                 if (!__builtin_isfinite(chk)) set_status(a.status, TPL_A2B_STATUS_NAN);
             }
             wave_lds_sync();
-            wave_store_run<T, E9>(x_b + g0, wcount, min(max(9 * n - g0, 0), wcount), s_w, lane);
+            wave_store_run<T>(x_b + g0, wcount, min(max(9 * n - g0, 0), wcount), s_w, lane);
         } else if (wcount > 0) {
-            wave_store_run<T, E9>(x_b + g0, wcount, 0, s_w, lane);
+            wave_store_run<T>(x_b + g0, wcount, 0, s_w, lane);
         }
         if (ahead) {
 #pragma unroll
@@ -1428,8 +1441,8 @@ int launch_scan(bool fwd, int dtype, ScanArgs args, void* state, void* sync,
     const int r = fwd ? cfg->res_per_lane : tl.rb;
     const bool f32 = dtype == TPL_A2B_F32;
     KernelFn k = f32 ? pick_kernel<float>(fwd, r) : pick_kernel<double>(fwd, r);
-    // The forward stages 9R coordinates per lane in LDS (lane stride 9R + 1).
-    const size_t lds = fwd ? (size_t)(9 * r + 1) * (f32 ? 4 : 8) * threads : 0;
+    // The forward stages each lane's 9R coordinates in LDS, in output order.
+    const size_t lds = fwd ? (size_t)(9 * r) * (f32 ? 4 : 8) * threads : 0;
     hipLaunchKernelGGL(k, dim3(grid), dim3(threads), lds, stream, args);
     return hipGetLastError() == hipSuccess ? TPL_A2B_OK : TPL_A2B_ERR_LAUNCH;
 }

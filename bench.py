@@ -15,6 +15,8 @@ second).  The line also carries:
   the stream it runs on; algorithmic bytes per launch (``alg_bytes``) / average
   duration against the MI355X 8 TB/s HBM peak, and the HBM bytes per launch the
   counters measured (``profiles/traffic_<workload>.json``);
+* ``graph``: the same step replayed from HIP graphs (one captured forward +
+  backward per buffer set);
 * ``plain``: the same step with the plain kernels (serial forward, O(L^2)
   backward) and the speedup of the scan path over it;
 * ``cpu_baseline``: the reference CPU algorithm on a bounded sample of the same
@@ -309,6 +311,11 @@ def main():
             "kernels": kern,
         },
     }
+
+    # The same step replayed from HIP graphs (one captured fwd+bwd per buffer set).
+    gms, _ = ext.bench_graph([list(k) for k in ksets], list(cfg), args.steps, args.warmup)
+    out["graph"] = {"value": residues * args.steps / (gms / 1e3), "ms_per_step": gms / args.steps,
+                    "launch": "hipGraph replay, one captured fwd+bwd per buffer set"}
 
     if not args.no_plain:
         fr_b = ext.plain_frames_bytes(False, B, L)

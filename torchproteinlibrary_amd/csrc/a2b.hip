@@ -954,7 +954,6 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
         double carry[6] = {0, 0, 0, 0, 0, 0};  // whole: sum over the later tiles
         for (int step = 0; step < (whole ? a.ntb : 1); ++step) {
             const int t = whole ? a.ntb - 1 - step : t0;
-            if (whole) __syncthreads();  // s_wsum of the previous tile no longer read
             const int r0 = t * TR;
             const int rl = r0 + tid * R;
             const int cnt = min(max(n - rl, 0), R);
@@ -964,6 +963,10 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) lane_store_zero<T, R>(ga_l + (size_t)k * L, crow);
             }
+            // A walked tile wholly past the chain only writes its zeros
+            // (uniform; it adds nothing to the carried sum).
+            if (whole && r0 >= n) continue;
+            if (whole) __syncthreads();  // s_wsum of the previous tile no longer read
             const bool wave_active = r0 + wave * 64 * R < n;  // uniform per wave
 
             // Pass A: W = sum over the lane's atoms 0 .. 3R-1 (chain atoms

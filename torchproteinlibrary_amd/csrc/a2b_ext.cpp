@@ -11,6 +11,7 @@
 #include <c10/hip/HIPStream.h>
 
 #include <chrono>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -281,6 +282,93 @@ std::tuple<double, double> bench_steps(std::vector<std::vector<torch::Tensor>> s
     return {(double)ms, std::chrono::duration<double, std::milli>(t1 - t0).count()};
 }
 
+// The same step captured as a HIP graph: for each buffer set, one forward and
+// one backward launch are captured on a private stream (thread-local capture
+// mode) and instantiated; the graphs are then replayed back to back on the
+// current stream, `warmup` untimed and `steps` timed.  Returns (event ms,
+// wall ms).  Nothing is allocated or cleared inside the captured work.
+std::tuple<double, double> bench_graph(std::vector<std::vector<torch::Tensor>> sets,
+                                       std::vector<int64_t> config, int64_t steps,
+                                       int64_t warmup) {
+    TORCH_CHECK(!sets.empty() && steps >= 0 && warmup >= 0, "bench_graph: bad arguments");
+    for (auto& s : sets) TORCH_CHECK(s.size() == 7, "bench_graph: a set has 7 tensors");
+    const tpl_a2b_config cfg = to_config(config);
+    const torch::Tensor& a0 = sets[0][0];
+    auto [B, L] = check_inputs(a0, sets[0][1]);
+    const int dt = dtype_code(a0);
+    c10::DeviceGuard guard(a0.device());
+    for (auto& s : sets) {
+        check_inputs(s[0], s[1]);
+        check_out(s[2], a0, {B, 9 * (int64_t)L}, "gradOutput");
+        check_out(s[3], a0, {B, 9 * (int64_t)L}, "output_coords");
+        check_out(s[4], a0, {B, 3, L}, "gradInput");
+        opt_ptr(s[5], tpl_a2b_state_bytes(dt, B, L, &cfg), "state", a0);
+        opt_ptr(s[6], tpl_a2b_sync_bytes(dt, B, L, &cfg), "sync", a0);
+    }
+    hipStream_t stream = stream_of(a0);
+    hipStream_t cs;
+    TORCH_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) == hipSuccess,
+                "hipStreamCreateWithFlags failed");
+    std::vector<hipGraphExec_t> execs;
+    std::string err;
+    for (auto& s : sets) {
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            err = "hipStreamBeginCapture failed";
+            break;
+        }
+        int rc = tpl_a2b_forward(dt, s[0].data_ptr(), s[3].data_ptr(), s[1].data_ptr<int32_t>(),
+                                 s[5].data_ptr(), s[6].data_ptr(), nullptr, B, L, &cfg, cs);
+        if (rc == TPL_A2B_OK)
+            rc = tpl_a2b_backward(dt, s[4].data_ptr(), s[2].data_ptr(), s[3].data_ptr(),
+                                  s[1].data_ptr<int32_t>(), s[5].data_ptr(), nullptr, B, L, &cfg,
+                                  cs);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(cs, &g);
+        if (rc != TPL_A2B_OK || ec != hipSuccess || !g) {
+            err = rc != TPL_A2B_OK ? tpl_a2b_strerror(rc) : "hipStreamEndCapture failed";
+            if (g) (void)hipGraphDestroy(g);
+            break;
+        }
+        hipGraphExec_t e = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            err = "hipGraphInstantiate failed";
+            break;
+        }
+        execs.push_back(e);
+    }
+    auto cleanup = [&]() {
+        for (auto e : execs) (void)hipGraphExecDestroy(e);
+        (void)hipStreamDestroy(cs);
+    };
+    if (!err.empty()) {
+        cleanup();
+        TORCH_CHECK(false, "bench_graph: ", err);
+    }
+    auto replay = [&](int64_t i) {
+        TORCH_CHECK(hipGraphLaunch(execs[i % execs.size()], stream) == hipSuccess,
+                    "hipGraphLaunch failed");
+    };
+    for (int64_t i = 0; i < warmup; ++i) replay(i);
+    hipEvent_t e0, e1;
+    TORCH_CHECK(hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess,
+                "hipEventCreate failed");
+    TORCH_CHECK(hipStreamSynchronize(stream) == hipSuccess, "hipStreamSynchronize failed");
+    const auto t0 = std::chrono::steady_clock::now();
+    TORCH_CHECK(hipEventRecord(e0, stream) == hipSuccess, "hipEventRecord failed");
+    for (int64_t i = 0; i < steps; ++i) replay(i);
+    TORCH_CHECK(hipEventRecord(e1, stream) == hipSuccess, "hipEventRecord failed");
+    TORCH_CHECK(hipEventSynchronize(e1) == hipSuccess, "hipEventSynchronize failed");
+    const auto t1 = std::chrono::steady_clock::now();
+    float ms = 0.f;
+    TORCH_CHECK(hipEventElapsedTime(&ms, e0, e1) == hipSuccess, "hipEventElapsedTime failed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    cleanup();
+    return {(double)ms, std::chrono::duration<double, std::milli>(t1 - t0).count()};
+}
+
 }  // namespace
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
@@ -294,4 +382,5 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("default_config", &default_config, "untuned config for (batch, max_length)");
     m.def("tune", &tune, "time candidate configs; returns (best, ms per candidate)");
     m.def("bench_steps", &bench_steps, "time back-to-back fwd+bwd steps (event ms, wall ms)");
+    m.def("bench_graph", &bench_graph, "time fwd+bwd steps replayed from HIP graphs (event ms, wall ms)");
 }

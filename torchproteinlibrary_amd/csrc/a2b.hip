@@ -649,7 +649,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 //      their residual planes;
 //   5. each wave stores its 9*64*R-float output span with 16-byte stores from
 //      LDS, zeros past the chain.
-template <typename T, int R, int MAXT>
+// LB: chains span several tiles (look-back over tiles); AHEAD: one tile per
+// chain on a grid smaller than the batch (load the next chain ahead).  The
+// launch picks the variant, so each carries only the state it needs.
+template <typename T, int R, int MAXT, bool LB, bool AHEAD>
 __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     constexpr int E9 = 9 * R;  // output floats per lane = the lane's LDS stride
@@ -675,7 +678,7 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
     // tiles per chain the tile comes from the chain's ticket at the start of
     // the item, so nothing is loaded ahead.
     const int total = a.B * a.ntf;
-    const bool ahead = a.ntf == 1;
+    constexpr bool ahead = AHEAD && !LB;
     T ang[3][R];  // this lane's angles of the current item (rows phi, psi, omega)
     int n_ahead = 0;
     auto load_ahead = [&](int it, T (&dst)[3][R]) {
@@ -691,12 +694,12 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
         __syncthreads();  // s_tab written; LDS of the previous tile no longer read
         const int b = item % a.B;
         int t = 0, n;
-        if (ahead) {
+        if constexpr (ahead) {
             n = n_ahead;
             if (tid == 0) clamp_length(a, a.lengths[b], true);  // reports a bad length
         } else {
             const int n_raw = a.lengths[b];  // issued before the ticket round trip
-            item_tile(b, a.ntf, a.ftickets, false, &s_tile, &t);
+            if constexpr (LB) item_tile(b, a.ntf, a.ftickets, false, &s_tile, &t);
             n = clamp_length(a, n_raw, t == 0 && tid == 0);
         }
         const int r0 = t * TR;
@@ -757,7 +760,7 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
             for (int e = 0; e < kF; ++e) s_tot[wave][e] = ident(e);
         }
         __syncthreads();
-        if (a.ntf > 1) {
+        if constexpr (LB) {
             if (tid == 0) {
                 const size_t s0 = (size_t)b * a.ntf;
                 uint32_t* fl = a.fflags + s0;
@@ -804,7 +807,7 @@ __global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
         const int wcount = min(9 * 64 * R, 9 * L - g0);
         if (wave_active) {
             if (cnt > 0) {
-                lane_prefix(s_tot, wave, a.ntf > 1 ? s_pfx : nullptr, E);
+                lane_prefix(s_tot, wave, LB ? s_pfx : nullptr, E);
                 // Pass 2, in place in the lane's slot.  Residuals go to plane
                 // m at residue rl + q as soon as a pair (q even, q + 1) is
                 // complete (one half when R = 1): the lane's R halves per
@@ -1380,18 +1383,24 @@ using KernelFn = void (*)(ScanArgs);
 constexpr int kMaxThreads = 256;  // up to 4 waves per workgroup
 constexpr int kWholeMaxTiles = 4, kWholeMinChains = 512;
 
+// Forward variants: (LB, AHEAD) = (1, 0) several tiles per chain, (0, 0) one
+// tile per chain and one workgroup each, (0, 1) one tile per chain on a grid
+// smaller than the batch.
 template <typename T, int R>
-KernelFn pick_kernel(bool fwd) {
-    return fwd ? a2b_fwd_scan<T, R, kMaxThreads> : a2b_bwd_scan<T, R, kMaxThreads>;
+KernelFn pick_kernel(bool fwd, bool lb, bool ahead) {
+    if (!fwd) return a2b_bwd_scan<T, R, kMaxThreads>;
+    if (lb) return a2b_fwd_scan<T, R, kMaxThreads, true, false>;
+    return ahead ? a2b_fwd_scan<T, R, kMaxThreads, false, true>
+                 : a2b_fwd_scan<T, R, kMaxThreads, false, false>;
 }
 
 template <typename T>
-KernelFn pick_kernel(bool fwd, int r) {
+KernelFn pick_kernel(bool fwd, int r, bool lb, bool ahead) {
     switch (r) {
-        case 1: return pick_kernel<T, 1>(fwd);
-        case 2: return pick_kernel<T, 2>(fwd);
-        case 4: return pick_kernel<T, 4>(fwd);
-        default: return pick_kernel<T, 8>(fwd);
+        case 1: return pick_kernel<T, 1>(fwd, lb, ahead);
+        case 2: return pick_kernel<T, 2>(fwd, lb, ahead);
+        case 4: return pick_kernel<T, 4>(fwd, lb, ahead);
+        default: return pick_kernel<T, 8>(fwd, lb, ahead);
     }
 }
 
@@ -1443,7 +1452,9 @@ int launch_scan(bool fwd, int dtype, ScanArgs args, void* state, void* sync,
     const int threads = 64 * cfg->waves;
     const int r = fwd ? cfg->res_per_lane : tl.rb;
     const bool f32 = dtype == TPL_A2B_F32;
-    KernelFn k = f32 ? pick_kernel<float>(fwd, r) : pick_kernel<double>(fwd, r);
+    const bool lb = fwd && tl.ntf > 1;
+    const bool ahead = fwd && !lb && grid < args.B;
+    KernelFn k = f32 ? pick_kernel<float>(fwd, r, lb, ahead) : pick_kernel<double>(fwd, r, lb, ahead);
     // The forward stages each lane's 9R coordinates in LDS, in output order.
     const size_t lds = fwd ? (size_t)(9 * r) * (f32 ? 4 : 8) * threads : 0;
     hipLaunchKernelGGL(k, dim3(grid), dim3(threads), lds, stream, args);

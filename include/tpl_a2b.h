@@ -69,8 +69,7 @@ enum tpl_a2b_error {
  * Every lane keeps its residues' inputs and outputs in registers, which bounds
  * the tile: at most 1024 residues for TPL_A2B_F32 and 512 for TPL_A2B_F64
  * (larger tiles return TPL_A2B_ERR_CONFIG).  The backward holds more per
- * residue and runs min(res_per_lane, 2) (F32) or 1 (F64) residues per lane
- * with the same waves. */
+ * residue and runs one residue per lane with the same waves. */
 typedef struct tpl_a2b_config {
     int32_t waves;        /* 64-lane waves per workgroup: 1, 2 or 4 */
     int32_t res_per_lane; /* residues composed serially per lane: 1, 2, 4 or 8 */

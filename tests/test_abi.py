@@ -45,14 +45,14 @@ def test_config_validation_and_sizes(lib):
     def sync(B, ntf):  # flags + tickets (zeroed), then 24-double payload per tile
         return 0 if ntf <= 1 else align(4 * (B * ntf + B)) + B * ntf * 24 * 8
 
-    # (4,4): forward tiles of 1024; the backward runs 2 residues per lane (f32),
-    # so 512-residue tiles
-    assert lib.tpl_a2b_state_bytes(0, 1024, 1000, ctypes.byref(c)) == state(True, 1024, 1000, 2)
+    # (4,4): forward tiles of 1024; the backward runs 1 residue per lane, so
+    # 256-residue tiles
+    assert lib.tpl_a2b_state_bytes(0, 1024, 1000, ctypes.byref(c)) == state(True, 1024, 1000, 4)
     assert lib.tpl_a2b_sync_bytes(0, 1024, 1000, ctypes.byref(c)) == 0
-    assert lib.tpl_a2b_state_bytes(0, 64, 16384, ctypes.byref(c)) == state(True, 64, 16384, 32)
+    assert lib.tpl_a2b_state_bytes(0, 64, 16384, ctypes.byref(c)) == state(True, 64, 16384, 64)
     assert lib.tpl_a2b_sync_bytes(0, 64, 16384, ctypes.byref(c)) == sync(64, 16)
-    c2 = abi.Config(2, 4, 0, 0)  # forward 512, backward 256 (f32) / 128 (f64) residues
-    assert lib.tpl_a2b_state_bytes(0, 512, 500, ctypes.byref(c2)) == state(True, 512, 500, 2)
+    c2 = abi.Config(2, 4, 0, 0)  # forward 512, backward 128 residues
+    assert lib.tpl_a2b_state_bytes(0, 512, 500, ctypes.byref(c2)) == state(True, 512, 500, 4)
     assert lib.tpl_a2b_sync_bytes(0, 512, 500, ctypes.byref(c2)) == 0
     assert lib.tpl_a2b_state_bytes(1, 512, 500, ctypes.byref(c2)) == state(False, 512, 500, 4)
     c1 = abi.Config(4, 1, 0, 0)  # 256 both ways: f64 needs no state

@@ -89,13 +89,16 @@ def test_ragged_batch_all_configs(cfg):
 def test_backward_whole_chain_walk():
     """B >= 512 chains of at most 4 backward tiles: one workgroup walks each
     chain's tiles from the last to the first (no look-back); lengths cover
-    every tile count and the boundaries of tiles and lanes."""
+    every tile count and the boundaries of tiles and lanes.  With more tiles
+    per chain the same batch takes the look-back path."""
     rng = np.random.default_rng(7)
     n = rng.integers(0, 1001, size=512)
     n[:8] = [1000, 999, 768, 767, 513, 256, 255, 1]
     a, n, g = synthetic.custom(n, 1000, seed=8)
     x64, g64 = oracle(a, n, g)
-    for cfg in [(2, 2, 0), (4, 2, 0), (4, 4, 0)]:  # 4, 2, 2 backward tiles per chain
+    # (4,1,0), (4,4,0): 4 backward tiles of 256 per chain, walked by one
+    # workgroup; (2,2,0): 8 tiles of 128, chained by look-back instead
+    for cfg in [(4, 1, 0), (4, 4, 0), (2, 2, 0)]:
         x, ga = run(a, n, g, cfg)
         assert np.abs(x - x64).max() <= coord_tol(x64)
         assert grad_rel_err(ga, g64, n) <= 1e-5

@@ -921,8 +921,9 @@ struct LaneCoords {
 // sums its own atoms 3rl .. 3rl+3R-1 and writes the gradients of its residues
 // (atoms 3rl+1 .. 3rl+3R, the last one being the next lane's first).  Steps:
 //   1. lanes wholly past the chain write their zero gradients at once; the
-//      others load their coordinates (FP64, with the residuals for FP32) and
-//      coordinate gradients into registers, zero past the chain;
+//      others load their coordinates and coordinate gradients (and, for FP32,
+//      the residuals) into registers, zero past the chain, and keep them there
+//      until pass B;
 //   2. per lane, W = sum over its atoms of (x x g, g);
 //   3. suffix sums of W over lanes (DPP), waves (LDS) and tiles (reverse
 //      look-back) give C = the sum over all later atoms;
@@ -977,9 +978,9 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
             const size_t o = (size_t)b * 9 * L + (size_t)9 * rl;
             const size_t ro = (size_t)b * 9 * a.res_lp + (size_t)rl;  // plane 0, residue rl
             double W[6] = {0, 0, 0, 0, 0, 0};
+            LaneCoords<T, R> xc;
+            T gr[9 * R];
             if (wave_active) {
-                LaneCoords<T, R> xc;
-                T gr[9 * R];
                 xc.load(xin + o, gin + o, gr, a.res ? a.res + (size_t)b * 9 * a.res_lp : nullptr,
                         a.res_lp, rl, cnt, L - rl);
 #pragma unroll
@@ -1063,12 +1064,8 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
                 for (int e = 0; e < 6; ++e) C[e] += s_carry[e];
             }
             if (wave_active && cnt > 0) {
-                // Pass B: the lane's atoms again (from L2), plus chain atom
-                // 3(rl+R), the next lane's first.
-                LaneCoords<T, R> xc;
-                T gr[9 * R];
-                xc.load(xin + o, gin + o, gr, a.res ? a.res + (size_t)b * 9 * a.res_lp : nullptr,
-                        a.res_lp, rl, cnt, L - rl);
+                // Pass B, on the lane's atoms still in registers from pass A,
+                // plus chain atom 3(rl+R), the next lane's first, from memory.
                 const bool has_next = rl + R < n;
                 double xn[3], gn[3];  // chain atom 3(rl+R)
                 // Without a next atom the loads read the lane's own first atom
@@ -1306,10 +1303,12 @@ bool valid_config(const tpl_a2b_config* c) {
     return wok && rok && c->max_wgs >= 0 && c->flags == 0;
 }
 
-// Largest tiles, and the backward's residues per lane: bounds that keep
-// each lane's runs in registers (no spills) at the shapes the tuner tries.
+// Largest tiles (each lane's runs stay in registers, no spills), and the
+// backward's residues per lane: one, so that a lane keeps its coordinates,
+// gradients and residuals in registers from pass A to pass B (about 100
+// VGPRs, four or more waves per SIMD).
 constexpr int kMaxTileF32 = 1024, kMaxTileF64 = 512;
-constexpr int kMaxBwdRF32 = 2, kMaxBwdRF64 = 1;
+constexpr int kBwdR = 1;
 
 bool fits_dtype(int dtype, const tpl_a2b_config* c) {
     const int tr = 64 * c->waves * c->res_per_lane;
@@ -1317,7 +1316,7 @@ bool fits_dtype(int dtype, const tpl_a2b_config* c) {
 }
 
 // Tiling of one call: forward tiles of trf = 64*W*R residues; backward tiles
-// of trb = 64*W*rb with rb = min(R, the backward's bound), a divisor of trf.
+// of trb = 64*W*kBwdR residues, a divisor of trf.
 struct Tiling {
     int trf, ntf, trb, ntb, rb;
 };
@@ -1325,7 +1324,7 @@ struct Tiling {
 Tiling tiling(int dtype, int L, const tpl_a2b_config* c) {
     Tiling t{};
     t.trf = 64 * c->waves * c->res_per_lane;
-    t.rb = std::min(c->res_per_lane, dtype == TPL_A2B_F32 ? kMaxBwdRF32 : kMaxBwdRF64);
+    t.rb = kBwdR;
     t.trb = 64 * c->waves * t.rb;
     t.ntf = L <= 0 ? 0 : (L + t.trf - 1) / t.trf;
     t.ntb = L <= 0 ? 0 : (L + t.trb - 1) / t.trb;
@@ -1388,7 +1387,7 @@ constexpr int kWholeMaxTiles = 4, kWholeMinChains = 512;
 // smaller than the batch.
 template <typename T, int R>
 KernelFn pick_kernel(bool fwd, bool lb, bool ahead) {
-    if (!fwd) return a2b_bwd_scan<T, R, kMaxThreads>;
+    if (!fwd) return a2b_bwd_scan<T, kBwdR, kMaxThreads>;
     if (lb) return a2b_fwd_scan<T, R, kMaxThreads, true, false>;
     return ahead ? a2b_fwd_scan<T, R, kMaxThreads, false, true>
                  : a2b_fwd_scan<T, R, kMaxThreads, false, false>;

@@ -242,13 +242,38 @@ __device__ __forceinline__ double read_lane(double v, int lane) {
 constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
 constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143, kWaveShr1 = 0x138;
 
+// DPP move of a double where lanes that receive nothing read 0
+// (bound_ctrl); needs no `old` value, hence no register initialisation.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move_zero(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, true);
+    return dbl_from(lo, hi);
+}
+
 // One Hillis-Steele step: f <- (frame of the source lane) * f; lanes that
-// receive nothing compose with the identity.
+// receive nothing compose with the identity.  For the row shifts (every row
+// enabled) the zero elements of the identity come from bound_ctrl and only
+// the high words of the three diagonal 1.0s need an `old` value.
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ void frame_scan_step(double f[kF]) {
     double g[kF];
+    if constexpr (ROW_MASK == 0xf) {
 #pragma unroll
-    for (int e = 0; e < kF; ++e) g[e] = dpp_move<CTRL, ROW_MASK>(f[e], ident(e));
+        for (int e = 0; e < kF; ++e) {
+            if (ident(e) == 0.0) {
+                g[e] = dpp_move_zero<CTRL>(f[e]);
+            } else {
+                const int lo = __builtin_amdgcn_mov_dpp(__double2loint(f[e]), CTRL, 0xf, 0xf, true);
+                const int hi = __builtin_amdgcn_update_dpp(__double2hiint(1.0), __double2hiint(f[e]),
+                                                           CTRL, 0xf, 0xf, false);
+                g[e] = dbl_from(lo, hi);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < kF; ++e) g[e] = dpp_move<CTRL, ROW_MASK>(f[e], ident(e));
+    }
     frame_lmul(g, f);
 }
 
@@ -270,16 +295,17 @@ __device__ __forceinline__ void wave_shift_frames(double f[kF]) {
     for (int e = 0; e < kF; ++e) f[e] = dpp_move<kWaveShr1, 0xf>(f[e], ident(e));
 }
 
-// Inclusive prefix sum over the wave of N doubles.
+// Inclusive prefix sum over the wave of N doubles (row shifts zero-fill
+// through bound_ctrl).
 template <int N>
 __device__ __forceinline__ void wave_scan_sum(double v[N]) {
 #pragma unroll
     for (int e = 0; e < N; ++e) {
         double x = v[e];
-        x += dpp_move<kRowShr1, 0xf>(x, 0.0);
-        x += dpp_move<kRowShr2, 0xf>(x, 0.0);
-        x += dpp_move<kRowShr4, 0xf>(x, 0.0);
-        x += dpp_move<kRowShr8, 0xf>(x, 0.0);
+        x += dpp_move_zero<kRowShr1>(x);
+        x += dpp_move_zero<kRowShr2>(x);
+        x += dpp_move_zero<kRowShr4>(x);
+        x += dpp_move_zero<kRowShr8>(x);
         x += dpp_move<kRowBcast15, 0xa>(x, 0.0);
         x += dpp_move<kRowBcast31, 0xc>(x, 0.0);
         v[e] = x;

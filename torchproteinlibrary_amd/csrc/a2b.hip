@@ -614,11 +614,16 @@ __device__ __forceinline__ void cross(const double a[3], const double b[3], doub
     c[2] = a[0] * b[1] - a[1] * b[0];
 }
 
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 // Wave-wide store of the run dst[0, count) in global memory: element f <
 // nsrc is s_w[f], the wave's LDS staging in output order; the rest is 0.  The
 // aligned body moves in 16-byte LDS reads and 16-byte global stores,
 // lane-consecutive, so every store instruction of the wave writes 1 KiB.  A
-// wave with nothing staged (nsrc == 0, uniform) reads nothing.
+// wave with nothing staged (nsrc == 0, uniform) reads nothing.  The body's
+// stores are nontemporal (streaming): this kernel never reads the
+// coordinates back, and the forward at B=1024, L=1000 measured 20.0 us
+// with them against 22.0 us with ordinary stores.
 template <typename T>
 __device__ __forceinline__ void wave_store_run(T* dst, int count, int nsrc, const T* s_w, int lane) {
     constexpr int V = 16 / (int)sizeof(T);
@@ -645,9 +650,9 @@ __device__ __forceinline__ void wave_store_run(T* dst, int count, int nsrc, cons
 #pragma unroll
             for (int e = 0; e < V; ++e) w[e] = (T)0;
         }
-        uint4 u;
+        u32x4 u;
         __builtin_memcpy(&u, w, sizeof(u));
-        *reinterpret_cast<uint4*>(dst + f0) = u;
+        __builtin_nontemporal_store(u, reinterpret_cast<u32x4*>(dst + f0));
     }
     if (tail0 + lane < count) dst[tail0 + lane] = (staged && tail0 + lane < nsrc) ? s_w[tail0 + lane] : (T)0;
 }

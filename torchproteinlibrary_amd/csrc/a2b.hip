@@ -1002,6 +1002,16 @@ __global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
             // (uniform; it adds nothing to the carried sum).
             if (whole && r0 >= n) continue;
             if (whole) __syncthreads();  // s_wsum of the previous tile no longer read
+            // Issue priority by the tiles still to walk after this one: the
+            // longest chains are the kernel's critical path, and the waves of
+            // shorter ones have slack (measured 21.3 -> 20.0 us at B=1024,
+            // lengths 30-1000).
+            switch (min(whole ? t : 0, 3)) {
+                case 3: __builtin_amdgcn_s_setprio(3); break;
+                case 2: __builtin_amdgcn_s_setprio(2); break;
+                case 1: __builtin_amdgcn_s_setprio(1); break;
+                default: __builtin_amdgcn_s_setprio(0); break;
+            }
             const bool wave_active = r0 + wave * 64 * R < n;  // uniform per wave
 
             // Pass A: W = sum over the lane's atoms 0 .. 3R-1 (chain atoms

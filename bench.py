@@ -15,8 +15,8 @@ second).  The line also carries:
   the stream it runs on; algorithmic bytes per launch (``alg_bytes``) / average
   duration against the MI355X 8 TB/s HBM peak, and the HBM bytes per launch the
   counters measured (``profiles/traffic_<workload>.json``);
-* ``graph``: the same step replayed from HIP graphs (one captured forward +
-  backward per buffer set);
+* ``graph``: the same step replayed from a HIP graph that holds one forward +
+  backward per buffer set;
 * ``plain``: the same step with the plain kernels (serial forward, O(L^2)
   backward) and the speedup of the scan path over it;
 * ``cpu_baseline``: the reference CPU algorithm on a bounded sample of the same
@@ -312,10 +312,11 @@ def main():
         },
     }
 
-    # The same step replayed from HIP graphs (one captured fwd+bwd per buffer set).
-    gms, _ = ext.bench_graph([list(k) for k in ksets], list(cfg), args.steps, args.warmup)
-    out["graph"] = {"value": residues * args.steps / (gms / 1e3), "ms_per_step": gms / args.steps,
-                    "launch": "hipGraph replay, one captured fwd+bwd per buffer set"}
+    # The same step replayed from a HIP graph holding one fwd+bwd per buffer set.
+    gms, _, gsteps = ext.bench_graph([list(k) for k in ksets], list(cfg), args.steps, args.warmup)
+    out["graph"] = {"value": residues * gsteps / (gms / 1e3), "ms_per_step": gms / gsteps,
+                    "steps": gsteps,
+                    "launch": f"hipGraph replay, one graph of {len(ksets)} fwd+bwd steps (one per buffer set)"}
 
     if not args.no_plain:
         fr_b = ext.plain_frames_bytes(False, B, L)

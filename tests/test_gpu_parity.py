@@ -269,10 +269,11 @@ def test_graph_capture_replays_match_eager():
 
 @pytest.mark.parametrize("cfg", [(4, 2, 0), (1, 1, 0), (4, 2, 24)])
 def test_hip_graph_step_replays_match_eager(cfg):
-    """One forward + backward captured as a HIP graph on a private stream
-    (the extension's bench_graph) and replayed: every replay writes exactly the
-    eager outputs, also with eager work in between and with the look-back
-    zeroing kernel inside the graph ((1,1,0): 8 tiles per chain)."""
+    """A forward + backward per buffer set captured into one HIP graph on a
+    private stream (the extension's bench_graph) and replayed: every replay
+    writes exactly the eager outputs into both sets, also with eager work in
+    between and with the look-back zeroing kernel inside the graph ((1,1,0):
+    8 tiles per chain)."""
     from torchproteinlibrary_amd._native import ext
     a, n, g = synthetic.workload("fit500", batch=64)
     B, _, L = a.shape
@@ -286,15 +287,18 @@ def test_hip_graph_step_replays_match_eager(cfg):
     e = buffers()
     ext.Angles2BackboneGPU_forward(A, e[3], N, e[5], e[6], None, list(cfg))
     ext.Angles2BackboneGPU_backward(e[4], G, e[3], N, e[5], None, list(cfg))
-    gs = buffers()
+    gs, gs2 = buffers(), buffers()  # two buffer sets: one graph holds a step for each
     for rep in range(3):
         junk = torch.randn(1000, device=DEV).sum()  # eager work between replays
-        gs[3].zero_()
-        gs[4].zero_()
-        ext.bench_graph([gs], list(cfg), 2, 1)
+        for b_ in (gs, gs2):
+            b_[3].zero_()
+            b_[4].zero_()
+        _, _, done = ext.bench_graph([gs, gs2], list(cfg), 4, 2)
         torch.cuda.synchronize()
-        assert torch.equal(gs[3], e[3]), f"replay round {rep}: coords differ"
-        assert torch.equal(gs[4], e[4]), f"replay round {rep}: grads differ"
+        assert done == 4
+        for k, b_ in enumerate((gs, gs2)):
+            assert torch.equal(b_[3], e[3]), f"replay round {rep}, set {k}: coords differ"
+            assert torch.equal(b_[4], e[4]), f"replay round {rep}, set {k}: grads differ"
         del junk
     x64, g64 = oracle(a, n, g)
     assert np.abs(gs[3].cpu().numpy() - x64).max() <= coord_tol(x64)

@@ -287,9 +287,12 @@ std::tuple<double, double> bench_steps(std::vector<std::vector<torch::Tensor>> s
 // mode) and instantiated; the graphs are then replayed back to back on the
 // current stream, `warmup` untimed and `steps` timed.  Returns (event ms,
 // wall ms).  Nothing is allocated or cleared inside the captured work.
-std::tuple<double, double> bench_graph(std::vector<std::vector<torch::Tensor>> sets,
-                                       std::vector<int64_t> config, int64_t steps,
-                                       int64_t warmup) {
+// One HIP graph holds a forward + backward per buffer set, in order; it is
+// replayed ceil(steps / sets) times (graph launches amortised over the sets).
+// Returns (event ms, wall ms, steps replayed).
+std::tuple<double, double, int64_t> bench_graph(std::vector<std::vector<torch::Tensor>> sets,
+                                                std::vector<int64_t> config, int64_t steps,
+                                                int64_t warmup) {
     TORCH_CHECK(!sets.empty() && steps >= 0 && warmup >= 0, "bench_graph: bad arguments");
     for (auto& s : sets) TORCH_CHECK(s.size() == 7, "bench_graph: a set has 7 tensors");
     const tpl_a2b_config cfg = to_config(config);
@@ -311,32 +314,31 @@ std::tuple<double, double> bench_graph(std::vector<std::vector<torch::Tensor>> s
                 "hipStreamCreateWithFlags failed");
     std::vector<hipGraphExec_t> execs;
     std::string err;
-    for (auto& s : sets) {
-        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-            err = "hipStreamBeginCapture failed";
-            break;
-        }
-        int rc = tpl_a2b_forward(dt, s[0].data_ptr(), s[3].data_ptr(), s[1].data_ptr<int32_t>(),
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        err = "hipStreamBeginCapture failed";
+    } else {
+        int rc = TPL_A2B_OK;
+        for (auto& s : sets) {
+            rc = tpl_a2b_forward(dt, s[0].data_ptr(), s[3].data_ptr(), s[1].data_ptr<int32_t>(),
                                  s[5].data_ptr(), s[6].data_ptr(), nullptr, B, L, &cfg, cs);
-        if (rc == TPL_A2B_OK)
-            rc = tpl_a2b_backward(dt, s[4].data_ptr(), s[2].data_ptr(), s[3].data_ptr(),
-                                  s[1].data_ptr<int32_t>(), s[5].data_ptr(), nullptr, B, L, &cfg,
-                                  cs);
+            if (rc == TPL_A2B_OK)
+                rc = tpl_a2b_backward(dt, s[4].data_ptr(), s[2].data_ptr(), s[3].data_ptr(),
+                                      s[1].data_ptr<int32_t>(), s[5].data_ptr(), nullptr, B, L,
+                                      &cfg, cs);
+            if (rc != TPL_A2B_OK) break;
+        }
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(cs, &g);
         if (rc != TPL_A2B_OK || ec != hipSuccess || !g) {
             err = rc != TPL_A2B_OK ? tpl_a2b_strerror(rc) : "hipStreamEndCapture failed";
-            if (g) (void)hipGraphDestroy(g);
-            break;
+        } else {
+            hipGraphExec_t e = nullptr;
+            if (hipGraphInstantiate(&e, g, nullptr, nullptr, 0) != hipSuccess)
+                err = "hipGraphInstantiate failed";
+            else
+                execs.push_back(e);
         }
-        hipGraphExec_t e = nullptr;
-        const hipError_t ei = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ei != hipSuccess) {
-            err = "hipGraphInstantiate failed";
-            break;
-        }
-        execs.push_back(e);
+        if (g) (void)hipGraphDestroy(g);
     }
     auto cleanup = [&]() {
         for (auto e : execs) (void)hipGraphExecDestroy(e);
@@ -346,18 +348,19 @@ std::tuple<double, double> bench_graph(std::vector<std::vector<torch::Tensor>> s
         cleanup();
         TORCH_CHECK(false, "bench_graph: ", err);
     }
-    auto replay = [&](int64_t i) {
-        TORCH_CHECK(hipGraphLaunch(execs[i % execs.size()], stream) == hipSuccess,
-                    "hipGraphLaunch failed");
+    const int64_t per = (int64_t)sets.size();
+    const int64_t reps = (steps + per - 1) / per, wreps = (warmup + per - 1) / per;
+    auto replay = [&]() {
+        TORCH_CHECK(hipGraphLaunch(execs[0], stream) == hipSuccess, "hipGraphLaunch failed");
     };
-    for (int64_t i = 0; i < warmup; ++i) replay(i);
+    for (int64_t i = 0; i < wreps; ++i) replay();
     hipEvent_t e0, e1;
     TORCH_CHECK(hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess,
                 "hipEventCreate failed");
     TORCH_CHECK(hipStreamSynchronize(stream) == hipSuccess, "hipStreamSynchronize failed");
     const auto t0 = std::chrono::steady_clock::now();
     TORCH_CHECK(hipEventRecord(e0, stream) == hipSuccess, "hipEventRecord failed");
-    for (int64_t i = 0; i < steps; ++i) replay(i);
+    for (int64_t i = 0; i < reps; ++i) replay();
     TORCH_CHECK(hipEventRecord(e1, stream) == hipSuccess, "hipEventRecord failed");
     TORCH_CHECK(hipEventSynchronize(e1) == hipSuccess, "hipEventSynchronize failed");
     const auto t1 = std::chrono::steady_clock::now();
@@ -366,7 +369,7 @@ std::tuple<double, double> bench_graph(std::vector<std::vector<torch::Tensor>> s
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     cleanup();
-    return {(double)ms, std::chrono::duration<double, std::milli>(t1 - t0).count()};
+    return {(double)ms, std::chrono::duration<double, std::milli>(t1 - t0).count(), reps * per};
 }
 
 }  // namespace

@@ -1,9 +1,12 @@
 """Build recipe for the native parts of torchproteinlibrary_amd (in-tree).
 
-* ``libtpl_a2b.so``  -- HIP kernels + the C ABI of include/tpl_a2b.h, built by
-  ``hipcc --offload-arch=gfx950`` (cross-compiles without a GPU).
+* ``libtpl_a2b.so``  -- Angles2Backbone: HIP kernels + the C ABI of
+  include/tpl_a2b.h, built by ``hipcc --offload-arch=gfx950`` (cross-compiles
+  without a GPU).
 * ``_ReducedModel<EXT_SUFFIX>`` -- the pybind11 torch extension that unpacks
-  tensors and calls the C ABI; links libtpl_a2b.so with an $ORIGIN rpath.
+  tensors and calls that C ABI; links libtpl_a2b.so with an $ORIGIN rpath.
+* ``libtpl_rmsd.so`` and ``_RMSD<EXT_SUFFIX>`` -- the same pair for
+  Coords2RMSD (include/tpl_rmsd.h).
 
 Run ``python -m torchproteinlibrary_amd._build``.  Every call recompiles.
 """
@@ -25,14 +28,22 @@ ARCH = "gfx950"
 
 LIB_NAME = "libtpl_a2b.so"
 EXT_NAME = "_ReducedModel"
+RMSD_LIB_NAME = "libtpl_rmsd.so"
+RMSD_EXT_NAME = "_RMSD"
+
+# (library, its HIP source, extension, its C++ source)
+PARTS = [
+    (LIB_NAME, "a2b.hip", EXT_NAME, "a2b_ext.cpp"),
+    (RMSD_LIB_NAME, "rmsd.hip", RMSD_EXT_NAME, "rmsd_ext.cpp"),
+]
 
 
-def lib_path() -> str:
-    return os.path.join(PKG, LIB_NAME)
+def lib_path(name: str = LIB_NAME) -> str:
+    return os.path.join(PKG, name)
 
 
-def ext_path() -> str:
-    return os.path.join(PKG, EXT_NAME + sysconfig.get_config_var("EXT_SUFFIX"))
+def ext_path(name: str = EXT_NAME) -> str:
+    return os.path.join(PKG, name + sysconfig.get_config_var("EXT_SUFFIX"))
 
 
 def _run(cmd, verbose):
@@ -51,15 +62,15 @@ def _atomic_out(final: str) -> str:
     return tmp
 
 
-def build_lib(verbose=True) -> str:
-    out = lib_path()
+def build_lib(verbose=True, name: str = LIB_NAME, source: str = "a2b.hip") -> str:
+    out = lib_path(name)
     tmp = _atomic_out(out)
     cmd = [
         os.path.join(ROCM, "bin", "hipcc"), f"--offload-arch={ARCH}", "-O3", "-std=c++17",
         "-fPIC", "-shared", "-Wall", "-Werror", "-Wno-unused-command-line-argument",
         # No packed FP32 instructions anywhere in this library.
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-I", INCLUDE, os.path.join(CSRC, "a2b.hip"), "-o", tmp,
+        "-I", INCLUDE, os.path.join(CSRC, source), "-o", tmp,
     ]
     try:
         _run(cmd, verbose)
@@ -70,11 +81,12 @@ def build_lib(verbose=True) -> str:
     return out
 
 
-def build_ext(verbose=True) -> str:
+def build_ext(verbose=True, name: str = EXT_NAME, source: str = "a2b_ext.cpp",
+              lib: str = LIB_NAME) -> str:
     import torch
     from torch.utils import cpp_extension
 
-    out = ext_path()
+    out = ext_path(name)
     tmp = _atomic_out(out)
     torch_lib = os.path.join(os.path.dirname(torch.__file__), "lib")
     abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
@@ -87,9 +99,9 @@ def build_ext(verbose=True) -> str:
         "g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Werror",
         "-Wno-unused-variable", "-fvisibility=hidden",
         "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1", "-DTORCH_API_INCLUDE_EXTENSION_H",
-        f"-DTORCH_EXTENSION_NAME={EXT_NAME}", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
-        *incs, os.path.join(CSRC, "a2b_ext.cpp"), "-o", tmp,
-        "-L", PKG, "-l:" + LIB_NAME, "-Wl,-rpath,$ORIGIN",
+        f"-DTORCH_EXTENSION_NAME={name}", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
+        *incs, os.path.join(CSRC, source), "-o", tmp,
+        "-L", PKG, "-l:" + lib, "-Wl,-rpath,$ORIGIN",
         "-L", torch_lib, "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-ltorch_python",
         "-Wl,-rpath," + torch_lib,
         "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
@@ -104,13 +116,19 @@ def build_ext(verbose=True) -> str:
 
 
 def build(verbose=True):
-    """Compile the HIP library, then the extension that links it."""
-    build_lib(verbose)
-    build_ext(verbose)
+    """Compile each HIP library, then the extension that links it (the parts
+    in parallel)."""
+    def part(p):
+        lib, hip, ext, cpp = p
+        build_lib(verbose, lib, hip)
+        build_ext(verbose, ext, cpp, lib)
+    with ThreadPoolExecutor(len(PARTS)) as pool:
+        for f in [pool.submit(part, p) for p in PARTS]:
+            f.result()
 
 
 def built() -> bool:
-    return os.path.exists(lib_path()) and os.path.exists(ext_path())
+    return all(os.path.exists(lib_path(p[0])) and os.path.exists(ext_path(p[2])) for p in PARTS)
 
 
 if __name__ == "__main__":

@@ -20,7 +20,9 @@ second).  The line also carries:
 * ``plain``: the same step with the plain kernels (serial forward, O(L^2)
   backward) and the speedup of the scan path over it;
 * ``cpu_baseline``: the reference CPU algorithm on a bounded sample of the same
-  workload, on the host, one thread.
+  workload, on the host, one thread;
+* ``rmsd``: the Coords2RMSD loss (forward + backward) on the workload's
+  backbone atoms against a moved, noisy copy, with its own roofline.
 """
 from __future__ import annotations
 
@@ -66,6 +68,7 @@ def parse():
     p.add_argument("--plain-steps", type=int, default=20)
     p.add_argument("--kernel-reps", type=int, default=200)
     p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--no-rmsd", action="store_true", help="skip the Coords2RMSD line")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--config", default="", help="pin tile config 'waves,res_per_lane,max_wgs'")
@@ -196,6 +199,65 @@ def fit_loop(steps: int):
     return res
 
 
+def rmsd_bench(ksets, n, L, steps, warmup, reps):
+    """Coords2RMSD fwd+bwd on each buffer set's forward coordinates (the
+    backbone atoms N, CA, C: num_atoms = 3 * length, stride 3 * L) against a
+    rotated, shifted copy with 0.3 A noise.  Buffers rotate like the main
+    loop's; times are HIP events around native launch loops."""
+    import torch
+    from torchproteinlibrary_amd._native_rmsd import ext as rext
+
+    dev = ksets[0][3].device
+    B, A = len(n), 3 * L
+    natoms = torch.tensor(3 * n, dtype=torch.int32, device=dev)
+    atoms = int(3 * n.sum())
+    gen = torch.Generator(device=dev).manual_seed(5)
+    valid = (torch.arange(A, device=dev)[None, :] < natoms[:, None]).unsqueeze(-1)
+    rsets = []
+    for k in ksets:
+        x = k[3]
+        q = torch.randn(4, generator=gen, device=dev, dtype=torch.float64)
+        q = q / q.norm()
+        w, i, j, kk = q.tolist()
+        Q = torch.tensor([[w * w + i * i - j * j - kk * kk, 2 * (i * j - w * kk), 2 * (i * kk + w * j)],
+                          [2 * (i * j + w * kk), w * w - i * i + j * j - kk * kk, 2 * (j * kk - w * i)],
+                          [2 * (i * kk - w * j), 2 * (j * kk + w * i), w * w - i * i - j * j + kk * kk]],
+                         dtype=x.dtype, device=dev)
+        y = x.view(B, A, 3) @ Q.T + 5.0
+        y = y + 0.3 * torch.randn(y.shape, generator=gen, device=dev, dtype=x.dtype)
+        y = torch.where(valid, y, torch.zeros_like(y)).reshape(B, 3 * A).contiguous()
+        rsets.append([x, y, natoms, torch.empty(B, dtype=x.dtype, device=dev),
+                      torch.empty(B, 3, 3, dtype=x.dtype, device=dev),
+                      torch.empty(max(rext.state_bytes(B), 1), dtype=torch.uint8, device=dev),
+                      torch.randn(B, generator=gen, device=dev, dtype=x.dtype),
+                      torch.empty(B, 3 * A, dtype=x.dtype, device=dev)])
+    torch.cuda.synchronize()
+    step_ms, f_ms, b_ms = rext.bench(rsets, steps, warmup, reps)
+    # Compulsory bytes (FP32): both inputs' valid atoms in each direction; the
+    # backward writes every gradient slot (zeros past num_atoms); per structure
+    # num_atoms, rmsd, UT and the 16-double state.
+    fwd_b = 24 * atoms + B * (4 + 4 + 36 + 128)
+    bwd_b = 24 * atoms + 12 * B * A + B * (4 + 4 + 128)
+    kern = {"forward": {"avg_ms": f_ms, "alg_bytes": fwd_b, "GBs": fwd_b / (f_ms * 1e-3) / 1e9},
+            "backward": {"avg_ms": b_ms, "alg_bytes": bwd_b, "GBs": bwd_b / (b_ms * 1e-3) / 1e9}}
+    dom = max(kern, key=lambda k: kern[k]["avg_ms"])
+    ms_step = step_ms / steps
+    return {
+        "workload": f"Coords2RMSD fwd+bwd, backbone atoms of the workload: B={B}, "
+                    f"atoms_stride={A}, num_atoms=3*length, f32",
+        "atoms": atoms,
+        "value": atoms / (ms_step * 1e-3),
+        "unit": "atoms/s",
+        "ms_per_step": ms_step,
+        "steps": steps,
+        "buffer_sets": len(rsets),
+        "kernels": kern,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["GBs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": kern[dom]["GBs"] / HBM_PEAK_GBS},
+    }
+
+
 def main():
     args = parse()
     if args.fit:
@@ -317,6 +379,10 @@ def main():
     out["graph"] = {"value": residues * gsteps / (gms / 1e3), "ms_per_step": gms / gsteps,
                     "steps": gsteps,
                     "launch": f"hipGraph replay, one graph of {len(ksets)} fwd+bwd steps (one per buffer set)"}
+
+    if not args.no_rmsd:
+        out["rmsd"] = rmsd_bench(ksets, n, L, min(args.steps, 1000), min(args.warmup, 100),
+                                 args.kernel_reps)
 
     if not args.no_plain:
         fr_b = ext.plain_frames_bytes(False, B, L)

@@ -55,9 +55,13 @@ def test_parity_with_oracle(dtype):
     # With two atoms the rotation about their axis is free (the largest
     # eigenvalue of F is double), so U is compared from three atoms on; the
     # RMSD and the gradient (through U^T y_c) are unique in every case.
+    # The double root of two atoms takes the shifted adjugate (rmsd_math.h),
+    # whose rotation is good to about 1e-8: its gradient row gets 1e-7.
     many = n >= 3
     assert np.abs(ut[many] - ut64[many]).max() <= tu
-    assert np.abs(ga - g64).max() <= tg * np.abs(g64).max()
+    gmax = np.abs(g64).max()
+    assert np.abs(ga[many] - g64[many]).max() <= tg * gmax
+    assert np.abs(ga[~many] - g64[~many]).max() <= max(tg, 1e-7) * gmax
     for b in range(48):
         assert not ga[b, 3 * int(n[b]):].any(), b  # exact zeros past num_atoms
 

@@ -6,13 +6,15 @@
 // RMSDKernels.cu:5-39) with everything on the device:
 //   forward, one workgroup per structure:
 //     1. FP64 moments of the atoms relative to the structure's first atom
-//        (sum x, sum y, sum x y^T, sum |x|^2, sum |y|^2), summed over lanes
-//        (DPP-free lane butterflies through ds_bpermute) and waves (LDS);
+//        (sum x, sum y, sum x y^T, sum |x|^2, sum |y|^2), four atoms per
+//        thread and pass, summed over the workgroup in two levels through LDS;
 //     2. lane 0 centres them, builds the symmetric 4x4 matrix F of the
 //        centred correlation R (the reference's element layout,
-//        coords2rmsd_interface.cpp:153-158), takes its largest eigenpair by
-//        cyclic Jacobi rotations, forms U from the quaternion (same formula,
-//        :35-45) and rmsd = sqrt((R2 - 2 |lambda| + 1e-10) / n) (:182);
+//        coords2rmsd_interface.cpp:153-158), takes its largest eigenpair
+//        (rmsd_math.h: Newton on the characteristic quartic, eigenvector from
+//        the adjugate; a shifted adjugate for a double root), forms U from the quaternion
+//        (same formula, :35-45) and rmsd = sqrt((R2 - 2 |lambda| + 1e-10) / n)
+//        (:182);
 //   backward, one thread per atom: (x - cx - U^T (y - cy)) * g / (rmsd * n)
 //     (Coords2RMSD.py:47-56), zero past num_atoms.
 // Shifting by the first atom keeps the raw moments at the size of the
@@ -23,6 +25,7 @@
 
 #include <algorithm>
 
+#include "rmsd_math.h"
 #include "tpl_rmsd.h"
 
 namespace {
@@ -33,20 +36,29 @@ constexpr int kBwdThreads = 256;     // backward: one thread per atom
 constexpr int kState = 16;           // doubles per structure: cx[3] cy[3] U[9] rmsd
 constexpr int kMoments = 17;         // sx[3] sy[3] sxy[9] sxx syy
 
-__device__ __forceinline__ double dbl_from(int lo, int hi) {
-    return __hiloint2double(hi, lo);
+// Atoms a0 .. a0+3 of one structure (xyz interleaved) as 12 values: three
+// 16-byte loads per 4 floats (six for doubles) when the run is aligned and
+// whole, else element loads; atoms at or past `lim` read as 0.
+template <typename T>
+__device__ __forceinline__ void load4(const T* p, int a0, int lim, bool vec, double v[12]) {
+    if (vec && a0 + 4 <= lim) {
+        constexpr int E = 16 / (int)sizeof(T);  // elements per 16-byte load
+        T t[12];
+#pragma unroll
+        for (int k = 0; k < 12; k += E) {
+            const uint4 u = *reinterpret_cast<const uint4*>(p + 3 * (size_t)a0 + k);
+            __builtin_memcpy(t + k, &u, 16);
+        }
+#pragma unroll
+        for (int k = 0; k < 12; ++k) v[k] = (double)t[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) v[k] = (a0 + k / 3 < lim) ? (double)p[3 * (size_t)a0 + k] : 0.0;
+    }
 }
 
-// Sum over the 64 lanes of the wave; every lane gets the total.
-__device__ __forceinline__ double wave_sum(double v, int lane) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        const int src = (lane ^ m) << 2;
-        const int lo = __builtin_amdgcn_ds_bpermute(src, __double2loint(v));
-        const int hi = __builtin_amdgcn_ds_bpermute(src, __double2hiint(v));
-        v += dbl_from(lo, hi);
-    }
-    return v;
+__device__ __forceinline__ bool aligned16(const void* p) {
+    return (reinterpret_cast<uintptr_t>(p) & 15) == 0;
 }
 
 __device__ __forceinline__ void report(uint32_t* status, uint32_t bit) {
@@ -61,85 +73,21 @@ __device__ __forceinline__ int clamp_count(int n, int A, uint32_t* status, bool 
     return n;
 }
 
-// One Jacobi rotation of the symmetric 4x4 matrix a (row-major) in the plane
-// (p, q), chosen to zero a[p][q]; v accumulates the rotations (columns are
-// the eigenvectors).  With theta = (a_qq - a_pp) / (2 a_pq), t = tan(phi) is
-// the smaller root of t^2 + 2 theta t - 1 = 0.
-template <int P, int Q>
-__device__ __forceinline__ void jacobi_rotate(double (&a)[16], double (&v)[16]) {
-    const double apq = a[4 * P + Q];
-    if (apq == 0.0) return;
-    const double theta = (a[4 * Q + Q] - a[4 * P + P]) / (2.0 * apq);
-    const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-    const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {  // columns p, q
-        const double akp = a[4 * k + P], akq = a[4 * k + Q];
-        a[4 * k + P] = c * akp - s * akq;
-        a[4 * k + Q] = s * akp + c * akq;
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {  // rows p, q
-        const double apk = a[4 * P + k], aqk = a[4 * Q + k];
-        a[4 * P + k] = c * apk - s * aqk;
-        a[4 * Q + k] = s * apk + c * aqk;
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const double vkp = v[4 * k + P], vkq = v[4 * k + Q];
-        v[4 * k + P] = c * vkp - s * vkq;
-        v[4 * k + Q] = s * vkp + c * vkq;
-    }
-}
-
-// Largest eigenvalue of the symmetric 4x4 matrix f and a unit eigenvector.
-// Cyclic Jacobi sweeps until the off-diagonal mass is below 1e-32 of the
-// diagonal's (or vanishes), at most 20 sweeps.
-__device__ double max_eigenpair(const double (&f)[16], double (&q)[4]) {
-    double a[16], v[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        a[i] = f[i];
-        v[i] = (i % 5 == 0) ? 1.0 : 0.0;
-    }
-    for (int sweep = 0; sweep < 20; ++sweep) {
-        const double off = a[1] * a[1] + a[2] * a[2] + a[3] * a[3] + a[6] * a[6] + a[7] * a[7] +
-                           a[11] * a[11];
-        const double dg = a[0] * a[0] + a[5] * a[5] + a[10] * a[10] + a[15] * a[15];
-        if (off == 0.0 || off <= 1e-32 * dg) break;
-        jacobi_rotate<0, 1>(a, v);
-        jacobi_rotate<0, 2>(a, v);
-        jacobi_rotate<0, 3>(a, v);
-        jacobi_rotate<1, 2>(a, v);
-        jacobi_rotate<1, 3>(a, v);
-        jacobi_rotate<2, 3>(a, v);
-    }
-    int best = 0;
-    double lam = a[0];
-#pragma unroll
-    for (int k = 1; k < 4; ++k) {
-        if (a[5 * k] > lam) {
-            lam = a[5 * k];
-            best = k;
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        double e = v[4 * j];
-#pragma unroll
-        for (int k = 1; k < 4; ++k) e = (best == k) ? v[4 * j + k] : e;
-        q[j] = e;
-    }
-    return lam;
-}
-
 template <typename T>
 __global__ void __launch_bounds__(kThreads) rmsd_fwd(const T* __restrict__ x, const T* __restrict__ y,
                                                      const int32_t* __restrict__ num_atoms,
                                                      T* rmsd_out, T* ut_out, double* state,
                                                      uint32_t* status, int A) {
-    __shared__ double s_red[kThreads / 64][kMoments];
-    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // Moment sums over the workgroup: every thread's 17 partial sums (laid
+    // out [moment][thread]), then kParts partial sums per moment, then the
+    // totals.  A cross-lane butterfly over 17 doubles measured 5 us per
+    // workgroup, bound by the LDS crossbar at 16 waves per CU; this takes
+    // three barriers and plain LDS reads.
+    constexpr int kParts = kThreads / kMoments;  // 15 threads per moment in the middle level
+    __shared__ double s_m[kMoments * kThreads];
+    __shared__ double s_p[kMoments * kParts];
+    __shared__ double s_tot[kMoments];
+    const int b = blockIdx.x, tid = threadIdx.x;
     const int n = clamp_count(num_atoms[b], A, status, tid == 0);
     const T* xb = x + (size_t)b * 3 * A;
     const T* yb = y + (size_t)b * 3 * A;
@@ -154,39 +102,55 @@ __global__ void __launch_bounds__(kThreads) rmsd_fwd(const T* __restrict__ x, co
     double m[kMoments];
 #pragma unroll
     for (int e = 0; e < kMoments; ++e) m[e] = 0.0;
-    for (int i = tid; i < n; i += kThreads) {
-        double dx[3], dy[3];
+    // Four consecutive atoms per thread and pass: one 48-byte run of each
+    // structure per lane, loaded in 16-byte pieces.
+    const bool vec = aligned16(xb) && aligned16(yb);
+    for (int a0 = 4 * tid; a0 < n; a0 += 4 * kThreads) {
+        double xv[12], yv[12];
+        load4(xb, a0, n, vec, xv);
+        load4(yb, a0, n, vec, yv);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            dx[c] = (double)xb[3 * i + c] - x0[c];
-            dy[c] = (double)yb[3 * i + c] - y0[c];
+        for (int j = 0; j < 4; ++j) {
+            const bool in = a0 + j < n;
+            double dx[3], dy[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                dx[c] = in ? xv[3 * j + c] - x0[c] : 0.0;
+                dy[c] = in ? yv[3 * j + c] - y0[c] : 0.0;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                m[c] += dx[c];
+                m[3 + c] += dy[c];
+                m[15] += dx[c] * dx[c];
+                m[16] += dy[c] * dy[c];
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) m[6 + 3 * r + c] += dx[r] * dy[c];
         }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            m[c] += dx[c];
-            m[3 + c] += dy[c];
-            m[15] += dx[c] * dx[c];
-            m[16] += dy[c] * dy[c];
-        }
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) m[6 + 3 * r + c] += dx[r] * dy[c];
     }
 #pragma unroll
-    for (int e = 0; e < kMoments; ++e) m[e] = wave_sum(m[e], lane);
-    if (lane == 0) {
+    for (int e = 0; e < kMoments; ++e) s_m[e * kThreads + tid] = m[e];
+    __syncthreads();
+    if (tid < kMoments * kParts) {
+        const int e = tid / kParts, part = tid - e * kParts;
+        double acc = 0.0;
+        for (int t = part; t < kThreads; t += kParts) acc += s_m[e * kThreads + t];
+        s_p[e * kParts + part] = acc;
+    }
+    __syncthreads();
+    if (tid < kMoments) {
+        double acc = 0.0;
 #pragma unroll
-        for (int e = 0; e < kMoments; ++e) s_red[wave][e] = m[e];
+        for (int part = 0; part < kParts; ++part) acc += s_p[tid * kParts + part];
+        s_tot[tid] = acc;
     }
     __syncthreads();
     if (tid != 0) return;
-
-    const int nw = blockDim.x >> 6;
-    for (int w = 1; w < nw; ++w) {
 #pragma unroll
-        for (int e = 0; e < kMoments; ++e) m[e] += s_red[w][e];
-    }
+    for (int e = 0; e < kMoments; ++e) m[e] = s_tot[e];
     const double nd = (double)n;
     double mx[3], my[3];
 #pragma unroll
@@ -202,21 +166,11 @@ __global__ void __launch_bounds__(kThreads) rmsd_fwd(const T* __restrict__ x, co
         for (int c = 0; c < 3; ++c) R[3 * r + c] = m[6 + 3 * r + c] - nd * mx[r] * my[c];
     const double r2 = (m[15] - nd * (mx[0] * mx[0] + mx[1] * mx[1] + mx[2] * mx[2])) +
                       (m[16] - nd * (my[0] * my[0] + my[1] * my[1] + my[2] * my[2]));
-    // F: the reference's element layout (coords2rmsd_interface.cpp:155-158).
-    const double F[16] = {
-        R[0] + R[4] + R[8], R[5] - R[7],        R[6] - R[2],         R[1] - R[3],
-        R[5] - R[7],        R[0] - R[4] - R[8], R[1] + R[3],         R[2] + R[6],
-        R[6] - R[2],        R[1] + R[3],        -R[0] + R[4] - R[8], R[5] + R[7],
-        R[1] - R[3],        R[2] + R[6],        R[5] + R[7],         -R[0] - R[4] + R[8]};
-    double q[4];
-    const double lam = max_eigenpair(F, q);
-    const double U[9] = {
-        q[0] * q[0] + q[1] * q[1] - q[2] * q[2] - q[3] * q[3], 2.0 * (q[1] * q[2] - q[0] * q[3]),
-        2.0 * (q[1] * q[3] + q[0] * q[2]),
-        2.0 * (q[1] * q[2] + q[0] * q[3]), q[0] * q[0] - q[1] * q[1] + q[2] * q[2] - q[3] * q[3],
-        2.0 * (q[2] * q[3] - q[0] * q[1]),
-        2.0 * (q[1] * q[3] - q[0] * q[2]), 2.0 * (q[2] * q[3] + q[0] * q[1]),
-        q[0] * q[0] - q[1] * q[1] - q[2] * q[2] + q[3] * q[3]};
+    double F[16], U[9];
+    rmsd_math::f_matrix(R, F);
+    const rmsd_math::Eigenpair eig = rmsd_math::max_eigenpair(F, 0.5 * r2);
+    const double lam = eig.lam;
+    rmsd_math::u_from_q(eig, U);
     const double rmsd = sqrt((r2 - 2.0 * fabs(lam) + kEpsLimit) / nd);
     rmsd_out[b] = (T)rmsd;
 #pragma unroll

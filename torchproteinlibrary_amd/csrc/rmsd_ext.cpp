@@ -9,6 +9,9 @@
 #include <torch/extension.h>
 #include <c10/hip/HIPStream.h>
 
+#include <tuple>
+#include <vector>
+
 #include "tpl_rmsd.h"
 
 namespace {
@@ -111,6 +114,67 @@ void backward(torch::Tensor grad_input, torch::Tensor grad_rmsd, torch::Tensor i
              "Coords2RMSDGPU_backward");
 }
 
+// Timing loop for bench.py: sets of [input, target, num_atoms, rmsd, UT,
+// state, grad_rmsd, grad_input], rotated per step.  Returns HIP-event ms for
+// `steps` back-to-back forward+backward steps, and the average ms of one
+// forward and of one backward launched `reps` times alone.
+std::tuple<double, double, double> bench(std::vector<std::vector<torch::Tensor>> sets,
+                                         int64_t steps, int64_t warmup, int64_t reps) {
+    TORCH_CHECK(!sets.empty() && steps > 0 && warmup >= 0 && reps > 0, "bench: bad arguments");
+    struct Set {
+        int dt, B, A;
+        const void *x, *y, *gr;
+        const int32_t* n;
+        void *r, *ut, *st, *gx;
+    };
+    std::vector<Set> ss;
+    for (auto& t : sets) {
+        TORCH_CHECK(t.size() == 8, "bench: a set has 8 tensors");
+        auto [B, A] = check_inputs(t[0], t[1], t[2]);
+        check_like(t[3], t[0], {B}, "output");
+        check_like(t[4], t[0], {B, 3, 3}, "UT");
+        check_like(t[6], t[0], {B}, "gradRMSD");
+        check_like(t[7], t[0], {B, 3 * (int64_t)A}, "gradInput");
+        ss.push_back({dtype_code(t[0]), B, A, t[0].data_ptr(), t[1].data_ptr(), t[6].data_ptr(),
+                      t[2].data_ptr<int32_t>(), t[3].data_ptr(), t[4].data_ptr(),
+                      state_ptr(t[5], B, t[0]), t[7].data_ptr()});
+    }
+    c10::DeviceGuard guard(sets[0][0].device());
+    hipStream_t stream = stream_of(sets[0][0]);
+    auto fwd = [&](const Set& q) {
+        check_rc(tpl_rmsd_forward(q.dt, q.x, q.y, q.n, q.r, q.ut, q.st, nullptr, q.B, q.A, stream),
+                 "tpl_rmsd_forward");
+    };
+    auto bwd = [&](const Set& q) {
+        check_rc(tpl_rmsd_backward(q.dt, q.gx, q.gr, q.x, q.y, q.n, q.st, nullptr, q.B, q.A, stream),
+                 "tpl_rmsd_backward");
+    };
+    hipEvent_t e0, e1;
+    TORCH_CHECK(hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess,
+                "hipEventCreate failed");
+    auto timed = [&](auto&& body, int64_t n) {
+        TORCH_CHECK(hipEventRecord(e0, stream) == hipSuccess, "hipEventRecord failed");
+        for (int64_t i = 0; i < n; ++i) body(i);
+        TORCH_CHECK(hipEventRecord(e1, stream) == hipSuccess, "hipEventRecord failed");
+        TORCH_CHECK(hipEventSynchronize(e1) == hipSuccess, "hipEventSynchronize failed");
+        float ms = 0.f;
+        TORCH_CHECK(hipEventElapsedTime(&ms, e0, e1) == hipSuccess, "hipEventElapsedTime failed");
+        return (double)ms;
+    };
+    const size_t S = ss.size();
+    auto step = [&](int64_t i) {
+        fwd(ss[i % S]);
+        bwd(ss[i % S]);
+    };
+    for (int64_t i = 0; i < warmup; ++i) step(i);
+    const double step_ms = timed(step, steps);
+    const double f_ms = timed([&](int64_t i) { fwd(ss[i % S]); }, reps) / reps;
+    const double b_ms = timed([&](int64_t i) { bwd(ss[i % S]); }, reps) / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return {step_ms, f_ms, b_ms};
+}
+
 int64_t state_bytes(int64_t batch) {
     TORCH_CHECK(batch >= 0 && batch <= INT32_MAX, "bad batch");
     return (int64_t)tpl_rmsd_state_bytes((int)batch);
@@ -123,4 +187,5 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("Coords2RMSDGPU_forward", &forward, "Coords2RMSD forward on GPU (centring, moments, eigenproblem)");
     m.def("Coords2RMSDGPU_backward", &backward, "Coords2RMSD backward on GPU");
     m.def("state_bytes", &state_bytes, "forward->backward state bytes for a batch");
+    m.def("bench", &bench, "time fwd+bwd steps and each kernel alone (event ms)");
 }

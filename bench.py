@@ -199,7 +199,7 @@ def fit_loop(steps: int):
     return res
 
 
-def rmsd_bench(ksets, n, L, steps, warmup, reps):
+def rmsd_bench(ksets, n, L, steps, warmup, reps, workload):
     """Coords2RMSD fwd+bwd on each buffer set's forward coordinates (the
     backbone atoms N, CA, C: num_atoms = 3 * length, stride 3 * L) against a
     rotated, shifted copy with 0.3 A noise.  Buffers rotate like the main
@@ -242,6 +242,11 @@ def rmsd_bench(ksets, n, L, steps, warmup, reps):
             "backward": {"avg_ms": b_ms, "alg_bytes": bwd_b, "GBs": bwd_b / (b_ms * 1e-3) / 1e9}}
     dom = max(kern, key=lambda k: kern[k]["avg_ms"])
     ms_step = step_ms / steps
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_rmsd_{workload}.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get(dom)
     return {
         "workload": f"Coords2RMSD fwd+bwd, backbone atoms of the workload: B={B}, "
                     f"atoms_stride={A}, num_atoms=3*length, f32",
@@ -254,7 +259,8 @@ def rmsd_bench(ksets, n, L, steps, warmup, reps):
         "kernels": kern,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["GBs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": kern[dom]["GBs"] / HBM_PEAK_GBS},
+                     "frac": kern[dom]["GBs"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": kern[dom]["alg_bytes"]},
     }
 
 
@@ -382,7 +388,7 @@ def main():
 
     if not args.no_rmsd:
         out["rmsd"] = rmsd_bench(ksets, n, L, min(args.steps, 1000), min(args.warmup, 100),
-                                 args.kernel_reps)
+                                 args.kernel_reps, args.workload)
 
     if not args.no_plain:
         fr_b = ext.plain_frames_bytes(False, B, L)

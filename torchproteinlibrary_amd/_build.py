@@ -7,6 +7,8 @@
   tensors and calls that C ABI; links libtpl_a2b.so with an $ORIGIN rpath.
 * ``libtpl_rmsd.so`` and ``_RMSD<EXT_SUFFIX>`` -- the same pair for
   Coords2RMSD (include/tpl_rmsd.h).
+* ``libtpl_xform.so`` and ``_FullAtomModel<EXT_SUFFIX>`` -- the same pair for
+  the coordinate transforms (include/tpl_xform.h).
 
 Run ``python -m torchproteinlibrary_amd._build``.  Every call recompiles.
 """
@@ -30,11 +32,14 @@ LIB_NAME = "libtpl_a2b.so"
 EXT_NAME = "_ReducedModel"
 RMSD_LIB_NAME = "libtpl_rmsd.so"
 RMSD_EXT_NAME = "_RMSD"
+XFORM_LIB_NAME = "libtpl_xform.so"
+XFORM_EXT_NAME = "_FullAtomModel"
 
 # (library, its HIP source, extension, its C++ source)
 PARTS = [
     (LIB_NAME, "a2b.hip", EXT_NAME, "a2b_ext.cpp"),
     (RMSD_LIB_NAME, "rmsd.hip", RMSD_EXT_NAME, "rmsd_ext.cpp"),
+    (XFORM_LIB_NAME, "xform.hip", XFORM_EXT_NAME, "xform_ext.cpp"),
 ]
 
 

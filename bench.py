@@ -22,7 +22,9 @@ second).  The line also carries:
 * ``cpu_baseline``: the reference CPU algorithm on a bounded sample of the same
   workload, on the host, one thread;
 * ``rmsd``: the Coords2RMSD loss (forward + backward) on the workload's
-  backbone atoms against a moved, noisy copy, with its own roofline.
+  backbone atoms against a moved, noisy copy, with its own roofline;
+* ``xform``: the coordinate transforms (translate, rotate, centre and its
+  backward) on the same atoms, per-kernel times and bandwidth.
 """
 from __future__ import annotations
 
@@ -264,6 +266,40 @@ def rmsd_bench(ksets, n, L, steps, warmup, reps, workload):
     }
 
 
+def xform_bench(ksets, n, L, reps):
+    """The coordinate transforms on each buffer set's forward coordinates as
+    backbone structures (num_atoms = 3 * length, stride 3 * L), FP32: average
+    launch time of translate, rotate, centre and centre-backward (HIP events
+    around native launch loops, buffers rotating)."""
+    import torch
+    from torchproteinlibrary_amd._native_xform import ext as xext
+
+    dev = ksets[0][3].device
+    B, A = len(n), 3 * L
+    natoms = torch.tensor(3 * n, dtype=torch.int32, device=dev)
+    atoms = int(3 * n.sum())
+    from torchproteinlibrary_amd.FullAtomModel.CoordsTransform import getRandomRotation
+    Rm = getRandomRotation(B).to(device=dev, dtype=torch.float32)
+    T = torch.randn(B, 3, device=dev)
+    xsets = [[k[3], torch.empty(B, 3 * A, device=dev), T, Rm, torch.empty(B, 3, device=dev), natoms]
+             for k in ksets]
+    torch.cuda.synchronize()
+    tr, ro, ce, cb = xext.bench(xsets, reps)
+    # compulsory bytes: read the valid atoms, write every slot (zeros past
+    # num_atoms); per structure num_atoms and T/R/centre
+    rw = 12 * atoms + 12 * B * A
+    byt = {"translate": rw + B * 16, "rotate": rw + B * 40, "center": 12 * atoms + B * 16,
+           "center_backward": 12 * B * A + B * 16}
+    ms = {"translate": tr, "rotate": ro, "center": ce, "center_backward": cb}
+    kern = {k: {"avg_ms": ms[k], "alg_bytes": byt[k], "GBs": byt[k] / (ms[k] * 1e-3) / 1e9} for k in ms}
+    dom = max(kern, key=lambda k: kern[k]["GBs"])
+    return {"workload": f"CoordsTransform on the workload's backbone atoms: B={B}, atoms_stride={A}, "
+                        "num_atoms=3*length, f32",
+            "atoms": atoms, "kernels": kern,
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["GBs"], "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": kern[dom]["GBs"] / HBM_PEAK_GBS}}
+
+
 def main():
     args = parse()
     if args.fit:
@@ -389,6 +425,9 @@ def main():
     if not args.no_rmsd:
         out["rmsd"] = rmsd_bench(ksets, n, L, min(args.steps, 1000), min(args.warmup, 100),
                                  args.kernel_reps, args.workload)
+
+    if not args.no_rmsd:
+        out["xform"] = xform_bench(ksets, n, L, args.kernel_reps)
 
     if not args.no_plain:
         fr_b = ext.plain_frames_bytes(False, B, L)

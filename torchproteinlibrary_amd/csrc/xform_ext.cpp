@@ -10,6 +10,9 @@
 #include <torch/extension.h>
 #include <c10/hip/HIPStream.h>
 
+#include <tuple>
+#include <vector>
+
 #include "tpl_xform.h"
 
 namespace {
@@ -134,6 +137,59 @@ void center_backward(torch::Tensor grad_output_center, torch::Tensor grad_input_
              "Coords2CenterGPU_backward");
 }
 
+// Timing loop for bench.py: sets of [coords, out, T, R, center, num_atoms],
+// rotated per launch.  Returns the average HIP-event ms of one launch of
+// translate, rotate, centre and centre-backward, each run `reps` times.
+std::tuple<double, double, double, double> bench(std::vector<std::vector<torch::Tensor>> sets,
+                                                 int64_t reps) {
+    TORCH_CHECK(!sets.empty() && reps > 0, "bench: bad arguments");
+    for (auto& t : sets) {
+        TORCH_CHECK(t.size() == 6, "bench: a set has 6 tensors");
+        auto [B, A] = check_coords(t[0], t[5]);
+        check_like(t[1], t[0], {B, 3 * (int64_t)A}, "out");
+        check_like(t[2], t[0], {B, 3}, "T");
+        check_like(t[3], t[0], {B, 3, 3}, "R");
+        check_like(t[4], t[0], {B, 3}, "center");
+    }
+    c10::DeviceGuard guard(sets[0][0].device());
+    hipStream_t stream = stream_of(sets[0][0]);
+    const int dt = dtype_code(sets[0][0]);
+    const int B = (int)sets[0][0].size(0), A = (int)(sets[0][0].size(1) / 3);
+    hipEvent_t e0, e1;
+    TORCH_CHECK(hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess,
+                "hipEventCreate failed");
+    auto timed = [&](auto&& body) {
+        for (int64_t i = 0; i < 5; ++i) body(sets[i % sets.size()]);
+        TORCH_CHECK(hipEventRecord(e0, stream) == hipSuccess, "hipEventRecord failed");
+        for (int64_t i = 0; i < reps; ++i) body(sets[i % sets.size()]);
+        TORCH_CHECK(hipEventRecord(e1, stream) == hipSuccess, "hipEventRecord failed");
+        TORCH_CHECK(hipEventSynchronize(e1) == hipSuccess, "hipEventSynchronize failed");
+        float ms = 0.f;
+        TORCH_CHECK(hipEventElapsedTime(&ms, e0, e1) == hipSuccess, "hipEventElapsedTime failed");
+        return (double)ms / reps;
+    };
+    auto n = [](std::vector<torch::Tensor>& t) { return t[5].data_ptr<int32_t>(); };
+    const double tr = timed([&](std::vector<torch::Tensor>& t) {
+        check_rc(tpl_xform_translate(dt, t[1].data_ptr(), t[0].data_ptr(), t[2].data_ptr(), n(t),
+                                     nullptr, B, A, stream), "translate");
+    });
+    const double ro = timed([&](std::vector<torch::Tensor>& t) {
+        check_rc(tpl_xform_rotate(dt, t[1].data_ptr(), t[0].data_ptr(), t[3].data_ptr(), 0, n(t),
+                                  nullptr, B, A, stream), "rotate");
+    });
+    const double ce = timed([&](std::vector<torch::Tensor>& t) {
+        check_rc(tpl_xform_center(dt, t[4].data_ptr(), t[0].data_ptr(), n(t), nullptr, B, A, stream),
+                 "center");
+    });
+    const double cb = timed([&](std::vector<torch::Tensor>& t) {
+        check_rc(tpl_xform_center_backward(dt, t[1].data_ptr(), t[4].data_ptr(), n(t), nullptr, B,
+                                           A, stream), "center_backward");
+    });
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return {tr, ro, ce, cb};
+}
+
 }  // namespace
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
@@ -154,4 +210,5 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
           py::arg("input_coords"), py::arg("output_center"), py::arg("num_atoms"), st);
     m.def("Coords2CenterGPU_backward", &center_backward, "Backward of coordinates center on GPU",
           py::arg("grad_output_center"), py::arg("grad_input_coords"), py::arg("num_atoms"), st);
+    m.def("bench", &bench, "average event ms of translate, rotate, centre, centre-backward");
 }

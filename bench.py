@@ -24,7 +24,11 @@ second).  The line also carries:
 * ``rmsd``: the Coords2RMSD loss (forward + backward) on the workload's
   backbone atoms against a moved, noisy copy, with its own roofline;
 * ``xform``: the coordinate transforms (translate, rotate, centre and its
-  backward) on the same atoms, per-kernel times and bandwidth.
+  backward) on the same atoms, per-kernel times and bandwidth;
+* ``a2c``: the full-atom builder Angles2Coords (forward + backward) on the
+  reference's own benchmark shape (32 random sequences of 350 residues) and
+  on a batch of 1024 random sequences with the workload's lengths, with the
+  reference's CPU code timed beside it.
 """
 from __future__ import annotations
 
@@ -71,6 +75,7 @@ def parse():
     p.add_argument("--kernel-reps", type=int, default=200)
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--no-rmsd", action="store_true", help="skip the Coords2RMSD line")
+    p.add_argument("--no-a2c", action="store_true", help="skip the Angles2Coords line")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--config", default="", help="pin tile config 'waves,res_per_lane,max_wgs'")
@@ -300,6 +305,93 @@ def xform_bench(ksets, n, L, reps):
                          "unit": "GB/s", "frac": kern[dom]["GBs"] / HBM_PEAK_GBS}}
 
 
+AA20 = "ACDEFGHIKLMNPQRSTVWY"
+
+
+def a2c_cpu_baseline(seqs, ang, seconds):
+    """The reference's own full-atom code (FP64, one thread, compiled from its
+    sources into oracle/_ref) on structures of the given set, forward and
+    backward per structure, until the time budget is used."""
+    import numpy as np
+    from oracle import a2c_ref
+    if not a2c_ref.available():
+        return None
+    done, t_total, k = 0, 0.0, 0
+    while k < len(seqs) and t_total < seconds:
+        a1 = ang[k].astype(np.float64)
+        t0 = time.perf_counter()
+        c = a2c_ref.forward_one(seqs[k], a1)
+        a2c_ref.backward_one(seqs[k], a1, np.ones(c.size))
+        t_total += time.perf_counter() - t0
+        done += len(seqs[k])
+        k += 1
+    return {"value": done / t_total, "unit": "residues/s", "cores": 1, "kind": "reference",
+            "sample": f"{k} structures of the ref350 set, forward + backward each"}
+
+
+def a2c_bench(lengths_mixed, steps, warmup, cpu_seconds, with_cpu):
+    """Angles2Coords fwd+bwd on the GPU: HIP events around a native loop of
+    forward and backward launches over rotating buffer sets (names not
+    written).  Random sequences (uniform over the 20 amino acids) and angles
+    phi ~ N(-1.1, 0.3), psi ~ N(2.3, 0.4), omega = pi + N(0, 0.05), side-chain
+    angles ~ U(-pi, pi)."""
+    import numpy as np
+    import torch
+    from torchproteinlibrary_amd._native_xform import ext as xext
+    from torchproteinlibrary_amd.FullAtomModel.Angles2Coords import convertStringList
+
+    rng = np.random.default_rng(7)
+    dev = torch.device("cuda:0")
+
+    def make(lengths, stride):
+        seqs = ["".join(rng.choice(list(AA20), size=int(n))) for n in lengths]
+        ang = np.zeros((len(seqs), 8, stride), np.float32)
+        for b, n in enumerate(lengths):
+            ang[b, 0, :n] = rng.normal(-1.1, 0.3, n)
+            ang[b, 1, :n] = rng.normal(2.3, 0.4, n)
+            ang[b, 2, :n] = np.pi + rng.normal(0, 0.05, n)
+            ang[b, 3:, :n] = rng.uniform(-np.pi, np.pi, (5, n))
+        return seqs, ang
+
+    out = {}
+    cases = [("ref350_f32", [350] * 32, 350, torch.float32, 8),
+             ("ref350_f64", [350] * 32, 350, torch.float64, 8),
+             ("mixed1000_f32", list(lengths_mixed), int(max(lengths_mixed)), torch.float32, 3)]
+    ref_seqs = ref_ang = None
+    for name, lengths, stride, dtype, nsets in cases:
+        seqs, ang = make(lengths, stride)
+        if name == "ref350_f32":
+            ref_seqs, ref_ang = seqs, ang
+        natoms = [xext.getSeqNumAtoms(q) for q in seqs]
+        A = max(natoms)
+        S = convertStringList(seqs).to(dev)
+        sets = []
+        for _ in range(nsets):
+            X = torch.tensor(ang, dtype=dtype, device=dev)
+            sets.append([S, X, torch.empty(len(seqs), 3 * A, dtype=dtype, device=dev),
+                         torch.randn(len(seqs), 3 * A, dtype=dtype, device=dev), torch.empty_like(X)])
+        torch.cuda.synchronize()
+        st, f, b = xext.a2c_bench(sets, steps, warmup)
+        residues, atoms = int(sum(lengths)), int(sum(natoms))
+        io = 4 if dtype == torch.float32 else 8
+        # compulsory bytes: sequences and the 8 angle rows of every slot read,
+        # coordinates written for every slot (zeros past each structure); the
+        # backward reads the gradient of the valid atoms and writes all angles
+        fwd_b = len(seqs) * (S.shape[1] + 8 * stride * io + 3 * A * io)
+        bwd_b = len(seqs) * (S.shape[1] + 16 * stride * io) + 3 * atoms * io
+        out[name] = {"B": len(seqs), "residues": residues, "atoms": atoms, "ms_per_step": st,
+                     "fwd_ms": f, "bwd_ms": b, "residues_per_s": residues / (st * 1e-3),
+                     "atoms_per_s": atoms / (st * 1e-3),
+                     "hbm_frac_fwd": fwd_b / (f * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "hbm_frac_bwd": bwd_b / (b * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if with_cpu:
+        cb = a2c_cpu_baseline(ref_seqs, ref_ang, cpu_seconds)
+        if cb is not None:
+            out["cpu_baseline"] = cb
+            out["ref350_f64"]["speedup_over_cpu"] = out["ref350_f64"]["residues_per_s"] / cb["value"]
+    return out
+
+
 def main():
     args = parse()
     if args.fit:
@@ -428,6 +520,10 @@ def main():
 
     if not args.no_rmsd:
         out["xform"] = xform_bench(ksets, n, L, args.kernel_reps)
+
+    if not args.no_a2c:
+        out["a2c"] = a2c_bench(n, min(args.steps, 200), min(args.warmup, 20), args.cpu_seconds / 3,
+                               not args.no_cpu_baseline)
 
     if not args.no_plain:
         fr_b = ext.plain_frames_bytes(False, B, L)

@@ -1,0 +1,170 @@
+"""GPU parity of FullAtomModel.Angles2Coords (forward and backward) against
+the FP64 oracle (oracle/a2c_oracle.py, pinned to the reference's compiled
+code on the CPU), the reference's own backward test, and edge cases."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import a2c_oracle as O
+from tests import abi_a2c
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def _batch(rng, lengths, stride, dtype):
+    seqs = ["".join(rng.choice(list(O.AA20), size=n)) for n in lengths]
+    ang = np.zeros((len(seqs), 8, stride))
+    for b, n in enumerate(lengths):
+        ang[b, 0, :n] = rng.normal(-1.1, 0.5, n)
+        ang[b, 1, :n] = rng.normal(2.3, 0.5, n)
+        ang[b, 2, :n] = np.pi + rng.normal(0, 0.1, n)
+        ang[b, 3:, :n] = rng.uniform(-np.pi, np.pi, (5, n))
+    np_dt = np.float32 if dtype == torch.float32 else np.float64
+    return seqs, ang.astype(np_dt).astype(np.float64)
+
+
+def _names(t):
+    return [bytes(row).rstrip(b"\0").decode() for row in t.cpu().numpy()]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_parity_with_oracle(dtype):
+    from torchproteinlibrary_amd.FullAtomModel import Angles2Coords
+    rng = np.random.default_rng(1)
+    lengths = [1, 2, 20, 57, 80, 33, 5, 64, 3, 41, 77, 12]
+    print(f"a2c parity: B={len(lengths)} dtype={dtype}", flush=True)
+    seqs, ang = _batch(rng, lengths, 83, dtype)
+    seqs[2] = O.AA20
+    X = torch.tensor(ang, dtype=dtype, device=DEV, requires_grad=True)
+    coords, res_names, atom_names, num_atoms = Angles2Coords()(X, seqs)
+    want, counts = O.forward(seqs, ang)
+    assert num_atoms.cpu().tolist() == counts.tolist()
+    assert coords.shape == want.shape and res_names.shape == (len(seqs), counts.max(), 4)
+    c = coords.detach().cpu().numpy()
+    tol = 1e-10 if dtype == torch.float64 else 3e-5
+    assert np.abs(c - want).max() <= tol
+    for b, s in enumerate(seqs):
+        res3, atoms = O.names_one(s)
+        n = counts[b]
+        assert _names(atom_names[b, :n]) == atoms and _names(res_names[b, :n]) == res3
+        assert not atom_names[b, n:].cpu().numpy().any() and not c[b, 3 * n:].any()
+    g = rng.normal(size=c.shape)
+    for b, n in enumerate(counts):
+        g[b, 3 * n:] = 0.0
+    coords.backward(torch.tensor(g, dtype=dtype, device=DEV))
+    ga = X.grad.cpu().numpy()
+    want_g = O.backward(seqs, ang, g)
+    scale = np.abs(want_g).max(axis=(1, 2), keepdims=True)
+    rel = np.abs(ga - want_g) / np.maximum(scale, 1e-30)
+    assert rel.max() <= (1e-10 if dtype == torch.float64 else 1e-5), rel.max()
+    for b, n in enumerate(lengths):  # exact zeros past the sequence and for omega of the last residue
+        assert not ga[b, :, n:].any() and ga[b, 2, n - 1] == 0
+
+
+def test_reference_backward_test():
+    """UnitTests/FullAtomModel/Angles2Coords/test.py:83-142 on the GPU: the
+    gradient of sum(coords) against forward differences, mean error < 0.01."""
+    from torchproteinlibrary_amd.FullAtomModel import Angles2Coords
+    seqs = ["ACDEFGHIKLMNPQRSTVWY"]
+    x0 = torch.zeros(1, 8, 20, dtype=torch.float64, device=DEV)
+    x0[:, 0], x0[:, 1], x0[:, 2], x0[:, 3:] = -1.047, -0.698, np.pi, 110.4 * np.pi / 180.0
+    x0.requires_grad_()
+    a2c = Angles2Coords()
+    y0 = a2c(x0, seqs)[0].sum()
+    y0.backward()
+    err = []
+    with torch.no_grad():
+        for i in range(8):
+            for j in range(20):
+                x1 = x0.detach().clone()
+                x1[0, i, j] += 1e-5
+                dy = (a2c(x1, seqs)[0].sum().item() - y0.item()) / 1e-5
+                err.append(abs(dy - x0.grad[0, i, j].item()))
+    assert np.mean(err) < 0.01
+
+
+def test_gradcheck_fp64():
+    from torchproteinlibrary_amd.FullAtomModel.Angles2Coords import (Angles2CoordsFunction,
+                                                                     convertStringList)
+    seqs = ["MKW", "PG"]
+    rng = np.random.default_rng(4)
+    _, ang = _batch(rng, [3, 2], 3, torch.float64)
+    X = torch.tensor(ang, device=DEV, requires_grad=True)
+    S = convertStringList(seqs).to(DEV)
+    N = torch.tensor([O.num_atoms(s) for s in seqs], dtype=torch.int32, device=DEV)
+    assert torch.autograd.gradcheck(lambda a: Angles2CoordsFunction.apply(a, S, N)[0], (X,), eps=1e-6,
+                                    atol=1e-6)
+
+
+def test_c_abi_direct_matches_layer():
+    from torchproteinlibrary_amd.FullAtomModel import Angles2Coords
+    from torchproteinlibrary_amd.FullAtomModel.Angles2Coords import convertStringList
+    lib = abi_a2c.load()
+    rng = np.random.default_rng(2)
+    seqs, ang = _batch(rng, [10, 30, 7], 30, torch.float32)
+    X = torch.tensor(ang, dtype=torch.float32, device=DEV)
+    coords, rn, an, n = Angles2Coords()(X, seqs)
+    S = convertStringList(seqs).to(DEV)
+    A = coords.shape[1] // 3
+    out = torch.full_like(coords, 7.0)
+    st = torch.zeros(1, dtype=torch.int32, device=DEV)
+    torch.cuda.synchronize()
+    assert lib.tpl_a2c_forward(0, S.data_ptr(), S.shape[1], X.data_ptr(), 30, out.data_ptr(), None, None, A,
+                               st.data_ptr(), 3, None) == 0
+    g = torch.randn_like(coords)
+    ga = torch.full_like(X, 7.0)
+    assert lib.tpl_a2c_backward(0, S.data_ptr(), S.shape[1], X.data_ptr(), 30, g.data_ptr(), A, ga.data_ptr(),
+                                st.data_ptr(), 3, None) == 0
+    torch.cuda.synchronize()
+    assert int(st.item()) == 0
+    assert torch.equal(out, coords)
+    X2 = X.clone().requires_grad_()
+    Angles2Coords()(X2, seqs)[0].backward(g)
+    assert torch.equal(ga, X2.grad)
+
+
+def test_edge_cases_and_errors():
+    from torchproteinlibrary_amd.FullAtomModel import Angles2Coords
+    a2c = Angles2Coords()
+    X = torch.zeros(3, 8, 5, device=DEV)
+    coords, rn, an, n = a2c(X, ["", "G", "WWWWW"])  # an empty sequence gives no atoms
+    assert n.tolist() == [0, 4, 70] and coords.shape == (3, 210)
+    assert not coords[0].any() and not coords[1, 12:].any()
+    with pytest.raises(ValueError):
+        a2c(X, ["AXG", "G", "G"])  # unknown letter
+    with pytest.raises(ValueError):
+        a2c(torch.zeros(1, 8, 3, device=DEV), ["GGGG"])  # longer than the angles
+    with pytest.raises(ValueError):
+        a2c(torch.zeros(1, 3, 5, device=DEV), ["GG"])  # rank / rows
+    with pytest.raises(RuntimeError):
+        a2c(torch.zeros(1, 8, 5), ["GG"])  # GPU only
+
+
+def test_batch_mixed_lengths_full_size():
+    """B=256, lengths log-uniform in [30, 1000] (FP32): backbone bond lengths
+    of every structure, and oracle parity on a sample of structures."""
+    from torchproteinlibrary_amd.FullAtomModel import Angles2Coords
+    seqs, ang = O.workload(256, seed=9, dtype=np.float32)
+    print(f"a2c full size: B=256 residues={sum(map(len, seqs))}", flush=True)
+    X = torch.tensor(ang, dtype=torch.float32, device=DEV, requires_grad=True)
+    coords, rn, an, n = Angles2Coords()(X, seqs)
+    c = coords.detach().cpu().numpy().astype(np.float64)
+    for b in range(0, 256, 7):
+        s = seqs[b]
+        idx = np.cumsum([0] + [len(O.residue_atoms(a)) for a in s])[:-1]
+        X3 = c[b].reshape(-1, 3)
+        assert np.abs(np.linalg.norm(X3[idx + 1] - X3[idx], axis=1) - 1.46).max() < 1e-3
+    for b in (0, 17, 101):
+        want, _, _ = O.forward_one(seqs[b], ang[b])
+        assert np.abs(c[b, :want.size] - want).max() < 1e-3
+    g = torch.randn_like(coords)
+    coords.backward(g)
+    assert torch.isfinite(X.grad).all()
+    small = [b for b in range(256) if len(seqs[b]) <= 60][:2]
+    for b in small:
+        want = O.backward_one(seqs[b], ang[b], g[b].cpu().numpy().astype(np.float64))
+        got = X.grad[b].cpu().numpy()
+        assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()

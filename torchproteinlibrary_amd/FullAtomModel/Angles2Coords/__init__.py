@@ -1,0 +1,1 @@
+from .Angles2Coords import *  # noqa: F401,F403

@@ -7,8 +7,10 @@
   tensors and calls that C ABI; links libtpl_a2b.so with an $ORIGIN rpath.
 * ``libtpl_rmsd.so`` and ``_RMSD<EXT_SUFFIX>`` -- the same pair for
   Coords2RMSD (include/tpl_rmsd.h).
-* ``libtpl_xform.so`` and ``_FullAtomModel<EXT_SUFFIX>`` -- the same pair for
-  the coordinate transforms (include/tpl_xform.h).
+* ``libtpl_xform.so``, ``libtpl_a2c.so`` and ``_FullAtomModel<EXT_SUFFIX>``
+  -- the coordinate transforms (include/tpl_xform.h) and the full-atom builder
+  Angles2Coords (include/tpl_a2c.h), one extension over both libraries as in
+  the reference's module of that name.
 
 Run ``python -m torchproteinlibrary_amd._build``.  Every call recompiles.
 """
@@ -34,12 +36,14 @@ RMSD_LIB_NAME = "libtpl_rmsd.so"
 RMSD_EXT_NAME = "_RMSD"
 XFORM_LIB_NAME = "libtpl_xform.so"
 XFORM_EXT_NAME = "_FullAtomModel"
+A2C_LIB_NAME = "libtpl_a2c.so"
 
-# (library, its HIP source, extension, its C++ source)
+# ([(library, its HIP source)], extension, [its C++ sources])
 PARTS = [
-    (LIB_NAME, "a2b.hip", EXT_NAME, "a2b_ext.cpp"),
-    (RMSD_LIB_NAME, "rmsd.hip", RMSD_EXT_NAME, "rmsd_ext.cpp"),
-    (XFORM_LIB_NAME, "xform.hip", XFORM_EXT_NAME, "xform_ext.cpp"),
+    ([(LIB_NAME, "a2b.hip")], EXT_NAME, ["a2b_ext.cpp"]),
+    ([(RMSD_LIB_NAME, "rmsd.hip")], RMSD_EXT_NAME, ["rmsd_ext.cpp"]),
+    ([(XFORM_LIB_NAME, "xform.hip"), (A2C_LIB_NAME, "a2c.hip")], XFORM_EXT_NAME,
+     ["xform_ext.cpp", "a2c_ext.cpp"]),
 ]
 
 
@@ -86,8 +90,10 @@ def build_lib(verbose=True, name: str = LIB_NAME, source: str = "a2b.hip") -> st
     return out
 
 
-def build_ext(verbose=True, name: str = EXT_NAME, source: str = "a2b_ext.cpp",
-              lib: str = LIB_NAME) -> str:
+def build_ext(verbose=True, name: str = EXT_NAME, source="a2b_ext.cpp", lib=LIB_NAME) -> str:
+    """``source`` and ``lib`` are one name or a list of names."""
+    sources = [source] if isinstance(source, str) else list(source)
+    libs = [lib] if isinstance(lib, str) else list(lib)
     import torch
     from torch.utils import cpp_extension
 
@@ -105,8 +111,8 @@ def build_ext(verbose=True, name: str = EXT_NAME, source: str = "a2b_ext.cpp",
         "-Wno-unused-variable", "-fvisibility=hidden",
         "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1", "-DTORCH_API_INCLUDE_EXTENSION_H",
         f"-DTORCH_EXTENSION_NAME={name}", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
-        *incs, os.path.join(CSRC, source), "-o", tmp,
-        "-L", PKG, "-l:" + lib, "-Wl,-rpath,$ORIGIN",
+        *incs, *[os.path.join(CSRC, f) for f in sources], "-o", tmp,
+        "-L", PKG, *["-l:" + f for f in libs], "-Wl,-rpath,$ORIGIN",
         "-L", torch_lib, "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-ltorch_python",
         "-Wl,-rpath," + torch_lib,
         "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
@@ -124,16 +130,19 @@ def build(verbose=True):
     """Compile each HIP library, then the extension that links it (the parts
     in parallel)."""
     def part(p):
-        lib, hip, ext, cpp = p
-        build_lib(verbose, lib, hip)
-        build_ext(verbose, ext, cpp, lib)
+        libs, ext, cpps = p
+        with ThreadPoolExecutor(len(libs)) as pool:
+            for f in [pool.submit(build_lib, verbose, lib, hip) for lib, hip in libs]:
+                f.result()
+        build_ext(verbose, ext, cpps, [lib for lib, _ in libs])
     with ThreadPoolExecutor(len(PARTS)) as pool:
         for f in [pool.submit(part, p) for p in PARTS]:
             f.result()
 
 
 def built() -> bool:
-    return all(os.path.exists(lib_path(p[0])) and os.path.exists(ext_path(p[2])) for p in PARTS)
+    return all(all(os.path.exists(lib_path(lib)) for lib, _ in p[0]) and os.path.exists(ext_path(p[1]))
+               for p in PARTS)
 
 
 if __name__ == "__main__":

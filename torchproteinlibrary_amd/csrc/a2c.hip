@@ -1,0 +1,599 @@
+// Full-atom protein builder for MI355X (gfx950): FullAtomModel.Angles2Coords
+// forward and backward.  The C ABI is include/tpl_a2c.h.
+//
+// The reference builds a tree of heap-allocated 4x4 transforms per structure
+// on the CPU and walks it recursively (Layers/FullAtomModel/cConformation.cpp:
+// 29-158); its backward visits, for every angle, every atom below it
+// (:160-195), O(atoms x tree depth) with a chain's depth growing with its
+// length.  Here one workgroup takes one structure:
+//
+//   1. each thread takes a contiguous run of residues and multiplies their
+//      residue steps S_i = T_CA(phi_i) T_C(psi_i) T_N(omega_i) (FP64 3x4), the
+//      transform from residue i's N frame to residue i+1's;
+//   2. a workgroup scan of those products (Hillis-Steele through LDS) gives
+//      each thread the N frame of its first residue, and a scan of the atom
+//      counts gives its first atom;
+//   3. each thread walks its residues: N, CA, the side chain, C, writing atoms.
+//
+// The backward uses dL/dalpha = u . (sum_{x in subtree} x cross g
+// - p cross sum_{x in subtree} g) for a node with x axis u and origin p (the
+// reference's g . (F x) with F = M_parent dT M^-1, cConformation.cpp:148,
+// 171-184, reduces to this since F x = u cross (x - p)).  The subtree of a
+// backbone node is the rest of its residue plus every later residue, so a
+// second workgroup scan, a suffix sum of per-run (sum g, sum x cross g),
+// supplies the downstream part, and each thread walks its residues again.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "a2c_table.h"
+#include "tpl_a2c.h"
+
+namespace {
+
+using namespace a2c;
+
+constexpr int kThreads = 256;
+
+__constant__ Residue kRes[kTypes] = A2C_RESIDUES;
+__constant__ Side kSide[kTypes][kMaxSide] = A2C_SIDES;
+__constant__ Atom kAtom[kTypes][kMaxAtoms] = A2C_ATOMS;
+
+// Workgroup-shared tables, filled once per workgroup from the constant ones:
+// local atom positions, and per side-chain node cos/sin of beta and of a
+// constant alpha, d, and the packed (row, first atom, count).
+struct Tables {
+    double loc[kTypes * kMaxAtoms][3];
+    double side[kTypes * kMaxSide][5];
+    int side_i[kTypes * kMaxSide];
+    int res_i[kTypes];  // natoms | nca << 8 | nside << 16
+    double bb[3][2];    // cos/sin of beta for the CA, C and N transforms
+};
+
+__device__ void fill_tables(Tables& tb) {
+    for (int k = threadIdx.x; k < kTypes * kMaxAtoms; k += kThreads) {
+        const Atom at = kAtom[k / kMaxAtoms][k % kMaxAtoms];
+        local_position(at, tb.loc[k]);
+    }
+    for (int k = threadIdx.x; k < kTypes * kMaxSide; k += kThreads) {
+        const Side s = kSide[k / kMaxSide][k % kMaxSide];
+        sincos(s.beta, &tb.side[k][1], &tb.side[k][0]);
+        sincos(s.alpha, &tb.side[k][3], &tb.side[k][2]);
+        tb.side[k][4] = s.d;
+        tb.side_i[k] = (s.row & 0xff) | (s.first << 8) | (s.count << 16);
+    }
+    if (threadIdx.x < kTypes) {
+        const Residue r = kRes[threadIdx.x];
+        tb.res_i[threadIdx.x] = r.natoms | (r.nca << 8) | (r.nside << 16);
+    }
+    if (threadIdx.x == 0) {
+        sincos(geo::C_N_CA, &tb.bb[0][1], &tb.bb[0][0]);
+        sincos(geo::N_CA_C, &tb.bb[1][1], &tb.bb[1][0]);
+        sincos(geo::CA_C_N, &tb.bb[2][1], &tb.bb[2][0]);
+    }
+}
+
+// m (3x4, row-major) <- m * Ry(beta) Tx(d) Rx(alpha), with cb/sb of beta and
+// ca/sa of alpha.  Ry(b) Tx(d) Rx(a) = [[cb, sb sa, sb ca, d cb],
+// [0, ca, -sa, 0], [-sb, cb sa, cb ca, -d sb]].
+__device__ __forceinline__ void step(double m[12], double cb, double sb, double ca, double sa,
+                                     double d) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double a0 = m[4 * i], a1 = m[4 * i + 1], a2 = m[4 * i + 2];
+        const double n0 = a0 * cb - a2 * sb;
+        const double t = a0 * sb + a2 * cb;
+        m[4 * i] = n0;
+        m[4 * i + 1] = t * sa + a1 * ca;
+        m[4 * i + 2] = t * ca - a1 * sa;
+        m[4 * i + 3] += d * n0;
+    }
+}
+
+// c = a * b for 3x4 rigid transforms.
+__device__ __forceinline__ void compose(const double a[12], const double b[12], double c[12]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            c[4 * i + j] = a[4 * i] * b[j] + a[4 * i + 1] * b[4 + j] + a[4 * i + 2] * b[8 + j] +
+                           (j == 3 ? a[4 * i + 3] : 0.0);
+    }
+}
+
+__device__ __forceinline__ void identity(double m[12]) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) m[k] = (k == 0 || k == 5 || k == 10) ? 1.0 : 0.0;
+}
+
+__device__ __forceinline__ void apply(const double m[12], const double v[3], double x[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) x[i] = m[4 * i] * v[0] + m[4 * i + 1] * v[1] + m[4 * i + 2] * v[2] + m[4 * i + 3];
+}
+
+template <typename T>
+__device__ __forceinline__ double angle(const T* ang, int row, int stride, int i) {
+    return (double)ang[(size_t)row * stride + i];
+}
+
+// The residue step S_i applied to m: m <- m T_CA(phi) T_C(psi) T_N(omega).
+template <typename T>
+__device__ __forceinline__ void residue_step(double m[12], const Tables& tb, const T* ang, int stride,
+                                             int i) {
+    double s, c;
+    sincos(angle(ang, 0, stride, i), &s, &c);
+    step(m, tb.bb[0][0], tb.bb[0][1], c, s, geo::R_N_CA);
+    sincos(angle(ang, 1, stride, i), &s, &c);
+    step(m, tb.bb[1][0], tb.bb[1][1], c, s, geo::R_CA_C);
+    sincos(angle(ang, 2, stride, i), &s, &c);
+    step(m, tb.bb[2][0], tb.bb[2][1], c, s, geo::R_C_N);
+}
+
+// Per structure: the sequence length (first 0 byte), the residues' validity,
+// this thread's run [r0, r1), and the workgroup scans giving the N frame of
+// residue r0 (frame) and its first atom (atom0).  Returns false (for every
+// thread of the workgroup) if the structure is flagged.
+struct Run {
+    int L, r0, r1, atom0, total;
+};
+
+template <typename T>
+__device__ bool prepare(const uint8_t* seq, int seq_stride, const T* ang, int ang_stride,
+                        int atoms_stride, uint32_t* status, const Tables& tb, double (*s_m)[kThreads],
+                        int* s_c, int* s_flag, Run& run, double frame[12]) {
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        s_flag[0] = seq_stride;  // length
+        s_flag[1] = 0;           // status bits
+    }
+    __syncthreads();
+    for (int i = tid; i < seq_stride; i += kThreads)
+        if (seq[i] == 0) atomicMin(&s_flag[0], i);
+    __syncthreads();
+    int L = s_flag[0];
+    if (L > ang_stride) {
+        if (tid == 0) atomicOr(&s_flag[1], (int)TPL_A2C_STATUS_LENGTH);
+        L = 0;
+    }
+    const int chunk = (L + kThreads - 1) / kThreads;
+    const int r0 = min(L, tid * chunk), r1 = min(L, r0 + chunk);
+    double P[12];
+    identity(P);
+    int cnt = 0;
+    bool bad = false;
+    for (int i = r0; i < r1; ++i) {
+        const int t = residue_type(seq[i]);
+        if (t < 0) {
+            bad = true;
+            continue;
+        }
+        cnt += tb.res_i[t] & 0xff;
+        residue_step(P, tb, ang, ang_stride, i);
+    }
+    if (bad) atomicOr(&s_flag[1], (int)TPL_A2C_STATUS_SEQ);
+    // inclusive scan: P_t <- P_0 ... P_t (ordered product), cnt likewise
+#pragma unroll
+    for (int k = 0; k < 12; ++k) s_m[k][tid] = P[k];
+    s_c[tid] = cnt;
+    for (int off = 1; off < kThreads; off <<= 1) {
+        __syncthreads();
+        double Q[12];
+        int c2 = 0;
+        if (tid >= off) {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) Q[k] = s_m[k][tid - off];
+            c2 = s_c[tid - off];
+        }
+        __syncthreads();
+        if (tid >= off) {
+            double R[12];
+            compose(Q, P, R);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                P[k] = R[k];
+                s_m[k][tid] = R[k];
+            }
+            cnt += c2;
+            s_c[tid] = cnt;
+        }
+    }
+    __syncthreads();
+    run.total = s_c[kThreads - 1];
+    if (run.total > atoms_stride && tid == 0) atomicOr(&s_flag[1], (int)TPL_A2C_STATUS_ATOMS);
+    if (tid == 0) {
+        identity(frame);
+        run.atom0 = 0;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) frame[k] = s_m[k][tid - 1];
+        run.atom0 = s_c[tid - 1];
+    }
+    __syncthreads();
+    const int flags = s_flag[1];
+    if (flags && tid == 0 && status) __hip_atomic_fetch_or(status, (uint32_t)flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    run.L = L;
+    run.r0 = r0;
+    run.r1 = r1;
+    if (flags) {
+        run.L = run.r0 = run.r1 = run.total = 0;
+        return false;
+    }
+    return true;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) a2c_fwd(const uint8_t* __restrict__ seqs, int seq_stride,
+                                                    const T* __restrict__ angles, int ang_stride,
+                                                    T* __restrict__ coords, uint32_t* __restrict__ res_names,
+                                                    uint32_t* __restrict__ atom_names, int atoms_stride,
+                                                    uint32_t* status) {
+    __shared__ Tables tb;
+    __shared__ double s_m[12][kThreads];
+    __shared__ int s_c[kThreads];
+    __shared__ int s_flag[2];
+    const int b = blockIdx.x;
+    const uint8_t* seq = seqs + (size_t)b * seq_stride;
+    const T* ang = angles + (size_t)b * 8 * ang_stride;
+    T* out = coords + (size_t)b * 3 * atoms_stride;
+    fill_tables(tb);
+    __syncthreads();
+    Run run;
+    double G[12];
+    prepare(seq, seq_stride, ang, ang_stride, atoms_stride, status, tb, s_m, s_c, s_flag, run, G);
+    int a = run.atom0;
+    uint32_t* rn = res_names ? res_names + (size_t)b * atoms_stride : nullptr;
+    uint32_t* an = atom_names ? atom_names + (size_t)b * atoms_stride : nullptr;
+    for (int i = run.r0; i < run.r1; ++i) {
+        const int t = residue_type(seq[i]);
+        const int ri = tb.res_i[t];
+        const int natoms = ri & 0xff, nca = (ri >> 8) & 0xff, nside = ri >> 16;
+        const double* loc = tb.loc[t * kMaxAtoms];
+        double x[3];
+        auto put = [&](int slot, const double m[12]) {
+            apply(m, loc + 3 * slot, x);
+            T* o = out + 3 * (size_t)(a + slot);
+            o[0] = (T)x[0];
+            o[1] = (T)x[1];
+            o[2] = (T)x[2];
+        };
+        put(0, G);
+        double s, c, F[12], CA[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) CA[k] = G[k];
+        sincos(angle(ang, 0, ang_stride, i), &s, &c);
+        step(CA, tb.bb[0][0], tb.bb[0][1], c, s, geo::R_N_CA);
+        for (int k = 1; k <= nca; ++k) put(k, CA);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) F[k] = CA[k];
+        for (int n = 0; n < nside; ++n) {
+            const int q = t * kMaxSide + n;
+            const int si = tb.side_i[q];
+            const int row = (int)(int8_t)(si & 0xff), first = (si >> 8) & 0xff, count = si >> 16;
+            double ca = tb.side[q][2], sa = tb.side[q][3];
+            if (row >= 0) sincos(angle(ang, row, ang_stride, i), &sa, &ca);
+            step(F, tb.side[q][0], tb.side[q][1], ca, sa, tb.side[q][4]);
+            for (int k = first; k < first + count; ++k) put(k, F);
+        }
+        sincos(angle(ang, 1, ang_stride, i), &s, &c);
+        step(CA, tb.bb[1][0], tb.bb[1][1], c, s, geo::R_CA_C);  // CA now holds the C frame
+        put(natoms - 2, CA);
+        put(natoms - 1, CA);
+        if (rn) {
+            const uint32_t r3 = *reinterpret_cast<const uint32_t*>(kRes[t].name3);
+            for (int k = 0; k < natoms; ++k) {
+                rn[a + k] = r3;
+                an[a + k] = *reinterpret_cast<const uint32_t*>(kAtom[t][k].name);
+            }
+        }
+        sincos(angle(ang, 2, ang_stride, i), &s, &c);
+        step(CA, tb.bb[2][0], tb.bb[2][1], c, s, geo::R_C_N);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) G[k] = CA[k];
+        a += natoms;
+    }
+    for (int k = 3 * run.total + threadIdx.x; k < 3 * atoms_stride; k += kThreads) out[k] = (T)0;
+    if (rn)
+        for (int k = run.total + threadIdx.x; k < atoms_stride; k += kThreads) rn[k] = an[k] = 0u;
+}
+
+struct Sums {
+    double g[3], xg[3];
+};
+
+__device__ __forceinline__ void sums_zero(Sums& s) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s.g[c] = s.xg[c] = 0.0;
+}
+
+__device__ __forceinline__ void sums_add(Sums& s, const double x[3], const double g[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s.g[c] += g[c];
+    s.xg[0] += x[1] * g[2] - x[2] * g[1];
+    s.xg[1] += x[2] * g[0] - x[0] * g[2];
+    s.xg[2] += x[0] * g[1] - x[1] * g[0];
+}
+
+// u . (xg - p cross g) for the frame m (u = its x axis, p = its origin).
+__device__ __forceinline__ double torque(const double m[12], const double g[3], const double xg[3]) {
+    const double p0 = m[3], p1 = m[7], p2 = m[11];
+    const double v0 = xg[0] - (p1 * g[2] - p2 * g[1]);
+    const double v1 = xg[1] - (p2 * g[0] - p0 * g[2]);
+    const double v2 = xg[2] - (p0 * g[1] - p1 * g[0]);
+    return m[0] * v0 + m[4] * v1 + m[8] * v2;
+}
+
+// The residue's atom sums by group: N, the CA group, the side chain and the
+// C group, with the residue's frames (N = G given; CA and C returned).
+template <typename T>
+__device__ __forceinline__ void residue_sums(const Tables& tb, int t, const T* ang, int ang_stride, int i,
+                                             const T* gr, int a, const double G[12], double CA[12],
+                                             double C[12], Sums& sN, Sums& sCA, Sums& sSide, Sums& sC) {
+    const int ri = tb.res_i[t];
+    const int natoms = ri & 0xff, nca = (ri >> 8) & 0xff, nside = ri >> 16;
+    const double* loc = tb.loc[t * kMaxAtoms];
+    double x[3], g[3];
+    auto take = [&](int slot, const double m[12], Sums& acc) {
+        apply(m, loc + 3 * slot, x);
+        const T* p = gr + 3 * (size_t)(a + slot);
+        g[0] = (double)p[0];
+        g[1] = (double)p[1];
+        g[2] = (double)p[2];
+        sums_add(acc, x, g);
+    };
+    sums_zero(sN);
+    sums_zero(sCA);
+    sums_zero(sSide);
+    sums_zero(sC);
+    take(0, G, sN);
+    double s, c, F[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) CA[k] = G[k];
+    sincos(angle(ang, 0, ang_stride, i), &s, &c);
+    step(CA, tb.bb[0][0], tb.bb[0][1], c, s, geo::R_N_CA);
+    for (int k = 1; k <= nca; ++k) take(k, CA, sCA);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) F[k] = CA[k];
+    for (int n = 0; n < nside; ++n) {
+        const int q = t * kMaxSide + n;
+        const int si = tb.side_i[q];
+        const int row = (int)(int8_t)(si & 0xff), first = (si >> 8) & 0xff, count = si >> 16;
+        double ca = tb.side[q][2], sa = tb.side[q][3];
+        if (row >= 0) sincos(angle(ang, row, ang_stride, i), &sa, &ca);
+        step(F, tb.side[q][0], tb.side[q][1], ca, sa, tb.side[q][4]);
+        for (int k = first; k < first + count; ++k) take(k, F, sSide);
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) C[k] = CA[k];
+    sincos(angle(ang, 1, ang_stride, i), &s, &c);
+    step(C, tb.bb[1][0], tb.bb[1][1], c, s, geo::R_CA_C);
+    take(natoms - 2, C, sC);
+    take(natoms - 1, C, sC);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) a2c_bwd(const uint8_t* __restrict__ seqs, int seq_stride,
+                                                    const T* __restrict__ angles, int ang_stride,
+                                                    const T* __restrict__ grad_coords, int atoms_stride,
+                                                    T* __restrict__ grad_angles, uint32_t* status) {
+    __shared__ Tables tb;
+    __shared__ double s_m[12][kThreads];
+    __shared__ int s_c[kThreads];
+    __shared__ int s_flag[2];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const uint8_t* seq = seqs + (size_t)b * seq_stride;
+    const T* ang = angles + (size_t)b * 8 * ang_stride;
+    const T* gr = grad_coords + (size_t)b * 3 * atoms_stride;
+    T* ga = grad_angles + (size_t)b * 8 * ang_stride;
+    fill_tables(tb);
+    __syncthreads();
+    Run run;
+    double G0[12], G[12], CA[12], C[12];
+    prepare(seq, seq_stride, ang, ang_stride, atoms_stride, status, tb, s_m, s_c, s_flag, run, G0);
+    // pass A: this run's (sum g, sum x cross g)
+    Sums run_sum, sN, sCA, sSide, sC;
+    sums_zero(run_sum);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) G[k] = G0[k];
+    int a = run.atom0;
+    for (int i = run.r0; i < run.r1; ++i) {
+        const int t = residue_type(seq[i]);
+        residue_sums(tb, t, ang, ang_stride, i, gr, a, G, CA, C, sN, sCA, sSide, sC);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            run_sum.g[c] += sN.g[c] + sCA.g[c] + sSide.g[c] + sC.g[c];
+            run_sum.xg[c] += sN.xg[c] + sCA.xg[c] + sSide.xg[c] + sC.xg[c];
+        }
+        double s, c;
+        sincos(angle(ang, 2, ang_stride, i), &s, &c);
+        step(C, tb.bb[2][0], tb.bb[2][1], c, s, geo::R_C_N);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) G[k] = C[k];
+        a += tb.res_i[t] & 0xff;
+    }
+    // suffix scan: down = sums over the runs after this one (reusing s_m)
+    double v[6];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        v[c] = run_sum.g[c];
+        v[3 + c] = run_sum.xg[c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s_m[k][tid] = v[k];
+    for (int off = 1; off < kThreads; off <<= 1) {
+        __syncthreads();
+        double w[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) w[k] = tid + off < kThreads ? s_m[k][tid + off] : 0.0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            v[k] += w[k];
+            s_m[k][tid] = v[k];
+        }
+    }
+    __syncthreads();
+    // incl: sums over every atom from this run's first residue on
+    Sums incl;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        incl.g[c] = s_m[c][tid];
+        incl.xg[c] = s_m[3 + c][tid];
+    }
+    // pass B: gradients
+#pragma unroll
+    for (int k = 0; k < 12; ++k) G[k] = G0[k];
+    a = run.atom0;
+    for (int i = run.r0; i < run.r1; ++i) {
+        const int t = residue_type(seq[i]);
+        residue_sums(tb, t, ang, ang_stride, i, gr, a, G, CA, C, sN, sCA, sSide, sC);
+        Sums excl, s;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            excl.g[c] = incl.g[c] - (sN.g[c] + sCA.g[c] + sSide.g[c] + sC.g[c]);
+            excl.xg[c] = incl.xg[c] - (sN.xg[c] + sCA.xg[c] + sSide.xg[c] + sC.xg[c]);
+        }
+        for (int r = 3; r < 8; ++r) ga[(size_t)r * ang_stride + i] = (T)0;
+        // omega of residue i-1 binds N_i: the whole residue and everything after
+        if (i > 0) ga[(size_t)2 * ang_stride + i - 1] = (T)torque(G, incl.g, incl.xg);
+        if (i == run.L - 1) ga[(size_t)2 * ang_stride + i] = (T)0;
+        // phi binds CA: all but N
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            s.g[c] = incl.g[c] - sN.g[c];
+            s.xg[c] = incl.xg[c] - sN.xg[c];
+        }
+        ga[i] = (T)torque(CA, s.g, s.xg);
+        // psi binds C: the C group and everything after
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            s.g[c] = sC.g[c] + excl.g[c];
+            s.xg[c] = sC.xg[c] + excl.xg[c];
+        }
+        ga[(size_t)ang_stride + i] = (T)torque(C, s.g, s.xg);
+        // side chain: node n binds its group and the later nodes' groups
+        const int ri = tb.res_i[t];
+        const int nside = ri >> 16;
+        if (nside) {
+            const double* loc = tb.loc[t * kMaxAtoms];
+            double F[12], x[3], g[3];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) F[k] = CA[k];
+            Sums rest = sSide;
+            for (int n = 0; n < nside; ++n) {
+                const int q = t * kMaxSide + n;
+                const int si = tb.side_i[q];
+                const int row = (int)(int8_t)(si & 0xff), first = (si >> 8) & 0xff, count = si >> 16;
+                double ca = tb.side[q][2], sa = tb.side[q][3];
+                if (row >= 0) sincos(angle(ang, row, ang_stride, i), &sa, &ca);
+                step(F, tb.side[q][0], tb.side[q][1], ca, sa, tb.side[q][4]);
+                if (row >= 0) ga[(size_t)row * ang_stride + i] = (T)torque(F, rest.g, rest.xg);
+                Sums mine;
+                sums_zero(mine);
+                for (int k = first; k < first + count; ++k) {
+                    apply(F, loc + 3 * k, x);
+                    const T* p = gr + 3 * (size_t)(a + k);
+                    g[0] = (double)p[0];
+                    g[1] = (double)p[1];
+                    g[2] = (double)p[2];
+                    sums_add(mine, x, g);
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    rest.g[c] -= mine.g[c];
+                    rest.xg[c] -= mine.xg[c];
+                }
+            }
+        }
+        incl = excl;
+        double sn, cs;
+        sincos(angle(ang, 2, ang_stride, i), &sn, &cs);
+        step(C, tb.bb[2][0], tb.bb[2][1], cs, sn, geo::R_C_N);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) G[k] = C[k];
+        a += ri & 0xff;
+    }
+    // angles past the sequence (and every angle of a flagged structure)
+    for (int k = tid; k < 8 * ang_stride; k += kThreads)
+        if (k % ang_stride >= run.L) ga[k] = (T)0;
+}
+
+int check(int dtype, int B, int seq_stride, int ang_stride, int atoms_stride) {
+    if (dtype != TPL_A2C_F32 && dtype != TPL_A2C_F64) return TPL_A2C_ERR_ARG;
+    if (B < 0 || seq_stride < 0 || ang_stride < 0 || atoms_stride < 0) return TPL_A2C_ERR_ARG;
+    if ((int64_t)8 * ang_stride > INT32_MAX || (int64_t)3 * atoms_stride > INT32_MAX)
+        return TPL_A2C_ERR_SIZE;
+    return TPL_A2C_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tpl_a2c_strerror(int code) {
+    switch (code) {
+        case TPL_A2C_OK: return "ok";
+        case TPL_A2C_ERR_ARG: return "invalid argument (null pointer, negative size or unknown dtype)";
+        case TPL_A2C_ERR_LAUNCH: return "kernel launch failed";
+        case TPL_A2C_ERR_SIZE: return "sizes too large";
+        default: return "unknown error code";
+    }
+}
+
+int tpl_a2c_num_atoms(const char* sequence) {
+    static const Residue res[kTypes] = A2C_RESIDUES;
+    if (!sequence) return -1;
+    int n = 0;
+    for (const char* p = sequence; *p; ++p) {
+        const int t = residue_type((uint8_t)*p);
+        if (t < 0) return -1;
+        n += res[t].natoms;
+    }
+    return n;
+}
+
+int tpl_a2c_forward(int dtype, const uint8_t* sequences, int seq_stride, const void* angles,
+                    int angles_stride, void* coords, uint8_t* res_names, uint8_t* atom_names,
+                    int atoms_stride, uint32_t* status, int batch, tpl_a2c_stream_t stream) {
+    const int rc = check(dtype, batch, seq_stride, angles_stride, atoms_stride);
+    if (rc != TPL_A2C_OK) return rc;
+    if (batch == 0) return TPL_A2C_OK;
+    if (!sequences || (angles_stride > 0 && !angles) || (atoms_stride > 0 && !coords) ||
+        (!res_names) != (!atom_names))
+        return TPL_A2C_ERR_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    uint32_t* rn = reinterpret_cast<uint32_t*>(res_names);
+    uint32_t* an = reinterpret_cast<uint32_t*>(atom_names);
+    if (dtype == TPL_A2C_F32)
+        hipLaunchKernelGGL(a2c_fwd<float>, dim3(batch), dim3(kThreads), 0, s, sequences, seq_stride,
+                           static_cast<const float*>(angles), angles_stride, static_cast<float*>(coords),
+                           rn, an, atoms_stride, status);
+    else
+        hipLaunchKernelGGL(a2c_fwd<double>, dim3(batch), dim3(kThreads), 0, s, sequences, seq_stride,
+                           static_cast<const double*>(angles), angles_stride,
+                           static_cast<double*>(coords), rn, an, atoms_stride, status);
+    return hipGetLastError() == hipSuccess ? TPL_A2C_OK : TPL_A2C_ERR_LAUNCH;
+}
+
+int tpl_a2c_backward(int dtype, const uint8_t* sequences, int seq_stride, const void* angles,
+                     int angles_stride, const void* grad_coords, int atoms_stride, void* grad_angles,
+                     uint32_t* status, int batch, tpl_a2c_stream_t stream) {
+    const int rc = check(dtype, batch, seq_stride, angles_stride, atoms_stride);
+    if (rc != TPL_A2C_OK) return rc;
+    if (batch == 0 || angles_stride == 0) return TPL_A2C_OK;
+    if (!sequences || !angles || !grad_angles || (atoms_stride > 0 && !grad_coords))
+        return TPL_A2C_ERR_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (dtype == TPL_A2C_F32)
+        hipLaunchKernelGGL(a2c_bwd<float>, dim3(batch), dim3(kThreads), 0, s, sequences, seq_stride,
+                           static_cast<const float*>(angles), angles_stride,
+                           static_cast<const float*>(grad_coords), atoms_stride,
+                           static_cast<float*>(grad_angles), status);
+    else
+        hipLaunchKernelGGL(a2c_bwd<double>, dim3(batch), dim3(kThreads), 0, s, sequences, seq_stride,
+                           static_cast<const double*>(angles), angles_stride,
+                           static_cast<const double*>(grad_coords), atoms_stride,
+                           static_cast<double*>(grad_angles), status);
+    return hipGetLastError() == hipSuccess ? TPL_A2C_OK : TPL_A2C_ERR_LAUNCH;
+}
+
+}  // extern "C"

@@ -51,6 +51,9 @@ def test_residue_template(dump, aa):
     assert [n for n, _ in got] == [n for n, _ in want]
     for (n, v), (_, w) in zip(got, want):
         assert np.abs(v - w).max() < 1e-12, (aa, n, v, w)
+    # bound rows are 3, 4, ... in node order (the kernels take them in turn)
+    bound = [row for row, *_ in sides.get(aa, []) if row >= 0]
+    assert bound == list(range(3, 3 + len(bound)))
     first = 1 + nca
     for k, ((row, alpha, beta, d, f0, cnt), node) in enumerate(zip(sides.get(aa, []), O.SIDE[aa])):
         parent, al, b, dd, group = node

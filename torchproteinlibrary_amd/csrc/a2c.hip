@@ -115,16 +115,49 @@ __device__ __forceinline__ double angle(const T* ang, int row, int stride, int i
     return (double)ang[(size_t)row * stride + i];
 }
 
+// The 8 angles of residue i, loaded together at the top of the residue so
+// that their latencies overlap (one global-load latency per residue walk
+// instead of one per transform).
+template <typename T>
+struct ResAngles {
+    T v[8];
+    __device__ __forceinline__ void load(const T* ang, int stride, int i) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = ang[(size_t)r * stride + i];
+    }
+};
+
+// The side-chain angles xi1..xi5 of a residue in the order its side-chain
+// nodes bind them: every template binds rows 3, 4, ... consecutively (the
+// table test checks this), so each bound node takes the next one.  Shifting
+// keeps the values in registers where an index computed at run time would not.
+template <typename T>
+struct SideRows {
+    T v[5];
+    __device__ __forceinline__ explicit SideRows(const ResAngles<T>& a) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) v[k] = a.v[3 + k];
+    }
+    __device__ __forceinline__ double pop() {
+        const T x = v[0];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = v[k + 1];
+        return (double)x;
+    }
+};
+
 // The residue step S_i applied to m: m <- m T_CA(phi) T_C(psi) T_N(omega).
 template <typename T>
 __device__ __forceinline__ void residue_step(double m[12], const Tables& tb, const T* ang, int stride,
                                              int i) {
+    const double phi = angle(ang, 0, stride, i), psi = angle(ang, 1, stride, i),
+                 omega = angle(ang, 2, stride, i);
     double s, c;
-    sincos(angle(ang, 0, stride, i), &s, &c);
+    sincos(phi, &s, &c);
     step(m, tb.bb[0][0], tb.bb[0][1], c, s, geo::R_N_CA);
-    sincos(angle(ang, 1, stride, i), &s, &c);
+    sincos(psi, &s, &c);
     step(m, tb.bb[1][0], tb.bb[1][1], c, s, geo::R_CA_C);
-    sincos(angle(ang, 2, stride, i), &s, &c);
+    sincos(omega, &s, &c);
     step(m, tb.bb[2][0], tb.bb[2][1], c, s, geo::R_C_N);
 }
 
@@ -255,25 +288,28 @@ __global__ void __launch_bounds__(kThreads) a2c_fwd(const uint8_t* __restrict__ 
             o[1] = (T)x[1];
             o[2] = (T)x[2];
         };
+        ResAngles<T> av;
+        av.load(ang, ang_stride, i);
         put(0, G);
         double s, c, F[12], CA[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) CA[k] = G[k];
-        sincos(angle(ang, 0, ang_stride, i), &s, &c);
+        sincos((double)av.v[0], &s, &c);
         step(CA, tb.bb[0][0], tb.bb[0][1], c, s, geo::R_N_CA);
         for (int k = 1; k <= nca; ++k) put(k, CA);
 #pragma unroll
         for (int k = 0; k < 12; ++k) F[k] = CA[k];
+        SideRows<T> rows(av);
         for (int n = 0; n < nside; ++n) {
             const int q = t * kMaxSide + n;
             const int si = tb.side_i[q];
             const int row = (int)(int8_t)(si & 0xff), first = (si >> 8) & 0xff, count = si >> 16;
             double ca = tb.side[q][2], sa = tb.side[q][3];
-            if (row >= 0) sincos(angle(ang, row, ang_stride, i), &sa, &ca);
+            if (row >= 0) sincos(rows.pop(), &sa, &ca);
             step(F, tb.side[q][0], tb.side[q][1], ca, sa, tb.side[q][4]);
             for (int k = first; k < first + count; ++k) put(k, F);
         }
-        sincos(angle(ang, 1, ang_stride, i), &s, &c);
+        sincos((double)av.v[1], &s, &c);
         step(CA, tb.bb[1][0], tb.bb[1][1], c, s, geo::R_CA_C);  // CA now holds the C frame
         put(natoms - 2, CA);
         put(natoms - 1, CA);
@@ -284,7 +320,7 @@ __global__ void __launch_bounds__(kThreads) a2c_fwd(const uint8_t* __restrict__ 
                 an[a + k] = *reinterpret_cast<const uint32_t*>(kAtom[t][k].name);
             }
         }
-        sincos(angle(ang, 2, ang_stride, i), &s, &c);
+        sincos((double)av.v[2], &s, &c);
         step(CA, tb.bb[2][0], tb.bb[2][1], c, s, geo::R_C_N);
 #pragma unroll
         for (int k = 0; k < 12; ++k) G[k] = CA[k];
@@ -324,7 +360,7 @@ __device__ __forceinline__ double torque(const double m[12], const double g[3], 
 // The residue's atom sums by group: N, the CA group, the side chain and the
 // C group, with the residue's frames (N = G given; CA and C returned).
 template <typename T>
-__device__ __forceinline__ void residue_sums(const Tables& tb, int t, const T* ang, int ang_stride, int i,
+__device__ __forceinline__ void residue_sums(const Tables& tb, int t, const ResAngles<T>& av,
                                              const T* gr, int a, const double G[12], double CA[12],
                                              double C[12], Sums& sN, Sums& sCA, Sums& sSide, Sums& sC) {
     const int ri = tb.res_i[t];
@@ -347,23 +383,24 @@ __device__ __forceinline__ void residue_sums(const Tables& tb, int t, const T* a
     double s, c, F[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) CA[k] = G[k];
-    sincos(angle(ang, 0, ang_stride, i), &s, &c);
+    sincos((double)av.v[0], &s, &c);
     step(CA, tb.bb[0][0], tb.bb[0][1], c, s, geo::R_N_CA);
     for (int k = 1; k <= nca; ++k) take(k, CA, sCA);
 #pragma unroll
     for (int k = 0; k < 12; ++k) F[k] = CA[k];
+    SideRows<T> rows(av);
     for (int n = 0; n < nside; ++n) {
         const int q = t * kMaxSide + n;
         const int si = tb.side_i[q];
         const int row = (int)(int8_t)(si & 0xff), first = (si >> 8) & 0xff, count = si >> 16;
         double ca = tb.side[q][2], sa = tb.side[q][3];
-        if (row >= 0) sincos(angle(ang, row, ang_stride, i), &sa, &ca);
+        if (row >= 0) sincos(rows.pop(), &sa, &ca);
         step(F, tb.side[q][0], tb.side[q][1], ca, sa, tb.side[q][4]);
         for (int k = first; k < first + count; ++k) take(k, F, sSide);
     }
 #pragma unroll
     for (int k = 0; k < 12; ++k) C[k] = CA[k];
-    sincos(angle(ang, 1, ang_stride, i), &s, &c);
+    sincos((double)av.v[1], &s, &c);
     step(C, tb.bb[1][0], tb.bb[1][1], c, s, geo::R_CA_C);
     take(natoms - 2, C, sC);
     take(natoms - 1, C, sC);
@@ -396,14 +433,16 @@ __global__ void __launch_bounds__(kThreads) a2c_bwd(const uint8_t* __restrict__ 
     int a = run.atom0;
     for (int i = run.r0; i < run.r1; ++i) {
         const int t = residue_type(seq[i]);
-        residue_sums(tb, t, ang, ang_stride, i, gr, a, G, CA, C, sN, sCA, sSide, sC);
+        ResAngles<T> av;
+        av.load(ang, ang_stride, i);
+        residue_sums(tb, t, av, gr, a, G, CA, C, sN, sCA, sSide, sC);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             run_sum.g[c] += sN.g[c] + sCA.g[c] + sSide.g[c] + sC.g[c];
             run_sum.xg[c] += sN.xg[c] + sCA.xg[c] + sSide.xg[c] + sC.xg[c];
         }
         double s, c;
-        sincos(angle(ang, 2, ang_stride, i), &s, &c);
+        sincos((double)av.v[2], &s, &c);
         step(C, tb.bb[2][0], tb.bb[2][1], c, s, geo::R_C_N);
 #pragma unroll
         for (int k = 0; k < 12; ++k) G[k] = C[k];
@@ -445,7 +484,9 @@ __global__ void __launch_bounds__(kThreads) a2c_bwd(const uint8_t* __restrict__ 
     a = run.atom0;
     for (int i = run.r0; i < run.r1; ++i) {
         const int t = residue_type(seq[i]);
-        residue_sums(tb, t, ang, ang_stride, i, gr, a, G, CA, C, sN, sCA, sSide, sC);
+        ResAngles<T> av;
+        av.load(ang, ang_stride, i);
+        residue_sums(tb, t, av, gr, a, G, CA, C, sN, sCA, sSide, sC);
         Sums excl, s;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -479,12 +520,13 @@ __global__ void __launch_bounds__(kThreads) a2c_bwd(const uint8_t* __restrict__ 
 #pragma unroll
             for (int k = 0; k < 12; ++k) F[k] = CA[k];
             Sums rest = sSide;
+            SideRows<T> rows(av);
             for (int n = 0; n < nside; ++n) {
                 const int q = t * kMaxSide + n;
                 const int si = tb.side_i[q];
                 const int row = (int)(int8_t)(si & 0xff), first = (si >> 8) & 0xff, count = si >> 16;
                 double ca = tb.side[q][2], sa = tb.side[q][3];
-                if (row >= 0) sincos(angle(ang, row, ang_stride, i), &sa, &ca);
+                if (row >= 0) sincos(rows.pop(), &sa, &ca);
                 step(F, tb.side[q][0], tb.side[q][1], ca, sa, tb.side[q][4]);
                 if (row >= 0) ga[(size_t)row * ang_stride + i] = (T)torque(F, rest.g, rest.xg);
                 Sums mine;
@@ -506,7 +548,7 @@ __global__ void __launch_bounds__(kThreads) a2c_bwd(const uint8_t* __restrict__ 
         }
         incl = excl;
         double sn, cs;
-        sincos(angle(ang, 2, ang_stride, i), &sn, &cs);
+        sincos((double)av.v[2], &sn, &cs);
         step(C, tb.bb[2][0], tb.bb[2][1], cs, sn, geo::R_C_N);
 #pragma unroll
         for (int k = 0; k < 12; ++k) G[k] = C[k];

@@ -28,7 +28,9 @@ second).  The line also carries:
 * ``a2c``: the full-atom builder Angles2Coords (forward + backward) on the
   reference's own benchmark shape (32 random sequences of 350 residues) and
   on a batch of 1024 random sequences with the workload's lengths, with the
-  reference's CPU code timed beside it.
+  reference's CPU code timed beside it;
+* ``pdb``: the PDB reader (host code) on synthetic PDB files, atoms/s, with
+  the Python restatement of the reference's loader timed beside it.
 """
 from __future__ import annotations
 
@@ -392,6 +394,57 @@ def a2c_bench(lengths_mixed, steps, warmup, cpu_seconds, with_cpu):
     return out
 
 
+def pdb_bench(files, atoms_per_file, with_cpu, cpu_seconds):
+    """PDB2CoordsUnordered on synthetic files (ATOM records of random
+    backbone-like coordinates, 10 % hydrogens, TER/END lines), written to a
+    temporary directory first; the files are read once untimed to warm the
+    page cache."""
+    import tempfile
+
+    import numpy as np
+    from torchproteinlibrary_amd.FullAtomModel import PDB2CoordsUnordered
+
+    rng = np.random.default_rng(5)
+    res3 = ["GLY", "ALA", "SER", "LEU", "LYS", "TRP"]
+    names = ["N", "CA", "C", "O", "CB", "H"]
+    with tempfile.TemporaryDirectory() as d:
+        paths, heavy = [], 0
+        for f in range(files):
+            xyz = rng.normal(scale=30.0, size=(atoms_per_file, 3))
+            lines = []
+            for j in range(atoms_per_file):
+                nm = names[j % 6]
+                heavy += nm != "H"
+                lines.append("ATOM  %5d  %-3s %3s %c%4d    %8.3f%8.3f%8.3f  1.00 20.00\n" % (
+                    j % 100000, nm, res3[(j // 6) % 6], "AB"[j // 50000 % 2], (j // 6) % 10000,
+                    xyz[j, 0], xyz[j, 1], xyz[j, 2]))
+            lines.append("TER\nEND\n")
+            p = os.path.join(d, f"s{f}.pdb")
+            with open(p, "w") as fh:
+                fh.write("".join(lines))
+            paths.append(p)
+        p2c = PDB2CoordsUnordered()
+        p2c(paths)
+        t0 = time.perf_counter()
+        out = p2c(paths)
+        dt = time.perf_counter() - t0
+        assert int(out[5].sum()) == heavy
+        res = {"files": files, "records": files * atoms_per_file, "heavy_atoms": heavy, "seconds": dt,
+               "atoms_per_s": heavy / dt, "threads": __import__("torch").get_num_threads()}
+        if with_cpu:
+            from oracle import pdb_oracle
+            done, t_total, k = 0, 0.0, 0
+            while k < len(paths) and t_total < cpu_seconds:
+                t0 = time.perf_counter()
+                done += len(pdb_oracle.read_file(paths[k])[4])
+                t_total += time.perf_counter() - t0
+                k += 1
+            res["cpu_baseline"] = {"value": done / t_total, "unit": "atoms/s", "cores": 1, "kind": "port",
+                                   "sample": f"{k} of the files, the Python restatement of the "
+                                             "reference's loader (its C++ loader does not build here)"}
+    return res
+
+
 def main():
     args = parse()
     if args.fit:
@@ -524,6 +577,9 @@ def main():
     if not args.no_a2c:
         out["a2c"] = a2c_bench(n, min(args.steps, 200), min(args.warmup, 20), args.cpu_seconds / 3,
                                not args.no_cpu_baseline)
+
+    if not args.no_a2c:
+        out["pdb"] = pdb_bench(32, 20000, not args.no_cpu_baseline, 3.0)
 
     if not args.no_plain:
         fr_b = ext.plain_frames_bytes(False, B, L)

@@ -7,10 +7,12 @@
   tensors and calls that C ABI; links libtpl_a2b.so with an $ORIGIN rpath.
 * ``libtpl_rmsd.so`` and ``_RMSD<EXT_SUFFIX>`` -- the same pair for
   Coords2RMSD (include/tpl_rmsd.h).
-* ``libtpl_xform.so``, ``libtpl_a2c.so`` and ``_FullAtomModel<EXT_SUFFIX>``
-  -- the coordinate transforms (include/tpl_xform.h) and the full-atom builder
-  Angles2Coords (include/tpl_a2c.h), one extension over both libraries as in
-  the reference's module of that name.
+* ``libtpl_xform.so``, ``libtpl_a2c.so``, ``libtpl_pdb.so`` and
+  ``_FullAtomModel<EXT_SUFFIX>`` -- the coordinate transforms
+  (include/tpl_xform.h), the full-atom builder Angles2Coords
+  (include/tpl_a2c.h) and the PDB reader (include/tpl_pdb.h, host code built
+  by g++), one extension over the three libraries as in the reference's
+  module of that name.
 
 Run ``python -m torchproteinlibrary_amd._build``.  Every call recompiles.
 """
@@ -37,13 +39,14 @@ RMSD_EXT_NAME = "_RMSD"
 XFORM_LIB_NAME = "libtpl_xform.so"
 XFORM_EXT_NAME = "_FullAtomModel"
 A2C_LIB_NAME = "libtpl_a2c.so"
+PDB_LIB_NAME = "libtpl_pdb.so"
 
 # ([(library, its HIP source)], extension, [its C++ sources])
 PARTS = [
     ([(LIB_NAME, "a2b.hip")], EXT_NAME, ["a2b_ext.cpp"]),
     ([(RMSD_LIB_NAME, "rmsd.hip")], RMSD_EXT_NAME, ["rmsd_ext.cpp"]),
-    ([(XFORM_LIB_NAME, "xform.hip"), (A2C_LIB_NAME, "a2c.hip")], XFORM_EXT_NAME,
-     ["xform_ext.cpp", "a2c_ext.cpp"]),
+    ([(XFORM_LIB_NAME, "xform.hip"), (A2C_LIB_NAME, "a2c.hip"), (PDB_LIB_NAME, "pdb.cpp")],
+     XFORM_EXT_NAME, ["xform_ext.cpp", "a2c_ext.cpp", "pdb_ext.cpp"]),
 ]
 
 
@@ -74,13 +77,17 @@ def _atomic_out(final: str) -> str:
 def build_lib(verbose=True, name: str = LIB_NAME, source: str = "a2b.hip") -> str:
     out = lib_path(name)
     tmp = _atomic_out(out)
-    cmd = [
-        os.path.join(ROCM, "bin", "hipcc"), f"--offload-arch={ARCH}", "-O3", "-std=c++17",
-        "-fPIC", "-shared", "-Wall", "-Werror", "-Wno-unused-command-line-argument",
-        # No packed FP32 instructions anywhere in this library.
-        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-I", INCLUDE, os.path.join(CSRC, source), "-o", tmp,
-    ]
+    if source.endswith(".cpp"):  # host-only library
+        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Werror", "-I", INCLUDE,
+               os.path.join(CSRC, source), "-o", tmp]
+    else:
+        cmd = [
+            os.path.join(ROCM, "bin", "hipcc"), f"--offload-arch={ARCH}", "-O3", "-std=c++17",
+            "-fPIC", "-shared", "-Wall", "-Werror", "-Wno-unused-command-line-argument",
+            # No packed FP32 instructions anywhere in this library.
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-I", INCLUDE, os.path.join(CSRC, source), "-o", tmp,
+        ]
     try:
         _run(cmd, verbose)
         os.replace(tmp, out)

@@ -195,6 +195,8 @@ std::tuple<double, double, double, double> bench(std::vector<std::vector<torch::
 // Angles2Coords entry points (a2c_ext.cpp), in the same module as in the
 // reference (Layers/FullAtomModel/main.cpp:10-14).
 void register_a2c(pybind11::module& m);
+// PDB2CoordsUnordered (pdb_ext.cpp), likewise (main.cpp:21).
+void register_pdb(pybind11::module& m);
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.doc() = "Coordinate transforms for MI355X: tensor glue over the tpl_xform C ABI";
@@ -216,4 +218,5 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
           py::arg("grad_output_center"), py::arg("grad_input_coords"), py::arg("num_atoms"), st);
     m.def("bench", &bench, "average event ms of translate, rotate, centre, centre-backward");
     register_a2c(m);
+    register_pdb(m);
 }

@@ -168,3 +168,31 @@ def test_batch_mixed_lengths_full_size():
         want = O.backward_one(seqs[b], ang[b], g[b].cpu().numpy().astype(np.float64))
         got = X.grad[b].cpu().numpy()
         assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+
+
+def test_large_workgroups_match_small_and_oracle():
+    """A few long structures (batch < 256, stride > 256) run on 512-thread
+    workgroups; the same structures padded to a batch of 256 with empty
+    sequences run on 256-thread workgroups.  Both agree with each other and
+    with the oracle (FP64)."""
+    from torchproteinlibrary_amd.FullAtomModel import Angles2Coords
+    rng = np.random.default_rng(12)
+    lengths = [300, 520, 7, 260]
+    print("a2c 512-thread path: lengths", lengths, flush=True)
+    seqs, ang = _batch(rng, lengths, 520, torch.float64)
+    pad = seqs + [""] * 252
+    ang_pad = np.concatenate([ang, np.zeros((252, 8, 520))])
+    Xa = torch.tensor(ang, device=DEV, requires_grad=True)
+    Xb = torch.tensor(ang_pad, device=DEV, requires_grad=True)
+    ca = Angles2Coords()(Xa, seqs)[0]
+    cb = Angles2Coords()(Xb, pad)[0]
+    assert torch.allclose(ca, cb[:4], rtol=0, atol=1e-10)
+    want, counts = O.forward(seqs, ang)
+    assert np.abs(ca.detach().cpu().numpy() - want).max() < 1e-9
+    g = torch.randn_like(ca)
+    ca.backward(g)
+    cb.backward(torch.cat([g, torch.zeros(252, g.shape[1], dtype=g.dtype, device=DEV)]))
+    ga, gb = Xa.grad, Xb.grad[:4]
+    assert (ga - gb).abs().max().item() <= 1e-10 * gb.abs().max().item()
+    want_g = O.backward_one(seqs[2], ang[2], g[2].cpu().numpy())
+    assert np.abs(ga[2].cpu().numpy() - want_g).max() <= 1e-10 * max(1.0, np.abs(want_g).max())

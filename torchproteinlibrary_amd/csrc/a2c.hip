@@ -32,7 +32,10 @@ namespace {
 
 using namespace a2c;
 
-constexpr int kThreads = 256;
+// Workgroup sizes: 256 threads, or 512 for a few long structures (fewer
+// workgroups than CUs), where a thread then walks half as many residues.  Both
+// fit in LDS (37.9 / 63.3 KB) and in two waves per SIMD of VGPRs.
+constexpr int kThreadsSmall = 256, kThreadsLarge = 512;
 
 __constant__ Residue kRes[kTypes] = A2C_RESIDUES;
 __constant__ Side kSide[kTypes][kMaxSide] = A2C_SIDES;
@@ -49,12 +52,13 @@ struct Tables {
     double bb[3][2];    // cos/sin of beta for the CA, C and N transforms
 };
 
+template <int NT>
 __device__ void fill_tables(Tables& tb) {
-    for (int k = threadIdx.x; k < kTypes * kMaxAtoms; k += kThreads) {
+    for (int k = threadIdx.x; k < kTypes * kMaxAtoms; k += NT) {
         const Atom at = kAtom[k / kMaxAtoms][k % kMaxAtoms];
         local_position(at, tb.loc[k]);
     }
-    for (int k = threadIdx.x; k < kTypes * kMaxSide; k += kThreads) {
+    for (int k = threadIdx.x; k < kTypes * kMaxSide; k += NT) {
         const Side s = kSide[k / kMaxSide][k % kMaxSide];
         sincos(s.beta, &tb.side[k][1], &tb.side[k][0]);
         sincos(s.alpha, &tb.side[k][3], &tb.side[k][2]);
@@ -169,9 +173,9 @@ struct Run {
     int L, r0, r1, atom0, total;
 };
 
-template <typename T>
+template <int NT, typename T>
 __device__ bool prepare(const uint8_t* seq, int seq_stride, const T* ang, int ang_stride,
-                        int atoms_stride, uint32_t* status, const Tables& tb, double (*s_m)[kThreads],
+                        int atoms_stride, uint32_t* status, const Tables& tb, double (*s_m)[NT],
                         int* s_c, int* s_flag, Run& run, double frame[12]) {
     const int tid = threadIdx.x;
     if (tid == 0) {
@@ -179,7 +183,7 @@ __device__ bool prepare(const uint8_t* seq, int seq_stride, const T* ang, int an
         s_flag[1] = 0;           // status bits
     }
     __syncthreads();
-    for (int i = tid; i < seq_stride; i += kThreads)
+    for (int i = tid; i < seq_stride; i += NT)
         if (seq[i] == 0) atomicMin(&s_flag[0], i);
     __syncthreads();
     int L = s_flag[0];
@@ -187,7 +191,7 @@ __device__ bool prepare(const uint8_t* seq, int seq_stride, const T* ang, int an
         if (tid == 0) atomicOr(&s_flag[1], (int)TPL_A2C_STATUS_LENGTH);
         L = 0;
     }
-    const int chunk = (L + kThreads - 1) / kThreads;
+    const int chunk = (L + NT - 1) / NT;
     const int r0 = min(L, tid * chunk), r1 = min(L, r0 + chunk);
     double P[12];
     identity(P);
@@ -207,7 +211,7 @@ __device__ bool prepare(const uint8_t* seq, int seq_stride, const T* ang, int an
 #pragma unroll
     for (int k = 0; k < 12; ++k) s_m[k][tid] = P[k];
     s_c[tid] = cnt;
-    for (int off = 1; off < kThreads; off <<= 1) {
+    for (int off = 1; off < NT; off <<= 1) {
         __syncthreads();
         double Q[12];
         int c2 = 0;
@@ -230,7 +234,7 @@ __device__ bool prepare(const uint8_t* seq, int seq_stride, const T* ang, int an
         }
     }
     __syncthreads();
-    run.total = s_c[kThreads - 1];
+    run.total = s_c[NT - 1];
     if (run.total > atoms_stride && tid == 0) atomicOr(&s_flag[1], (int)TPL_A2C_STATUS_ATOMS);
     if (tid == 0) {
         identity(frame);
@@ -253,25 +257,25 @@ __device__ bool prepare(const uint8_t* seq, int seq_stride, const T* ang, int an
     return true;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kThreads) a2c_fwd(const uint8_t* __restrict__ seqs, int seq_stride,
+template <int NT, typename T>
+__global__ void __launch_bounds__(NT) a2c_fwd(const uint8_t* __restrict__ seqs, int seq_stride,
                                                     const T* __restrict__ angles, int ang_stride,
                                                     T* __restrict__ coords, uint32_t* __restrict__ res_names,
                                                     uint32_t* __restrict__ atom_names, int atoms_stride,
                                                     uint32_t* status) {
     __shared__ Tables tb;
-    __shared__ double s_m[12][kThreads];
-    __shared__ int s_c[kThreads];
+    __shared__ double s_m[12][NT];
+    __shared__ int s_c[NT];
     __shared__ int s_flag[2];
     const int b = blockIdx.x;
     const uint8_t* seq = seqs + (size_t)b * seq_stride;
     const T* ang = angles + (size_t)b * 8 * ang_stride;
     T* out = coords + (size_t)b * 3 * atoms_stride;
-    fill_tables(tb);
+    fill_tables<NT>(tb);
     __syncthreads();
     Run run;
     double G[12];
-    prepare(seq, seq_stride, ang, ang_stride, atoms_stride, status, tb, s_m, s_c, s_flag, run, G);
+    prepare<NT>(seq, seq_stride, ang, ang_stride, atoms_stride, status, tb, s_m, s_c, s_flag, run, G);
     int a = run.atom0;
     uint32_t* rn = res_names ? res_names + (size_t)b * atoms_stride : nullptr;
     uint32_t* an = atom_names ? atom_names + (size_t)b * atoms_stride : nullptr;
@@ -326,9 +330,9 @@ __global__ void __launch_bounds__(kThreads) a2c_fwd(const uint8_t* __restrict__ 
         for (int k = 0; k < 12; ++k) G[k] = CA[k];
         a += natoms;
     }
-    for (int k = 3 * run.total + threadIdx.x; k < 3 * atoms_stride; k += kThreads) out[k] = (T)0;
+    for (int k = 3 * run.total + threadIdx.x; k < 3 * atoms_stride; k += NT) out[k] = (T)0;
     if (rn)
-        for (int k = run.total + threadIdx.x; k < atoms_stride; k += kThreads) rn[k] = an[k] = 0u;
+        for (int k = run.total + threadIdx.x; k < atoms_stride; k += NT) rn[k] = an[k] = 0u;
 }
 
 struct Sums {
@@ -406,25 +410,25 @@ __device__ __forceinline__ void residue_sums(const Tables& tb, int t, const ResA
     take(natoms - 1, C, sC);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kThreads) a2c_bwd(const uint8_t* __restrict__ seqs, int seq_stride,
+template <int NT, typename T>
+__global__ void __launch_bounds__(NT) a2c_bwd(const uint8_t* __restrict__ seqs, int seq_stride,
                                                     const T* __restrict__ angles, int ang_stride,
                                                     const T* __restrict__ grad_coords, int atoms_stride,
                                                     T* __restrict__ grad_angles, uint32_t* status) {
     __shared__ Tables tb;
-    __shared__ double s_m[12][kThreads];
-    __shared__ int s_c[kThreads];
+    __shared__ double s_m[12][NT];
+    __shared__ int s_c[NT];
     __shared__ int s_flag[2];
     const int b = blockIdx.x, tid = threadIdx.x;
     const uint8_t* seq = seqs + (size_t)b * seq_stride;
     const T* ang = angles + (size_t)b * 8 * ang_stride;
     const T* gr = grad_coords + (size_t)b * 3 * atoms_stride;
     T* ga = grad_angles + (size_t)b * 8 * ang_stride;
-    fill_tables(tb);
+    fill_tables<NT>(tb);
     __syncthreads();
     Run run;
     double G0[12], G[12], CA[12], C[12];
-    prepare(seq, seq_stride, ang, ang_stride, atoms_stride, status, tb, s_m, s_c, s_flag, run, G0);
+    prepare<NT>(seq, seq_stride, ang, ang_stride, atoms_stride, status, tb, s_m, s_c, s_flag, run, G0);
     // pass A: this run's (sum g, sum x cross g)
     Sums run_sum, sN, sCA, sSide, sC;
     sums_zero(run_sum);
@@ -458,11 +462,11 @@ __global__ void __launch_bounds__(kThreads) a2c_bwd(const uint8_t* __restrict__ 
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 6; ++k) s_m[k][tid] = v[k];
-    for (int off = 1; off < kThreads; off <<= 1) {
+    for (int off = 1; off < NT; off <<= 1) {
         __syncthreads();
         double w[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) w[k] = tid + off < kThreads ? s_m[k][tid + off] : 0.0;
+        for (int k = 0; k < 6; ++k) w[k] = tid + off < NT ? s_m[k][tid + off] : 0.0;
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
@@ -555,8 +559,14 @@ __global__ void __launch_bounds__(kThreads) a2c_bwd(const uint8_t* __restrict__ 
         a += ri & 0xff;
     }
     // angles past the sequence (and every angle of a flagged structure)
-    for (int k = tid; k < 8 * ang_stride; k += kThreads)
+    for (int k = tid; k < 8 * ang_stride; k += NT)
         if (k % ang_stride >= run.L) ga[k] = (T)0;
+}
+
+// 512-thread workgroups when the structures are fewer than the CUs and some
+// may be longer than 256 residues.
+bool use_large(int batch, int ang_stride) {
+    return batch < 256 && ang_stride > kThreadsSmall;
 }
 
 int check(int dtype, int B, int seq_stride, int ang_stride, int atoms_stride) {
@@ -605,14 +615,26 @@ int tpl_a2c_forward(int dtype, const uint8_t* sequences, int seq_stride, const v
     hipStream_t s = static_cast<hipStream_t>(stream);
     uint32_t* rn = reinterpret_cast<uint32_t*>(res_names);
     uint32_t* an = reinterpret_cast<uint32_t*>(atom_names);
-    if (dtype == TPL_A2C_F32)
-        hipLaunchKernelGGL(a2c_fwd<float>, dim3(batch), dim3(kThreads), 0, s, sequences, seq_stride,
-                           static_cast<const float*>(angles), angles_stride, static_cast<float*>(coords),
-                           rn, an, atoms_stride, status);
-    else
-        hipLaunchKernelGGL(a2c_fwd<double>, dim3(batch), dim3(kThreads), 0, s, sequences, seq_stride,
-                           static_cast<const double*>(angles), angles_stride,
-                           static_cast<double*>(coords), rn, an, atoms_stride, status);
+    const bool large = use_large(batch, angles_stride);
+    if (dtype == TPL_A2C_F32) {
+        const float* a = static_cast<const float*>(angles);
+        float* c = static_cast<float*>(coords);
+        if (large)
+            hipLaunchKernelGGL((a2c_fwd<kThreadsLarge, float>), dim3(batch), dim3(kThreadsLarge), 0, s,
+                               sequences, seq_stride, a, angles_stride, c, rn, an, atoms_stride, status);
+        else
+            hipLaunchKernelGGL((a2c_fwd<kThreadsSmall, float>), dim3(batch), dim3(kThreadsSmall), 0, s,
+                               sequences, seq_stride, a, angles_stride, c, rn, an, atoms_stride, status);
+    } else {
+        const double* a = static_cast<const double*>(angles);
+        double* c = static_cast<double*>(coords);
+        if (large)
+            hipLaunchKernelGGL((a2c_fwd<kThreadsLarge, double>), dim3(batch), dim3(kThreadsLarge), 0, s,
+                               sequences, seq_stride, a, angles_stride, c, rn, an, atoms_stride, status);
+        else
+            hipLaunchKernelGGL((a2c_fwd<kThreadsSmall, double>), dim3(batch), dim3(kThreadsSmall), 0, s,
+                               sequences, seq_stride, a, angles_stride, c, rn, an, atoms_stride, status);
+    }
     return hipGetLastError() == hipSuccess ? TPL_A2C_OK : TPL_A2C_ERR_LAUNCH;
 }
 
@@ -625,16 +647,26 @@ int tpl_a2c_backward(int dtype, const uint8_t* sequences, int seq_stride, const 
     if (!sequences || !angles || !grad_angles || (atoms_stride > 0 && !grad_coords))
         return TPL_A2C_ERR_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (dtype == TPL_A2C_F32)
-        hipLaunchKernelGGL(a2c_bwd<float>, dim3(batch), dim3(kThreads), 0, s, sequences, seq_stride,
-                           static_cast<const float*>(angles), angles_stride,
-                           static_cast<const float*>(grad_coords), atoms_stride,
-                           static_cast<float*>(grad_angles), status);
-    else
-        hipLaunchKernelGGL(a2c_bwd<double>, dim3(batch), dim3(kThreads), 0, s, sequences, seq_stride,
-                           static_cast<const double*>(angles), angles_stride,
-                           static_cast<const double*>(grad_coords), atoms_stride,
-                           static_cast<double*>(grad_angles), status);
+    const bool large = use_large(batch, angles_stride);
+    if (dtype == TPL_A2C_F32) {
+        const float *a = static_cast<const float*>(angles), *g = static_cast<const float*>(grad_coords);
+        float* ga = static_cast<float*>(grad_angles);
+        if (large)
+            hipLaunchKernelGGL((a2c_bwd<kThreadsLarge, float>), dim3(batch), dim3(kThreadsLarge), 0, s,
+                               sequences, seq_stride, a, angles_stride, g, atoms_stride, ga, status);
+        else
+            hipLaunchKernelGGL((a2c_bwd<kThreadsSmall, float>), dim3(batch), dim3(kThreadsSmall), 0, s,
+                               sequences, seq_stride, a, angles_stride, g, atoms_stride, ga, status);
+    } else {
+        const double *a = static_cast<const double*>(angles), *g = static_cast<const double*>(grad_coords);
+        double* ga = static_cast<double*>(grad_angles);
+        if (large)
+            hipLaunchKernelGGL((a2c_bwd<kThreadsLarge, double>), dim3(batch), dim3(kThreadsLarge), 0, s,
+                               sequences, seq_stride, a, angles_stride, g, atoms_stride, ga, status);
+        else
+            hipLaunchKernelGGL((a2c_bwd<kThreadsSmall, double>), dim3(batch), dim3(kThreadsSmall), 0, s,
+                               sequences, seq_stride, a, angles_stride, g, atoms_stride, ga, status);
+    }
     return hipGetLastError() == hipSuccess ? TPL_A2C_OK : TPL_A2C_ERR_LAUNCH;
 }
 

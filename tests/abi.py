@@ -1,5 +1,5 @@
 """ctypes binding of libtpl_a2b.so, declared from include/tpl_a2b.h.  This is the
-binding a ctypes user of the C ABI writes (see INTEGRATION.md)."""
+binding a ctypes user of the C ABI writes."""
 import ctypes
 import os
 import re

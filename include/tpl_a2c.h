@@ -22,6 +22,8 @@
  *   atom_names                                  ("ALA", "CB"), 0 past the structure
  *   grad_angles [batch][8][angles_stride]   every entry written: 0 for angles
  *                                           the structure does not use
+ *                                           (FP32 coordinates give gradients
+ *                                           within about 1e-5 of FP64 ones)
  *
  * dtype is TPL_A2C_F32 or TPL_A2C_F64 for angles, coords and gradients; the
  * frames and sums are FP64 inside.  Streams, status words, return codes: as in
@@ -68,12 +70,16 @@ int tpl_a2c_forward(int dtype, const uint8_t *sequences, int seq_stride, const v
                     int angles_stride, void *coords, uint8_t *res_names, uint8_t *atom_names,
                     int atoms_stride, uint32_t *status, int batch, tpl_a2c_stream_t stream);
 
-/* grad_angles = d(sum grad_coords . coords) / d angles.  Replaces
- * Angles2Coords_backward (angles2coords_interface.cpp:71-111,
- * cConformation.cpp:160-195: O(atoms x depth) there, O(atoms) here). */
-int tpl_a2c_backward(int dtype, const uint8_t *sequences, int seq_stride, const void *angles,
-                     int angles_stride, const void *grad_coords, int atoms_stride,
-                     void *grad_angles, uint32_t *status, int batch, tpl_a2c_stream_t stream);
+/* grad_angles = d(sum grad_coords . coords) / d angles, from the forward's
+ * coordinates (coords, as tpl_a2c_forward wrote them for the same sequences
+ * and angles): each bound angle turns its subtree about a bond whose
+ * direction and origin are atoms of coords, so no angles are needed.
+ * Replaces Angles2Coords_backward (angles2coords_interface.cpp:71-111, which
+ * rebuilds the conformation from the angles; cConformation.cpp:160-195:
+ * O(atoms x depth) there, O(atoms) here). */
+int tpl_a2c_backward(int dtype, const uint8_t *sequences, int seq_stride, const void *coords,
+                     const void *grad_coords, int atoms_stride, void *grad_angles, int angles_stride,
+                     uint32_t *status, int batch, tpl_a2c_stream_t stream);
 
 #ifdef __cplusplus
 }

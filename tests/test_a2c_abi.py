@@ -23,8 +23,8 @@ def test_exports_counts_and_argument_errors():
     assert lib.tpl_a2c_forward(0, None, 4, p, 4, p, None, None, 10, None, 1, None) == 1
     assert lib.tpl_a2c_forward(0, p, 4, p, 4, p, p, None, 10, None, 1, None) == 1  # one names pointer
     assert lib.tpl_a2c_forward(0, p, 4, p, 2 ** 29, p, None, None, 10, None, 1, None) == 5
-    assert lib.tpl_a2c_backward(1, p, 4, None, 4, p, 10, p, None, 1, None) == 1
-    assert lib.tpl_a2c_backward(0, p, -1, p, 4, p, 10, p, None, 1, None) == 1
+    assert lib.tpl_a2c_backward(1, p, 4, None, p, 10, p, 4, None, 1, None) == 1  # no coordinates
+    assert lib.tpl_a2c_backward(0, p, -1, p, p, 10, p, 4, None, 1, None) == 1
     # empty problems are no-ops
     assert lib.tpl_a2c_forward(0, None, 0, None, 0, None, None, None, 0, None, 0, None) == 0
-    assert lib.tpl_a2c_backward(1, p, 4, p, 0, p, 10, p, None, 3, None) == 0
+    assert lib.tpl_a2c_backward(1, p, 4, p, p, 10, p, 0, None, 3, None) == 0

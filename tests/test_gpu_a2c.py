@@ -116,7 +116,7 @@ def test_c_abi_direct_matches_layer():
                                st.data_ptr(), 3, None) == 0
     g = torch.randn_like(coords)
     ga = torch.full_like(X, 7.0)
-    assert lib.tpl_a2c_backward(0, S.data_ptr(), S.shape[1], X.data_ptr(), 30, g.data_ptr(), A, ga.data_ptr(),
+    assert lib.tpl_a2c_backward(0, S.data_ptr(), S.shape[1], out.data_ptr(), g.data_ptr(), A, ga.data_ptr(), 30,
                                 st.data_ptr(), 3, None) == 0
     torch.cuda.synchronize()
     assert int(st.item()) == 0
@@ -196,3 +196,20 @@ def test_large_workgroups_match_small_and_oracle():
     assert (ga - gb).abs().max().item() <= 1e-10 * gb.abs().max().item()
     want_g = O.backward_one(seqs[2], ang[2], g[2].cpu().numpy())
     assert np.abs(ga[2].cpu().numpy() - want_g).max() <= 1e-10 * max(1.0, np.abs(want_g).max())
+
+
+def test_reference_signature_backward():
+    """Angles2Coords_backward with the reference's arguments (angles, not
+    coordinates) equals the layer's backward."""
+    from torchproteinlibrary_amd._native_xform import ext
+    from torchproteinlibrary_amd.FullAtomModel import Angles2Coords
+    from torchproteinlibrary_amd.FullAtomModel.Angles2Coords import convertStringList
+    rng = np.random.default_rng(6)
+    seqs, ang = _batch(rng, [9, 40, 1], 40, torch.float64)
+    X = torch.tensor(ang, device=DEV, requires_grad=True)
+    coords = Angles2Coords()(X, seqs)[0]
+    g = torch.randn_like(coords)
+    coords.backward(g)
+    ga = torch.empty_like(X)
+    ext.Angles2Coords_backward(g, ga, convertStringList(seqs).to(DEV), X.detach())
+    assert torch.allclose(ga, X.grad, rtol=0, atol=1e-12)

@@ -18,10 +18,11 @@
 // The backward uses dL/dalpha = u . (sum_{x in subtree} x cross g
 // - p cross sum_{x in subtree} g) for a node with x axis u and origin p (the
 // reference's g . (F x) with F = M_parent dT M^-1, cConformation.cpp:148,
-// 171-184, reduces to this since F x = u cross (x - p)).  The subtree of a
-// backbone node is the rest of its residue plus every later residue, so a
-// second workgroup scan, a suffix sum of per-run (sum g, sum x cross g),
-// supplies the downstream part, and each thread walks its residues again.
+// 171-184, reduces to this since F x = u cross (x - p)).  u and p follow from
+// the forward's coordinates, so the backward reads those and needs no frames.
+// The subtree of a backbone node is the rest of its residue plus every later
+// residue: a workgroup suffix scan of per-run (sum g, sum x cross g) supplies
+// the downstream part, and each thread walks its residues once more.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -352,211 +353,227 @@ __device__ __forceinline__ void sums_add(Sums& s, const double x[3], const doubl
     s.xg[2] += x[0] * g[1] - x[1] * g[0];
 }
 
-// u . (xg - p cross g) for the frame m (u = its x axis, p = its origin).
-__device__ __forceinline__ double torque(const double m[12], const double g[3], const double xg[3]) {
-    const double p0 = m[3], p1 = m[7], p2 = m[11];
-    const double v0 = xg[0] - (p1 * g[2] - p2 * g[1]);
-    const double v1 = xg[1] - (p2 * g[0] - p0 * g[2]);
-    const double v2 = xg[2] - (p0 * g[1] - p1 * g[0]);
-    return m[0] * v0 + m[4] * v1 + m[8] * v2;
+// Backward from the forward's coordinates.  A node bound to an angle turns
+// its subtree about its x axis through its origin; the origin is the node's
+// first atom (every group's first atom sits at the group origin), and the x
+// axis is the unit bond vector from the parent node's origin (the transform
+// Ry Tx(d) Rx moves the origin by d along Ry's x axis, which Rx keeps), so
+// the gradient needs no frames: dL/dalpha = u . (sum x cross g - p cross
+// sum g) over the subtree, with u = (p - q) / |p - q| for parent origin q.
+
+// Per structure, without frames: length, validity, this thread's run and its
+// first atom (workgroup scan of the atom counts).  Flags as in prepare().
+template <int NT>
+__device__ void count_run(const uint8_t* seq, int seq_stride, int ang_stride, int atoms_stride,
+                          uint32_t* status, const int* res_i, int* s_c, int* s_flag, Run& run) {
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        s_flag[0] = seq_stride;
+        s_flag[1] = 0;
+    }
+    __syncthreads();
+    for (int i = tid; i < seq_stride; i += NT)
+        if (seq[i] == 0) atomicMin(&s_flag[0], i);
+    __syncthreads();
+    int L = s_flag[0];
+    if (L > ang_stride) {
+        if (tid == 0) atomicOr(&s_flag[1], (int)TPL_A2C_STATUS_LENGTH);
+        L = 0;
+    }
+    const int chunk = (L + NT - 1) / NT;
+    const int r0 = min(L, tid * chunk), r1 = min(L, r0 + chunk);
+    int cnt = 0;
+    bool bad = false;
+    for (int i = r0; i < r1; ++i) {
+        const int t = residue_type(seq[i]);
+        if (t < 0)
+            bad = true;
+        else
+            cnt += res_i[t] & 0xff;
+    }
+    if (bad) atomicOr(&s_flag[1], (int)TPL_A2C_STATUS_SEQ);
+    s_c[tid] = cnt;
+    for (int off = 1; off < NT; off <<= 1) {
+        __syncthreads();
+        const int c2 = tid >= off ? s_c[tid - off] : 0;
+        __syncthreads();
+        cnt += c2;
+        s_c[tid] = cnt;
+    }
+    __syncthreads();
+    run.total = s_c[NT - 1];
+    run.atom0 = tid ? s_c[tid - 1] : 0;
+    if (run.total > atoms_stride && tid == 0) atomicOr(&s_flag[1], (int)TPL_A2C_STATUS_ATOMS);
+    __syncthreads();
+    const int flags = s_flag[1];
+    if (flags && tid == 0 && status)
+        __hip_atomic_fetch_or(status, (uint32_t)flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    run.L = L;
+    run.r0 = r0;
+    run.r1 = r1;
+    if (flags) run.L = run.r0 = run.r1 = run.total = 0;
 }
 
-// The residue's atom sums by group: N, the CA group, the side chain and the
-// C group, with the residue's frames (N = G given; CA and C returned).
 template <typename T>
-__device__ __forceinline__ void residue_sums(const Tables& tb, int t, const ResAngles<T>& av,
-                                             const T* gr, int a, const double G[12], double CA[12],
-                                             double C[12], Sums& sN, Sums& sCA, Sums& sSide, Sums& sC) {
-    const int ri = tb.res_i[t];
-    const int natoms = ri & 0xff, nca = (ri >> 8) & 0xff, nside = ri >> 16;
-    const double* loc = tb.loc[t * kMaxAtoms];
-    double x[3], g[3];
-    auto take = [&](int slot, const double m[12], Sums& acc) {
-        apply(m, loc + 3 * slot, x);
-        const T* p = gr + 3 * (size_t)(a + slot);
-        g[0] = (double)p[0];
-        g[1] = (double)p[1];
-        g[2] = (double)p[2];
+__device__ __forceinline__ void load3(const T* p, double v[3]) {
+    v[0] = (double)p[0];
+    v[1] = (double)p[1];
+    v[2] = (double)p[2];
+}
+
+// Sums over atoms [k0, k1) of a structure.  (Unrolling by four atoms with
+// the loads issued first measured slower: 153 -> 204 us for the mixed1000
+// backward with it in pass A only, 281 us with it everywhere.)
+template <typename T>
+__device__ __forceinline__ void atom_sums(const T* xs, const T* gs, int k0, int k1, Sums& acc) {
+    for (int k = k0; k < k1; ++k) {
+        double x[3], g[3];
+        load3(xs + 3 * (size_t)k, x);
+        load3(gs + 3 * (size_t)k, g);
         sums_add(acc, x, g);
-    };
-    sums_zero(sN);
-    sums_zero(sCA);
-    sums_zero(sSide);
-    sums_zero(sC);
-    take(0, G, sN);
-    double s, c, F[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) CA[k] = G[k];
-    sincos((double)av.v[0], &s, &c);
-    step(CA, tb.bb[0][0], tb.bb[0][1], c, s, geo::R_N_CA);
-    for (int k = 1; k <= nca; ++k) take(k, CA, sCA);
-#pragma unroll
-    for (int k = 0; k < 12; ++k) F[k] = CA[k];
-    SideRows<T> rows(av);
-    for (int n = 0; n < nside; ++n) {
-        const int q = t * kMaxSide + n;
-        const int si = tb.side_i[q];
-        const int row = (int)(int8_t)(si & 0xff), first = (si >> 8) & 0xff, count = si >> 16;
-        double ca = tb.side[q][2], sa = tb.side[q][3];
-        if (row >= 0) sincos(rows.pop(), &sa, &ca);
-        step(F, tb.side[q][0], tb.side[q][1], ca, sa, tb.side[q][4]);
-        for (int k = first; k < first + count; ++k) take(k, F, sSide);
     }
-#pragma unroll
-    for (int k = 0; k < 12; ++k) C[k] = CA[k];
-    sincos((double)av.v[1], &s, &c);
-    step(C, tb.bb[1][0], tb.bb[1][1], c, s, geo::R_CA_C);
-    take(natoms - 2, C, sC);
-    take(natoms - 1, C, sC);
+}
+
+// u . (xg - p cross g) with u the unit vector from q to p.
+__device__ __forceinline__ double bond_torque(const double p[3], const double q[3], const Sums& s) {
+    const double d0 = p[0] - q[0], d1 = p[1] - q[1], d2 = p[2] - q[2];
+    const double inv = 1.0 / sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    const double v0 = s.xg[0] - (p[1] * s.g[2] - p[2] * s.g[1]);
+    const double v1 = s.xg[1] - (p[2] * s.g[0] - p[0] * s.g[2]);
+    const double v2 = s.xg[2] - (p[0] * s.g[1] - p[1] * s.g[0]);
+    return (d0 * v0 + d1 * v1 + d2 * v2) * inv;
 }
 
 template <int NT, typename T>
 __global__ void __launch_bounds__(NT) a2c_bwd(const uint8_t* __restrict__ seqs, int seq_stride,
-                                                    const T* __restrict__ angles, int ang_stride,
-                                                    const T* __restrict__ grad_coords, int atoms_stride,
-                                                    T* __restrict__ grad_angles, uint32_t* status) {
-    __shared__ Tables tb;
-    __shared__ double s_m[12][NT];
+                                              const T* __restrict__ coords, const T* __restrict__ grad_coords,
+                                              int atoms_stride, T* __restrict__ grad_angles, int ang_stride,
+                                              uint32_t* status) {
+    __shared__ int s_res[kTypes];
+    __shared__ int s_side[kTypes * kMaxSide];
+    __shared__ double s_v[6][NT];
     __shared__ int s_c[NT];
     __shared__ int s_flag[2];
     const int b = blockIdx.x, tid = threadIdx.x;
     const uint8_t* seq = seqs + (size_t)b * seq_stride;
-    const T* ang = angles + (size_t)b * 8 * ang_stride;
-    const T* gr = grad_coords + (size_t)b * 3 * atoms_stride;
+    const T* xs = coords + (size_t)b * 3 * atoms_stride;
+    const T* gs = grad_coords + (size_t)b * 3 * atoms_stride;
     T* ga = grad_angles + (size_t)b * 8 * ang_stride;
-    fill_tables<NT>(tb);
+    for (int k = tid; k < kTypes * kMaxSide; k += NT) {
+        const Side sd = kSide[k / kMaxSide][k % kMaxSide];
+        s_side[k] = (sd.row & 0xff) | (sd.first << 8) | (sd.count << 16);
+    }
+    if (tid < kTypes) {
+        const Residue r = kRes[tid];
+        s_res[tid] = r.natoms | (r.nca << 8) | (r.nside << 16);
+    }
     __syncthreads();
     Run run;
-    double G0[12], G[12], CA[12], C[12];
-    prepare<NT>(seq, seq_stride, ang, ang_stride, atoms_stride, status, tb, s_m, s_c, s_flag, run, G0);
+    count_run<NT>(seq, seq_stride, ang_stride, atoms_stride, status, s_res, s_c, s_flag, run);
     // pass A: this run's (sum g, sum x cross g)
-    Sums run_sum, sN, sCA, sSide, sC;
+    Sums run_sum;
     sums_zero(run_sum);
-#pragma unroll
-    for (int k = 0; k < 12; ++k) G[k] = G0[k];
-    int a = run.atom0;
-    for (int i = run.r0; i < run.r1; ++i) {
-        const int t = residue_type(seq[i]);
-        ResAngles<T> av;
-        av.load(ang, ang_stride, i);
-        residue_sums(tb, t, av, gr, a, G, CA, C, sN, sCA, sSide, sC);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            run_sum.g[c] += sN.g[c] + sCA.g[c] + sSide.g[c] + sC.g[c];
-            run_sum.xg[c] += sN.xg[c] + sCA.xg[c] + sSide.xg[c] + sC.xg[c];
-        }
-        double s, c;
-        sincos((double)av.v[2], &s, &c);
-        step(C, tb.bb[2][0], tb.bb[2][1], c, s, geo::R_C_N);
-#pragma unroll
-        for (int k = 0; k < 12; ++k) G[k] = C[k];
-        a += tb.res_i[t] & 0xff;
+    {
+        int a = run.atom0;
+        for (int i = run.r0; i < run.r1; ++i) a += s_res[residue_type(seq[i])] & 0xff;
+        atom_sums(xs, gs, run.atom0, a, run_sum);
     }
-    // suffix scan: down = sums over the runs after this one (reusing s_m)
+    // suffix scan over runs: incl = sums over every atom from this run on
     double v[6];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         v[c] = run_sum.g[c];
         v[3 + c] = run_sum.xg[c];
     }
-    __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 6; ++k) s_m[k][tid] = v[k];
+    for (int k = 0; k < 6; ++k) s_v[k][tid] = v[k];
     for (int off = 1; off < NT; off <<= 1) {
         __syncthreads();
         double w[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) w[k] = tid + off < NT ? s_m[k][tid + off] : 0.0;
+        for (int k = 0; k < 6; ++k) w[k] = tid + off < NT ? s_v[k][tid + off] : 0.0;
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             v[k] += w[k];
-            s_m[k][tid] = v[k];
+            s_v[k][tid] = v[k];
         }
     }
-    __syncthreads();
-    // incl: sums over every atom from this run's first residue on
     Sums incl;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        incl.g[c] = s_m[c][tid];
-        incl.xg[c] = s_m[3 + c][tid];
+        incl.g[c] = v[c];
+        incl.xg[c] = v[3 + c];
     }
-    // pass B: gradients
-#pragma unroll
-    for (int k = 0; k < 12; ++k) G[k] = G0[k];
-    a = run.atom0;
+    // pass B: gradients, residue by residue
+    int a = run.atom0;
     for (int i = run.r0; i < run.r1; ++i) {
         const int t = residue_type(seq[i]);
-        ResAngles<T> av;
-        av.load(ang, ang_stride, i);
-        residue_sums(tb, t, av, gr, a, G, CA, C, sN, sCA, sSide, sC);
-        Sums excl, s;
+        const int ri = s_res[t];
+        const int natoms = ri & 0xff, nca = (ri >> 8) & 0xff, nside = ri >> 16;
+        Sums sN, sCA, sSide, sC, excl, s;
+        sums_zero(sN);
+        sums_zero(sCA);
+        sums_zero(sSide);
+        sums_zero(sC);
+        atom_sums(xs, gs, a, a + 1, sN);
+        atom_sums(xs, gs, a + 1, a + 1 + nca, sCA);
+        atom_sums(xs, gs, a + 1 + nca, a + natoms - 2, sSide);
+        atom_sums(xs, gs, a + natoms - 2, a + natoms, sC);
+        double pN[3], pCA[3], pC[3];
+        load3(xs + 3 * (size_t)a, pN);
+        load3(xs + 3 * (size_t)(a + 1), pCA);
+        load3(xs + 3 * (size_t)(a + natoms - 2), pC);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             excl.g[c] = incl.g[c] - (sN.g[c] + sCA.g[c] + sSide.g[c] + sC.g[c]);
             excl.xg[c] = incl.xg[c] - (sN.xg[c] + sCA.xg[c] + sSide.xg[c] + sC.xg[c]);
         }
         for (int r = 3; r < 8; ++r) ga[(size_t)r * ang_stride + i] = (T)0;
-        // omega of residue i-1 binds N_i: the whole residue and everything after
-        if (i > 0) ga[(size_t)2 * ang_stride + i - 1] = (T)torque(G, incl.g, incl.xg);
+        // omega of residue i-1 binds N_i (on the previous C): the whole residue and everything after
+        if (i > 0) {
+            double pCp[3];
+            load3(xs + 3 * (size_t)(a - 2), pCp);
+            ga[(size_t)2 * ang_stride + i - 1] = (T)bond_torque(pN, pCp, incl);
+        }
         if (i == run.L - 1) ga[(size_t)2 * ang_stride + i] = (T)0;
-        // phi binds CA: all but N
+        // phi binds CA (on N): all but N
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             s.g[c] = incl.g[c] - sN.g[c];
             s.xg[c] = incl.xg[c] - sN.xg[c];
         }
-        ga[i] = (T)torque(CA, s.g, s.xg);
-        // psi binds C: the C group and everything after
+        ga[i] = (T)bond_torque(pCA, pN, s);
+        // psi binds C (on CA): the C group and everything after
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             s.g[c] = sC.g[c] + excl.g[c];
             s.xg[c] = sC.xg[c] + excl.xg[c];
         }
-        ga[(size_t)ang_stride + i] = (T)torque(C, s.g, s.xg);
-        // side chain: node n binds its group and the later nodes' groups
-        const int ri = tb.res_i[t];
-        const int nside = ri >> 16;
-        if (nside) {
-            const double* loc = tb.loc[t * kMaxAtoms];
-            double F[12], x[3], g[3];
+        ga[(size_t)ang_stride + i] = (T)bond_torque(pC, pCA, s);
+        // side chain: node n binds its group and the later nodes' groups; a
+        // node without atoms (the fixed first node) sits at its parent's origin
+        double q[3] = {pCA[0], pCA[1], pCA[2]};
+        Sums rest = sSide;
+        for (int n = 0; n < nside; ++n) {
+            const int si = s_side[t * kMaxSide + n];
+            const int row = (int)(int8_t)(si & 0xff), first = (si >> 8) & 0xff, count = si >> 16;
+            if (count == 0) continue;
+            double p[3];
+            load3(xs + 3 * (size_t)(a + first), p);
+            if (row >= 0) ga[(size_t)row * ang_stride + i] = (T)bond_torque(p, q, rest);
+            Sums mine;
+            sums_zero(mine);
+            atom_sums(xs, gs, a + first, a + first + count, mine);
 #pragma unroll
-            for (int k = 0; k < 12; ++k) F[k] = CA[k];
-            Sums rest = sSide;
-            SideRows<T> rows(av);
-            for (int n = 0; n < nside; ++n) {
-                const int q = t * kMaxSide + n;
-                const int si = tb.side_i[q];
-                const int row = (int)(int8_t)(si & 0xff), first = (si >> 8) & 0xff, count = si >> 16;
-                double ca = tb.side[q][2], sa = tb.side[q][3];
-                if (row >= 0) sincos(rows.pop(), &sa, &ca);
-                step(F, tb.side[q][0], tb.side[q][1], ca, sa, tb.side[q][4]);
-                if (row >= 0) ga[(size_t)row * ang_stride + i] = (T)torque(F, rest.g, rest.xg);
-                Sums mine;
-                sums_zero(mine);
-                for (int k = first; k < first + count; ++k) {
-                    apply(F, loc + 3 * k, x);
-                    const T* p = gr + 3 * (size_t)(a + k);
-                    g[0] = (double)p[0];
-                    g[1] = (double)p[1];
-                    g[2] = (double)p[2];
-                    sums_add(mine, x, g);
-                }
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    rest.g[c] -= mine.g[c];
-                    rest.xg[c] -= mine.xg[c];
-                }
+            for (int c = 0; c < 3; ++c) {
+                rest.g[c] -= mine.g[c];
+                rest.xg[c] -= mine.xg[c];
+                q[c] = p[c];
             }
         }
         incl = excl;
-        double sn, cs;
-        sincos((double)av.v[2], &sn, &cs);
-        step(C, tb.bb[2][0], tb.bb[2][1], cs, sn, geo::R_C_N);
-#pragma unroll
-        for (int k = 0; k < 12; ++k) G[k] = C[k];
-        a += ri & 0xff;
+        a += natoms;
     }
     // angles past the sequence (and every angle of a flagged structure)
     for (int k = tid; k < 8 * ang_stride; k += NT)
@@ -638,34 +655,34 @@ int tpl_a2c_forward(int dtype, const uint8_t* sequences, int seq_stride, const v
     return hipGetLastError() == hipSuccess ? TPL_A2C_OK : TPL_A2C_ERR_LAUNCH;
 }
 
-int tpl_a2c_backward(int dtype, const uint8_t* sequences, int seq_stride, const void* angles,
-                     int angles_stride, const void* grad_coords, int atoms_stride, void* grad_angles,
+int tpl_a2c_backward(int dtype, const uint8_t* sequences, int seq_stride, const void* coords,
+                     const void* grad_coords, int atoms_stride, void* grad_angles, int angles_stride,
                      uint32_t* status, int batch, tpl_a2c_stream_t stream) {
     const int rc = check(dtype, batch, seq_stride, angles_stride, atoms_stride);
     if (rc != TPL_A2C_OK) return rc;
     if (batch == 0 || angles_stride == 0) return TPL_A2C_OK;
-    if (!sequences || !angles || !grad_angles || (atoms_stride > 0 && !grad_coords))
+    if (!sequences || !grad_angles || (atoms_stride > 0 && (!coords || !grad_coords)))
         return TPL_A2C_ERR_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const bool large = use_large(batch, angles_stride);
     if (dtype == TPL_A2C_F32) {
-        const float *a = static_cast<const float*>(angles), *g = static_cast<const float*>(grad_coords);
+        const float *x = static_cast<const float*>(coords), *g = static_cast<const float*>(grad_coords);
         float* ga = static_cast<float*>(grad_angles);
         if (large)
             hipLaunchKernelGGL((a2c_bwd<kThreadsLarge, float>), dim3(batch), dim3(kThreadsLarge), 0, s,
-                               sequences, seq_stride, a, angles_stride, g, atoms_stride, ga, status);
+                               sequences, seq_stride, x, g, atoms_stride, ga, angles_stride, status);
         else
             hipLaunchKernelGGL((a2c_bwd<kThreadsSmall, float>), dim3(batch), dim3(kThreadsSmall), 0, s,
-                               sequences, seq_stride, a, angles_stride, g, atoms_stride, ga, status);
+                               sequences, seq_stride, x, g, atoms_stride, ga, angles_stride, status);
     } else {
-        const double *a = static_cast<const double*>(angles), *g = static_cast<const double*>(grad_coords);
+        const double *x = static_cast<const double*>(coords), *g = static_cast<const double*>(grad_coords);
         double* ga = static_cast<double*>(grad_angles);
         if (large)
             hipLaunchKernelGGL((a2c_bwd<kThreadsLarge, double>), dim3(batch), dim3(kThreadsLarge), 0, s,
-                               sequences, seq_stride, a, angles_stride, g, atoms_stride, ga, status);
+                               sequences, seq_stride, x, g, atoms_stride, ga, angles_stride, status);
         else
             hipLaunchKernelGGL((a2c_bwd<kThreadsSmall, double>), dim3(batch), dim3(kThreadsSmall), 0, s,
-                               sequences, seq_stride, a, angles_stride, g, atoms_stride, ga, status);
+                               sequences, seq_stride, x, g, atoms_stride, ga, angles_stride, status);
     }
     return hipGetLastError() == hipSuccess ? TPL_A2C_OK : TPL_A2C_ERR_LAUNCH;
 }

@@ -81,28 +81,50 @@ void forward(torch::Tensor sequences, torch::Tensor input_angles, torch::Tensor 
              "Angles2Coords_forward");
 }
 
+// From the forward's coordinates (what the Python layer saves).
+void backward_coords(torch::Tensor grad_atoms, torch::Tensor grad_angles, torch::Tensor sequences,
+                     torch::Tensor coords, c10::optional<torch::Tensor> status) {
+    check_common(sequences, grad_angles);
+    A2C_CHECK_GPU(grad_atoms);
+    A2C_CHECK_GPU(coords);
+    const int64_t B = grad_angles.size(0);
+    TORCH_CHECK(grad_atoms.scalar_type() == grad_angles.scalar_type() &&
+                    coords.scalar_type() == grad_angles.scalar_type(),
+                "gradients and coordinates must have one dtype");
+    TORCH_CHECK(grad_atoms.dim() == 2 && grad_atoms.size(0) == B && grad_atoms.size(1) % 3 == 0,
+                "grad_atoms must be [batch, 3*atoms]");
+    TORCH_CHECK(coords.sizes() == grad_atoms.sizes(), "coords must have the shape of grad_atoms");
+    TORCH_CHECK(grad_atoms.device() == grad_angles.device() && coords.device() == grad_angles.device(),
+                "gradients and coordinates must be on one device");
+    const int64_t A = grad_atoms.size(1) / 3;
+    TORCH_CHECK(A <= INT32_MAX / 3, "grad_atoms too large");
+    c10::DeviceGuard guard(grad_angles.device());
+    check_rc(tpl_a2c_backward(dtype_code(grad_angles), sequences.data_ptr<uint8_t>(), (int)sequences.size(1),
+                              coords.data_ptr(), grad_atoms.data_ptr(), (int)A, grad_angles.data_ptr(),
+                              (int)grad_angles.size(2), status_ptr(status, grad_angles), (int)B,
+                              stream_of(grad_angles)),
+             "Angles2Coords_backward");
+}
+
+// The reference's signature (angles, not coordinates): the coordinates are
+// computed again first, as the reference rebuilds its conformation.
 void backward(torch::Tensor grad_atoms, torch::Tensor grad_angles, torch::Tensor sequences,
               torch::Tensor input_angles, c10::optional<torch::Tensor> status) {
     check_common(sequences, input_angles);
     A2C_CHECK_GPU(grad_atoms);
-    A2C_CHECK_GPU(grad_angles);
-    const int64_t B = input_angles.size(0);
-    TORCH_CHECK(grad_atoms.scalar_type() == input_angles.scalar_type() &&
-                    grad_angles.scalar_type() == input_angles.scalar_type(),
-                "gradients must have the dtype of the angles");
-    TORCH_CHECK(grad_atoms.dim() == 2 && grad_atoms.size(0) == B && grad_atoms.size(1) % 3 == 0,
-                "grad_atoms must be [batch, 3*atoms]");
     TORCH_CHECK(grad_angles.sizes() == input_angles.sizes(), "grad_angles must have the shape of the angles");
-    TORCH_CHECK(grad_atoms.device() == input_angles.device() && grad_angles.device() == input_angles.device(),
-                "gradients must be on the device of the angles");
+    TORCH_CHECK(grad_atoms.dim() == 2 && grad_atoms.size(0) == input_angles.size(0) && grad_atoms.size(1) % 3 == 0,
+                "grad_atoms must be [batch, 3*atoms]");
     const int64_t A = grad_atoms.size(1) / 3;
     TORCH_CHECK(A <= INT32_MAX / 3, "grad_atoms too large");
     c10::DeviceGuard guard(input_angles.device());
-    check_rc(tpl_a2c_backward(dtype_code(input_angles), sequences.data_ptr<uint8_t>(), (int)sequences.size(1),
-                              input_angles.data_ptr(), (int)input_angles.size(2), grad_atoms.data_ptr(), (int)A,
-                              grad_angles.data_ptr(), status_ptr(status, input_angles), (int)B,
-                              stream_of(input_angles)),
+    auto coords = torch::empty_like(grad_atoms);
+    check_rc(tpl_a2c_forward(dtype_code(input_angles), sequences.data_ptr<uint8_t>(), (int)sequences.size(1),
+                             input_angles.data_ptr(), (int)input_angles.size(2), coords.data_ptr(), nullptr,
+                             nullptr, (int)A, status_ptr(status, input_angles), (int)input_angles.size(0),
+                             stream_of(input_angles)),
              "Angles2Coords_backward");
+    backward_coords(grad_atoms, grad_angles, sequences, coords, status);
 }
 
 int num_atoms(const std::string& sequence) {
@@ -137,8 +159,8 @@ std::tuple<double, double, double> bench(std::vector<std::vector<torch::Tensor>>
                  "forward");
     };
     auto bwd = [&](std::vector<torch::Tensor>& t) {
-        check_rc(tpl_a2c_backward(dt, t[0].data_ptr<uint8_t>(), (int)t[0].size(1), t[1].data_ptr(),
-                                  (int)t[1].size(2), t[3].data_ptr(), (int)(t[3].size(1) / 3), t[4].data_ptr(),
+        check_rc(tpl_a2c_backward(dt, t[0].data_ptr<uint8_t>(), (int)t[0].size(1), t[2].data_ptr(),
+                                  t[3].data_ptr(), (int)(t[3].size(1) / 3), t[4].data_ptr(), (int)t[4].size(2),
                                   nullptr, (int)t[1].size(0), stream),
                  "backward");
     };
@@ -174,6 +196,9 @@ void register_a2c(pybind11::module& m) {
           py::arg("atom_names"), st);
     m.def("Angles2Coords_backward", &backward, "gradient of the coordinates with respect to the angles (GPU)",
           py::arg("grad_atoms"), py::arg("grad_angles"), py::arg("sequences"), py::arg("input_angles"), st);
+    m.def("Angles2Coords_backward_coords", &backward_coords,
+          "the same from the forward's coordinates (GPU)", py::arg("grad_atoms"), py::arg("grad_angles"),
+          py::arg("sequences"), py::arg("coords"), st);
     m.def("getSeqNumAtoms", &num_atoms, "atoms of a sequence", py::arg("sequence"));
     m.def("a2c_bench", &bench, "average event ms of Angles2Coords (step, fwd, bwd)");
 }

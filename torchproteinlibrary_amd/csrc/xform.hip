@@ -6,10 +6,11 @@
 // atom index, one thread per structure: TransformCUDAKernels.cu:14-17,
 // 34-36,51-53,71-74,100-107), so neighbouring threads touch addresses
 // 12*atoms_stride bytes apart, and its centre is a serial loop in one thread
-// per structure (:77-86).  Here neighbouring threads take neighbouring atoms
-// (rotation) or neighbouring 16-byte blocks (translation, the centre's
-// backward) of one structure, every slot past num_atoms is written 0, and the
-// centre is a workgroup sum in FP64 per structure.
+// per structure (:77-86).  Here every global access is a 16-byte one at
+// consecutive addresses across a wave: translation and rotation stage a
+// wave's chunk of atoms through LDS, the centre's backward writes 16-byte
+// blocks directly; every slot past num_atoms is written 0, and the centre is
+// a workgroup sum in FP64 per structure.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -74,7 +75,9 @@ __global__ void __launch_bounds__(kThreads) xform_shift(T* out, const T* __restr
 #pragma unroll
                 for (int e = 0; e < E; ++e) v[e] = (T)0;
             }
-            int c = (int)(v0 % 3);
+            // v0 < 3A <= INT32_MAX here: a 32-bit remainder is exact (a 64-bit
+            // one is a long software sequence on the GPU)
+            int c = (int)((uint32_t)v0 % 3u);
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 v[e] = v0 + e < valid ? v[e] + t[c] : (T)0;
@@ -86,59 +89,114 @@ __global__ void __launch_bounds__(kThreads) xform_shift(T* out, const T* __restr
         } else {
             for (int e = 0; e < E && v0 + e < vals; ++e) {
                 const int64_t i = v0 + e;
-                o[i] = i < valid ? (p ? p[i] : (T)0) + t[i % 3] : (T)0;
+                o[i] = i < valid ? (p ? p[i] : (T)0) + t[(uint32_t)i % 3u] : (T)0;
             }
         }
     }
 }
 
-// out_i = R in_i or R^T in_i (transpose) for the atoms of the structure, 0
-// past num_atoms: one thread per atom slot, so that each store instruction of
-// a wave covers 768 contiguous bytes (a run of four atoms per thread with
-// 16-byte stores measured slower: 16.9 against 16.5 us at 1024 x 3000 atoms).
-// Grid: nblk blocks of kThreads atom slots per structure.
+// out_i = M in_i + t for the atoms of the structure (M = R, R^T with
+// transpose, or I when R is null; t = tv[b], or 0 when tv is null), 0 past
+// num_atoms, with every global access a 16-byte one at consecutive addresses
+// across the wave.  Each wave takes a chunk of kWaveAtoms atoms
+// (3 * kWaveAtoms values): it loads the chunk into its own slice of LDS with
+// coalesced 16-byte loads, each lane rotates four whole atoms there, and the
+// wave stores the chunk back with coalesced 16-byte stores.  The slice is the
+// wave's alone, so the wave needs no workgroup barrier; the wavefront fences
+// and the wave barrier keep the LDS writes before the reads.  A chunk that
+// runs past the structure's stride or starts unaligned goes atom by atom.
+// Grid: nblk blocks of 4 chunks per structure.
+constexpr int kWaveAtoms = 256;
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T>
-__global__ void __launch_bounds__(kThreads) xform_affine(T* out, const T* __restrict__ in,
-                                                         const T* __restrict__ R, int transpose,
-                                                         const T* __restrict__ tv, int per_n,
-                                                         const int32_t* __restrict__ num_atoms,
-                                                         uint32_t* status, int A, int nblk) {
+__device__ __forceinline__ void rotate_atom(const T m[9], int transpose, const T v[3], T w[3]) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+        w[r] = transpose ? m[r] * v[0] + m[3 + r] * v[1] + m[6 + r] * v[2]
+                         : m[3 * r] * v[0] + m[3 * r + 1] * v[1] + m[3 * r + 2] * v[2];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) xform_waves(T* out, const T* __restrict__ in,
+                                                        const T* __restrict__ R, int transpose,
+                                                        const T* __restrict__ tv,
+                                                        const int32_t* __restrict__ num_atoms,
+                                                        uint32_t* status, int A, int nblk) {
+    constexpr int E = 16 / (int)sizeof(T);  // values per 16-byte access
+    constexpr int V = 3 * kWaveAtoms;       // values per chunk
+    constexpr int kPerLane = kWaveAtoms / 64;
+    __shared__ T s_buf[kThreads / 64][V];
     const int b = blockIdx.x / nblk;
-    const int i = (blockIdx.x - b * nblk) * kThreads + threadIdx.x;
-    if (i >= A) return;
-    const int n = clamp_count(num_atoms[b], A, status, i == 0);
-    T* o = out + (size_t)b * 3 * A + 3 * (size_t)i;
-    if (i >= n) {
-        o[0] = (T)0;
-        o[1] = (T)0;
-        o[2] = (T)0;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int a0 = ((blockIdx.x - b * nblk) * (kThreads / 64) + wave) * kWaveAtoms;
+    if (a0 >= A) return;  // the whole wave: no workgroup barrier follows
+    const int n = clamp_count(num_atoms[b], A, status, a0 == 0 && lane == 0);
+    T* o = out + (size_t)b * 3 * A;
+    const T* p = in + (size_t)b * 3 * A;
+    T m[9], t[3] = {(T)0, (T)0, (T)0};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m[k] = R ? R[(size_t)b * 9 + k] : (T)(k % 4 == 0 ? 1 : 0);
+    if (tv) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t[c] = tv[3 * (size_t)b + c];
+    }
+    if (a0 + kWaveAtoms <= A && aligned16(o + 3 * (size_t)a0) && aligned16(p + 3 * (size_t)a0)) {
+        T* buf = s_buf[wave];
+        if (a0 < n) {
+#pragma unroll
+            for (int j = 0; j < V / (64 * E); ++j) {
+                const int idx = (j * 64 + lane) * E;
+                const uint4 u = *reinterpret_cast<const uint4*>(p + 3 * (size_t)a0 + idx);
+                *reinterpret_cast<uint4*>(buf + idx) = u;
+            }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int k = 0; k < kPerLane; ++k) {
+            const int q = kPerLane * lane + k;
+            T v[3] = {buf[3 * q], buf[3 * q + 1], buf[3 * q + 2]}, w[3] = {(T)0, (T)0, (T)0};
+            if (a0 + q < n) {
+                if (R) rotate_atom(m, transpose, v, w);
+                else w[0] = v[0], w[1] = v[1], w[2] = v[2];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) w[c] += t[c];
+            }
+            buf[3 * q] = w[0];
+            buf[3 * q + 1] = w[1];
+            buf[3 * q + 2] = w[2];
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < V / (64 * E); ++j) {
+            const int idx = (j * 64 + lane) * E;
+            *reinterpret_cast<uint4*>(o + 3 * (size_t)a0 + idx) = *reinterpret_cast<const uint4*>(buf + idx);
+        }
         return;
     }
-    T v[3] = {(T)0, (T)0, (T)0};
-    if (in) {
-        const T* p = in + (size_t)b * 3 * A + 3 * (size_t)i;
-        v[0] = p[0];
-        v[1] = p[1];
-        v[2] = p[2];
-    }
-    if (R) {
-        const T* m = R + (size_t)b * 9;
-        T w[3];
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
-            w[r] = transpose ? m[r] * v[0] + m[3 + r] * v[1] + m[6 + r] * v[2]
-                             : m[3 * r] * v[0] + m[3 * r + 1] * v[1] + m[3 * r + 2] * v[2];
+    for (int k = 0; k < kPerLane; ++k) {
+        const int i = a0 + kPerLane * lane + k;
+        if (i >= A) break;
+        T v[3] = {(T)0, (T)0, (T)0}, w[3] = {(T)0, (T)0, (T)0};
+        if (i < n) {
+            v[0] = p[3 * (size_t)i];
+            v[1] = p[3 * (size_t)i + 1];
+            v[2] = p[3 * (size_t)i + 2];
+            if (R) rotate_atom(m, transpose, v, w);
+            else w[0] = v[0], w[1] = v[1], w[2] = v[2];
 #pragma unroll
-        for (int r = 0; r < 3; ++r) v[r] = w[r];
+            for (int c = 0; c < 3; ++c) w[c] += t[c];
+        }
+        o[3 * (size_t)i] = w[0];
+        o[3 * (size_t)i + 1] = w[1];
+        o[3 * (size_t)i + 2] = w[2];
     }
-    if (tv) {
-        const T s = per_n ? (T)(1.0 / (double)n) : (T)1;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) v[c] += tv[3 * (size_t)b + c] * s;
-    }
-    o[0] = v[0];
-    o[1] = v[1];
-    o[2] = v[2];
 }
 
 // center[b] = (sum of the first n atoms) / n, summed in FP64: each thread over
@@ -203,6 +261,23 @@ int launch_affine(int dtype, void* out, const void* in, const void* R, int trans
                   const void* tv, int per_n, const int32_t* num_atoms, uint32_t* status, int B,
                   int A, tpl_xform_stream_t stream) {
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (in && !per_n) {
+        const int64_t per = (int64_t)(kThreads / 64) * kWaveAtoms;
+        const int64_t nb = ((int64_t)A + per - 1) / per;
+        if (nb * B > INT32_MAX) return TPL_XFORM_ERR_SIZE;
+        const int nblk = (int)nb;
+        if (dtype == TPL_XFORM_F32)
+            hipLaunchKernelGGL(xform_waves<float>, dim3(nblk * B), dim3(kThreads), 0, s,
+                               static_cast<float*>(out), static_cast<const float*>(in),
+                               static_cast<const float*>(R), transpose, static_cast<const float*>(tv),
+                               num_atoms, status, A, nblk);
+        else
+            hipLaunchKernelGGL(xform_waves<double>, dim3(nblk * B), dim3(kThreads), 0, s,
+                               static_cast<double*>(out), static_cast<const double*>(in),
+                               static_cast<const double*>(R), transpose, static_cast<const double*>(tv),
+                               num_atoms, status, A, nblk);
+        return hipGetLastError() == hipSuccess ? TPL_XFORM_OK : TPL_XFORM_ERR_LAUNCH;
+    }
     if (!R) {
         const int64_t per = (int64_t)kThreads * kBlocksPerThread * (dtype == TPL_XFORM_F32 ? 4 : 2);
         const int64_t nb = (3 * (int64_t)A + per - 1) / per;
@@ -218,19 +293,7 @@ int launch_affine(int dtype, void* out, const void* in, const void* R, int trans
                                static_cast<const double*>(tv), per_n, num_atoms, status, A, nblk);
         return hipGetLastError() == hipSuccess ? TPL_XFORM_OK : TPL_XFORM_ERR_LAUNCH;
     }
-    const int nblk = (A + kThreads - 1) / kThreads;
-    if ((int64_t)nblk * B > INT32_MAX) return TPL_XFORM_ERR_SIZE;
-    if (dtype == TPL_XFORM_F32)
-        hipLaunchKernelGGL(xform_affine<float>, dim3(nblk * B), dim3(kThreads), 0, s,
-                           static_cast<float*>(out), static_cast<const float*>(in),
-                           static_cast<const float*>(R), transpose, static_cast<const float*>(tv),
-                           per_n, num_atoms, status, A, nblk);
-    else
-        hipLaunchKernelGGL(xform_affine<double>, dim3(nblk * B), dim3(kThreads), 0, s,
-                           static_cast<double*>(out), static_cast<const double*>(in),
-                           static_cast<const double*>(R), transpose,
-                           static_cast<const double*>(tv), per_n, num_atoms, status, A, nblk);
-    return hipGetLastError() == hipSuccess ? TPL_XFORM_OK : TPL_XFORM_ERR_LAUNCH;
+    return TPL_XFORM_ERR_ARG;  // a rotation needs an input
 }
 
 }  // namespace

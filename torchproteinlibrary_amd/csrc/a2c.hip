@@ -33,9 +33,9 @@ namespace {
 
 using namespace a2c;
 
-// Workgroup sizes: 256 threads, or 512 for a few long structures (fewer
-// workgroups than CUs), where a thread then walks half as many residues.  Both
-// fit in LDS (37.9 / 63.3 KB) and in two waves per SIMD of VGPRs.
+// Workgroup sizes: 256 threads, or 512 for long structures (see use_large_fwd
+// and use_large_bwd), where a thread then walks half as many residues.  Both
+// fit in LDS (forward 37.9 / 63.3 KB) and in two waves per SIMD of VGPRs.
 constexpr int kThreadsSmall = 256, kThreadsLarge = 512;
 
 __constant__ Residue kRes[kTypes] = A2C_RESIDUES;
@@ -580,10 +580,16 @@ __global__ void __launch_bounds__(NT) a2c_bwd(const uint8_t* __restrict__ seqs, 
         if (k % ang_stride >= run.L) ga[k] = (T)0;
 }
 
-// 512-thread workgroups when the structures are fewer than the CUs and some
-// may be longer than 256 residues.
-bool use_large(int batch, int ang_stride) {
+// 512-thread workgroups when some structure may be longer than 256 residues:
+// always for the backward (mixed lengths 30-1000 at B=1024: 155 -> 129 us),
+// and for the forward only when the structures are fewer than the CUs (at
+// B=1024 its 147 VGPRs leave too few workgroups per CU: 91 -> 118 us).
+bool use_large_fwd(int batch, int ang_stride) {
     return batch < 256 && ang_stride > kThreadsSmall;
+}
+
+bool use_large_bwd(int ang_stride) {
+    return ang_stride > kThreadsSmall;
 }
 
 int check(int dtype, int B, int seq_stride, int ang_stride, int atoms_stride) {
@@ -632,7 +638,7 @@ int tpl_a2c_forward(int dtype, const uint8_t* sequences, int seq_stride, const v
     hipStream_t s = static_cast<hipStream_t>(stream);
     uint32_t* rn = reinterpret_cast<uint32_t*>(res_names);
     uint32_t* an = reinterpret_cast<uint32_t*>(atom_names);
-    const bool large = use_large(batch, angles_stride);
+    const bool large = use_large_fwd(batch, angles_stride);
     if (dtype == TPL_A2C_F32) {
         const float* a = static_cast<const float*>(angles);
         float* c = static_cast<float*>(coords);
@@ -664,7 +670,7 @@ int tpl_a2c_backward(int dtype, const uint8_t* sequences, int seq_stride, const 
     if (!sequences || !grad_angles || (atoms_stride > 0 && (!coords || !grad_coords)))
         return TPL_A2C_ERR_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const bool large = use_large(batch, angles_stride);
+    const bool large = use_large_bwd(angles_stride);
     if (dtype == TPL_A2C_F32) {
         const float *x = static_cast<const float*>(coords), *g = static_cast<const float*>(grad_coords);
         float* ga = static_cast<float*>(grad_angles);

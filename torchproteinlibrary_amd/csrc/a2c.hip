@@ -27,6 +27,7 @@
 #include <stdint.h>
 
 #include "a2c_table.h"
+#include "sincos_tab.h"
 #include "tpl_a2c.h"
 
 namespace {
@@ -51,6 +52,7 @@ struct Tables {
     int side_i[kTypes * kMaxSide];
     int res_i[kTypes];  // natoms | nca << 8 | nside << 16
     double bb[3][2];    // cos/sin of beta for the CA, C and N transforms
+    double tab[128];    // the sincos table of sincos_tab.h
 };
 
 template <int NT>
@@ -70,6 +72,7 @@ __device__ void fill_tables(Tables& tb) {
         const Residue r = kRes[threadIdx.x];
         tb.res_i[threadIdx.x] = r.natoms | (r.nca << 8) | (r.nside << 16);
     }
+    for (int k = threadIdx.x; k < 128; k += NT) tb.tab[k] = kTabCS[k];
     if (threadIdx.x == 0) {
         sincos(geo::C_N_CA, &tb.bb[0][1], &tb.bb[0][0]);
         sincos(geo::N_CA_C, &tb.bb[1][1], &tb.bb[1][0]);
@@ -158,11 +161,11 @@ __device__ __forceinline__ void residue_step(double m[12], const Tables& tb, con
     const double phi = angle(ang, 0, stride, i), psi = angle(ang, 1, stride, i),
                  omega = angle(ang, 2, stride, i);
     double s, c;
-    sincos(phi, &s, &c);
+    sincos_tab(phi, tb.tab, &s, &c);
     step(m, tb.bb[0][0], tb.bb[0][1], c, s, geo::R_N_CA);
-    sincos(psi, &s, &c);
+    sincos_tab(psi, tb.tab, &s, &c);
     step(m, tb.bb[1][0], tb.bb[1][1], c, s, geo::R_CA_C);
-    sincos(omega, &s, &c);
+    sincos_tab(omega, tb.tab, &s, &c);
     step(m, tb.bb[2][0], tb.bb[2][1], c, s, geo::R_C_N);
 }
 
@@ -299,7 +302,7 @@ __global__ void __launch_bounds__(NT) a2c_fwd(const uint8_t* __restrict__ seqs, 
         double s, c, F[12], CA[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) CA[k] = G[k];
-        sincos((double)av.v[0], &s, &c);
+        sincos_tab((double)av.v[0], tb.tab, &s, &c);
         step(CA, tb.bb[0][0], tb.bb[0][1], c, s, geo::R_N_CA);
         for (int k = 1; k <= nca; ++k) put(k, CA);
 #pragma unroll
@@ -310,11 +313,11 @@ __global__ void __launch_bounds__(NT) a2c_fwd(const uint8_t* __restrict__ seqs, 
             const int si = tb.side_i[q];
             const int row = (int)(int8_t)(si & 0xff), first = (si >> 8) & 0xff, count = si >> 16;
             double ca = tb.side[q][2], sa = tb.side[q][3];
-            if (row >= 0) sincos(rows.pop(), &sa, &ca);
+            if (row >= 0) sincos_tab(rows.pop(), tb.tab, &sa, &ca);
             step(F, tb.side[q][0], tb.side[q][1], ca, sa, tb.side[q][4]);
             for (int k = first; k < first + count; ++k) put(k, F);
         }
-        sincos((double)av.v[1], &s, &c);
+        sincos_tab((double)av.v[1], tb.tab, &s, &c);
         step(CA, tb.bb[1][0], tb.bb[1][1], c, s, geo::R_CA_C);  // CA now holds the C frame
         put(natoms - 2, CA);
         put(natoms - 1, CA);
@@ -325,7 +328,7 @@ __global__ void __launch_bounds__(NT) a2c_fwd(const uint8_t* __restrict__ seqs, 
                 an[a + k] = *reinterpret_cast<const uint32_t*>(kAtom[t][k].name);
             }
         }
-        sincos((double)av.v[2], &s, &c);
+        sincos_tab((double)av.v[2], tb.tab, &s, &c);
         step(CA, tb.bb[2][0], tb.bb[2][1], c, s, geo::R_C_N);
 #pragma unroll
         for (int k = 0; k < 12; ++k) G[k] = CA[k];

@@ -343,3 +343,33 @@ def workload(B, seed, min_len=30, max_len=1000, dtype=np.float64):
         ang[b, 2, :n] = PI + rng.normal(0, 0.05, n)
         ang[b, 3:, :n] = rng.uniform(-PI, PI, (5, n))
     return seqs, ang.astype(dtype).astype(np.float64)
+
+
+def backward_fast(seq, angles, grad_coords):
+    """The same gradient in O(atoms): subtree sums (sum g, sum x cross g)
+    accumulated from the leaves (every node comes after its parent in build
+    order), then dL/dalpha = u . (sum x cross g - p cross sum g) with u the
+    node frame's x axis and p its origin (F x = u cross (x - p) for the
+    reference's F = M_parent dT M^-1, cConformation.cpp:148).  Pinned to
+    backward_one in tests/test_a2c_oracle.py; used for long chains."""
+    coords, nodes, frames = forward_one(seq, angles)
+    x = coords.reshape(-1, 3)
+    g = np.asarray(grad_coords[:coords.size], dtype=np.float64).reshape(-1, 3)
+    S1 = np.zeros((len(nodes), 3))
+    S2 = np.zeros((len(nodes), 3))
+    for j, node in enumerate(nodes):
+        idx = node["index"]
+        if idx:
+            S1[j] = g[idx].sum(axis=0)
+            S2[j] = np.cross(x[idx], g[idx]).sum(axis=0)
+    for j in range(len(nodes) - 1, 0, -1):
+        par = nodes[j]["parent"]
+        if par >= 0:
+            S1[par] += S1[j]
+            S2[par] += S2[j]
+    grad = np.zeros_like(angles, dtype=np.float64)
+    for j, node in enumerate(nodes):
+        if isinstance(node["alpha"], tuple):
+            u, p = frames[j][:3, 0], frames[j][:3, 3]
+            grad[node["alpha"]] = u @ (S2[j] - np.cross(p, S1[j]))
+    return grad

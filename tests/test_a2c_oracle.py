@@ -166,3 +166,15 @@ def test_counts_and_names():
     assert res == ["GLY"] * 4 + ["PRO"] * 7
     with pytest.raises(ValueError):
         O.build("AXG")
+
+
+def test_fast_backward_equals_reference_form():
+    """backward_fast (subtree sums) equals backward_one (the reference's
+    per-angle sum of g . (F x) over the subtree) on every residue type."""
+    rng = np.random.default_rng(21)
+    for seq in (O.AA20, "".join(rng.choice(list(O.AA20), size=45)), "G", "PW"):
+        ang = _angles(rng, len(seq), len(seq) + 2)
+        c, _, _ = O.forward_one(seq, ang)
+        g = rng.normal(size=c.size)
+        a, b = O.backward_fast(seq, ang, g), O.backward_one(seq, ang, g)
+        assert np.abs(a - b).max() <= 1e-11 * max(1.0, np.abs(b).max())

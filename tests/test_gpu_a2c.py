@@ -213,3 +213,26 @@ def test_reference_signature_backward():
     ga = torch.empty_like(X)
     ext.Angles2Coords_backward(g, ga, convertStringList(seqs).to(DEV), X.detach())
     assert torch.allclose(ga, X.grad, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_long_chain_gradient_full_size(dtype):
+    """Two 1000-residue structures (the workloads' longest) against the O(atoms)
+    form of the oracle's backward: FP64 to 1e-9 relative, FP32 (whose
+    backward works from FP32 coordinates) to 1e-4 relative."""
+    from torchproteinlibrary_amd.FullAtomModel import Angles2Coords
+    rng = np.random.default_rng(31)
+    seqs, ang = _batch(rng, [1000, 1000], 1000, dtype)
+    print(f"a2c long chains: 2 x 1000 residues, dtype={dtype}", flush=True)
+    X = torch.tensor(ang, dtype=dtype, device=DEV, requires_grad=True)
+    coords = Angles2Coords()(X, seqs)[0]
+    g = rng.normal(size=tuple(coords.shape))
+    coords.backward(torch.tensor(g, dtype=dtype, device=DEV))
+    c = coords.detach().cpu().numpy()
+    for b in range(2):
+        want_c, _, _ = O.forward_one(seqs[b], ang[b])
+        assert np.abs(c[b, :want_c.size] - want_c).max() < (1e-9 if dtype == torch.float64 else 1e-3)
+        gq = np.asarray(g[b], np.float32 if dtype == torch.float32 else np.float64).astype(np.float64)
+        want = O.backward_fast(seqs[b], ang[b], gq)
+        rel = np.abs(X.grad[b].cpu().numpy() - want).max() / np.abs(want).max()
+        assert rel <= (1e-9 if dtype == torch.float64 else 1e-4), rel

@@ -54,6 +54,8 @@ def test_residue_template(dump, aa):
     # bound rows are 3, 4, ... in node order (the kernels take them in turn)
     bound = [row for row, *_ in sides.get(aa, []) if row >= 0]
     assert bound == list(range(3, 3 + len(bound)))
+    # only the first side-chain node is without atoms (the backward's parent rule)
+    assert all(cnt > 0 for k, (*_, cnt) in enumerate(sides.get(aa, [])) if k > 0)
     first = 1 + nca
     for k, ((row, alpha, beta, d, f0, cnt), node) in enumerate(zip(sides.get(aa, []), O.SIDE[aa])):
         parent, al, b, dd, group = node

@@ -521,18 +521,38 @@ __global__ void __launch_bounds__(NT) a2c_bwd(const uint8_t* __restrict__ seqs, 
         sums_zero(sC);
         atom_sums(xs, gs, a, a + 1, sN);
         atom_sums(xs, gs, a + 1, a + 1 + nca, sCA);
-        atom_sums(xs, gs, a + 1 + nca, a + natoms - 2, sSide);
         atom_sums(xs, gs, a + natoms - 2, a + natoms, sC);
         double pN[3], pCA[3], pC[3];
         load3(xs + 3 * (size_t)a, pN);
         load3(xs + 3 * (size_t)(a + 1), pCA);
         load3(xs + 3 * (size_t)(a + natoms - 2), pC);
+        for (int r = 3; r < 8; ++r) ga[(size_t)r * ang_stride + i] = (T)0;
+        // side chain, last node first, so that sSide holds node n's group and
+        // the later nodes' groups when node n is reached (each atom read
+        // once).  Node n hangs on node n-1, or on CA when n-1 is the fixed
+        // first node (no atoms, at CA's origin; every other node has atoms).
+        for (int n = nside - 1; n >= 0; --n) {
+            const int si = s_side[t * kMaxSide + n];
+            const int row = (int)(int8_t)(si & 0xff), first = (si >> 8) & 0xff, count = si >> 16;
+            if (count == 0) continue;
+            atom_sums(xs, gs, a + first, a + first + count, sSide);
+            if (row >= 0) {
+                const int sp = n > 0 ? s_side[t * kMaxSide + n - 1] : 0;
+                const int pfirst = (sp >> 8) & 0xff, pcount = sp >> 16;
+                double p[3], q[3];
+                load3(xs + 3 * (size_t)(a + first), p);
+                if (pcount > 0)
+                    load3(xs + 3 * (size_t)(a + pfirst), q);
+                else
+                    q[0] = pCA[0], q[1] = pCA[1], q[2] = pCA[2];
+                ga[(size_t)row * ang_stride + i] = (T)bond_torque(p, q, sSide);
+            }
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             excl.g[c] = incl.g[c] - (sN.g[c] + sCA.g[c] + sSide.g[c] + sC.g[c]);
             excl.xg[c] = incl.xg[c] - (sN.xg[c] + sCA.xg[c] + sSide.xg[c] + sC.xg[c]);
         }
-        for (int r = 3; r < 8; ++r) ga[(size_t)r * ang_stride + i] = (T)0;
         // omega of residue i-1 binds N_i (on the previous C): the whole residue and everything after
         if (i > 0) {
             double pCp[3];
@@ -554,27 +574,6 @@ __global__ void __launch_bounds__(NT) a2c_bwd(const uint8_t* __restrict__ seqs, 
             s.xg[c] = sC.xg[c] + excl.xg[c];
         }
         ga[(size_t)ang_stride + i] = (T)bond_torque(pC, pCA, s);
-        // side chain: node n binds its group and the later nodes' groups; a
-        // node without atoms (the fixed first node) sits at its parent's origin
-        double q[3] = {pCA[0], pCA[1], pCA[2]};
-        Sums rest = sSide;
-        for (int n = 0; n < nside; ++n) {
-            const int si = s_side[t * kMaxSide + n];
-            const int row = (int)(int8_t)(si & 0xff), first = (si >> 8) & 0xff, count = si >> 16;
-            if (count == 0) continue;
-            double p[3];
-            load3(xs + 3 * (size_t)(a + first), p);
-            if (row >= 0) ga[(size_t)row * ang_stride + i] = (T)bond_torque(p, q, rest);
-            Sums mine;
-            sums_zero(mine);
-            atom_sums(xs, gs, a + first, a + first + count, mine);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                rest.g[c] -= mine.g[c];
-                rest.xg[c] -= mine.xg[c];
-                q[c] = p[c];
-            }
-        }
         incl = excl;
         a += natoms;
     }

@@ -171,10 +171,11 @@ def test_batch_mixed_lengths_full_size():
 
 
 def test_large_workgroups_match_small_and_oracle():
-    """A few long structures (batch < 256, stride > 256) run on 512-thread
-    workgroups; the same structures padded to a batch of 256 with empty
-    sequences run on 256-thread workgroups.  Both agree with each other and
-    with the oracle (FP64)."""
+    """A few long structures (batch < 256, stride > 256) run the forward on
+    512-thread workgroups; the same structures padded to a batch of 256 with
+    empty sequences run it on 256-thread workgroups (the backward takes 512
+    threads in both, for the stride).  Both agree with each other and with
+    the oracle (FP64)."""
     from torchproteinlibrary_amd.FullAtomModel import Angles2Coords
     rng = np.random.default_rng(12)
     lengths = [300, 520, 7, 260]

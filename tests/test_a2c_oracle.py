@@ -178,3 +178,25 @@ def test_fast_backward_equals_reference_form():
         g = rng.normal(size=c.size)
         a, b = O.backward_fast(seq, ang, g), O.backward_one(seq, ang, g)
         assert np.abs(a - b).max() <= 1e-11 * max(1.0, np.abs(b).max())
+
+
+def test_matches_reference_fixtures():
+    """Against outputs the reference's compiled code wrote
+    (tests/golden/a2c_reference.npz, made by tests/golden/make_a2c_fixtures.py):
+    the pin that holds where the reference sources are absent."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "a2c_reference.npz")
+    z = np.load(path)
+    k = 0
+    while f"seq{k}" in z:
+        seq = bytes(z[f"seq{k}"]).decode()
+        ang = z[f"angles{k}"]
+        c, _, _ = O.forward_one(seq, ang)
+        assert np.abs(c - z[f"coords{k}"]).max() < 1e-12
+        assert O.names_one(seq)[1] == list(z[f"atom_names{k}"])
+        ga = O.backward_one(seq, ang, z[f"grad_coords{k}"])
+        want = z[f"grad_angles{k}"]
+        assert np.abs(ga - want).max() <= 1e-11 * max(1.0, np.abs(want).max())
+        assert np.abs(O.backward_fast(seq, ang, z[f"grad_coords{k}"]) - want).max() <= 1e-11 * max(1.0, np.abs(want).max())
+        k += 1
+    assert k == 3

@@ -36,7 +36,8 @@ using namespace a2c;
 
 // Workgroup sizes: 256 threads, or 512 for long structures (see use_large_fwd
 // and use_large_bwd), where a thread then walks half as many residues.  Both
-// fit in LDS (forward 37.9 / 63.3 KB) and in two waves per SIMD of VGPRs.
+// fit in LDS (forward 38.8 / 64.4 KB, backward 14 / 27 KB) and in two waves
+// per SIMD of VGPRs.
 constexpr int kThreadsSmall = 256, kThreadsLarge = 512;
 
 __constant__ Residue kRes[kTypes] = A2C_RESIDUES;
@@ -588,7 +589,8 @@ __global__ void __launch_bounds__(NT) a2c_bwd(const uint8_t* __restrict__ seqs, 
 // 512-thread workgroups when some structure may be longer than 256 residues:
 // always for the backward (mixed lengths 30-1000 at B=1024: 155 -> 129 us),
 // and for the forward only when the structures are fewer than the CUs (at
-// B=1024 its 147 VGPRs leave too few workgroups per CU: 91 -> 118 us).
+// B=1024 512 threads measured slower: 85 -> 89 us, with fewer workgroups per
+// CU).
 bool use_large_fwd(int batch, int ang_stride) {
     return batch < 256 && ang_stride > kThreadsSmall;
 }

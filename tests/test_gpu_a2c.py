@@ -237,3 +237,26 @@ def test_long_chain_gradient_full_size(dtype):
         want = O.backward_fast(seqs[b], ang[b], gq)
         rel = np.abs(X.grad[b].cpu().numpy() - want).max() / np.abs(want).max()
         assert rel <= (1e-9 if dtype == torch.float64 else 1e-4), rel
+
+
+def test_matches_reference_outputs():
+    """FP64 forward and backward on the GPU against outputs the reference's own
+    compiled CPU code wrote (tests/golden/a2c_reference.npz): coordinates,
+    atom names and gradients for the 20-residue sequence and two others."""
+    import os
+    from torchproteinlibrary_amd.FullAtomModel import Angles2Coords
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "a2c_reference.npz"))
+    k = 0
+    while f"seq{k}" in z:
+        seq = bytes(z[f"seq{k}"]).decode()
+        X = torch.tensor(z[f"angles{k}"][None], device=DEV, requires_grad=True)
+        coords, _, atom_names, n = Angles2Coords()(X, [seq])
+        want = z[f"coords{k}"]
+        assert int(n[0]) * 3 == want.size
+        assert np.abs(coords[0].detach().cpu().numpy() - want).max() < 1e-10
+        assert _names(atom_names[0]) == list(z[f"atom_names{k}"])
+        coords.backward(torch.tensor(z[f"grad_coords{k}"][None], device=DEV))
+        wg = z[f"grad_angles{k}"]
+        assert np.abs(X.grad[0].cpu().numpy() - wg).max() <= 1e-10 * max(1.0, np.abs(wg).max())
+        k += 1
+    assert k == 3

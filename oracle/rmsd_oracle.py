@@ -15,7 +15,9 @@ It follows the reference step by step, in numpy FP64:
   uses numpy.linalg.eigh, which is exact for a symmetric matrix);
 * U from q (:35-45) and UT[j][k] = U[k][j] (:90-93);
 * R2 = sum |xc|^2 + |yc|^2 (:169-180) and
-  rmsd = sqrt((R2 - 2 |lambda| + 1e-10) / n) (:182);
+  rmsd = sqrt((R2 - 2 |lambda| + 1e-10) / n) (:182), with R2 - 2 |lambda|
+  clamped at 0 (rounding can make it negative for identical structures,
+  where the reference's formula gives NaN);
 * the gradient (xc - UT yc) * g / (rmsd * n), zero past n
   (Coords2RMSD.py:47-56; CoordsRotate is dst = R src,
   Layers/FullAtomModel/TransformCUDAKernels.cu:41-50).
@@ -75,7 +77,9 @@ def forward_one(x, y, n):
     lam = w[k]
     U = u_from_q(V[:, k]) if lam > 0 else np.eye(3)
     r2 = float((xc * xc).sum() + (yc * yc).sum())
-    rmsd = np.sqrt((r2 - 2.0 * abs(lam) + EPS_LIMIT) / n)
+    # clamped at 0 (FP64 rounding can make r2 - 2|lambda| negative for
+    # identical structures; the reference's formula then gives NaN)
+    rmsd = np.sqrt((max(r2 - 2.0 * abs(lam), 0.0) + EPS_LIMIT) / n)
     return rmsd, U.T.copy(), U, cx, cy
 
 

@@ -83,6 +83,57 @@ def test_rigid_copy_has_zero_rmsd():
         assert np.abs(r.cpu().numpy()).max() < 0.5 * 10 ** (-places), dtype
 
 
+def test_identical_structures_give_identity_and_zero_gradient():
+    """input == target: UT = I, RMSD of the order of sqrt(1e-10 / n) (the
+    reference's epsilon; the FP64 rounding of R2 - 2 lambda is of the same
+    order and is clamped at 0 from below) and a zero gradient.  The
+    quaternion is (1, 0, 0, 0), so three of its four adjugate rows are
+    rounding noise: the eigenvector must come from the row of the largest
+    |q_i|."""
+    from torchproteinlibrary_amd.RMSD import Coords2RMSD
+    x, _, n, _ = R.workload(16, 200, seed=5, min_atoms=3)
+    for dtype in (torch.float64, torch.float32):
+        X = torch.tensor(x, dtype=dtype, device=DEV, requires_grad=True)
+        N = torch.tensor(n, device=DEV)
+        m = Coords2RMSD()
+        r = m(X, X.detach().clone(), N)
+        r.backward(torch.ones_like(r))
+        torch.cuda.synchronize()
+        ut = m.UT.cpu().numpy()
+        assert np.abs(ut - np.eye(3)[None]).max() <= 1e-6, dtype
+        assert np.all(np.isfinite(r.detach().cpu().numpy())), dtype
+        assert np.all(r.detach().cpu().numpy() <= np.sqrt(3e-10 / n)), dtype
+        assert np.abs(X.grad.cpu().numpy()).max() <= 1e-3, dtype
+
+
+@pytest.mark.parametrize("axis,angle", [((0, 0, 1), 0.7), ((1, 0, 0), 0.7), ((0, 1, 0), -1.3),
+                                        ((0, 0, 1), np.pi), ((1, 1, 0), np.pi)])
+def test_exact_special_rotations(axis, angle):
+    """y = Q x for rotations whose quaternions have zero components
+    (single-axis turns, half turns): U = Q and RMSD ~ 0, against the oracle
+    and the exact rotation."""
+    from torchproteinlibrary_amd.RMSD import Coords2RMSD
+    ax = np.asarray(axis, np.float64) / np.linalg.norm(axis)
+    Q = R.u_from_q(np.concatenate([[np.cos(angle / 2)], np.sin(angle / 2) * ax]))
+    x, _, n, g = R.workload(16, 120, seed=6, min_atoms=3)
+    y = np.zeros_like(x)
+    for b in range(16):
+        k = int(n[b])
+        y[b, : 3 * k] = (x[b, : 3 * k].reshape(k, 3) @ Q.T).reshape(-1)
+    X = torch.tensor(x, device=DEV, requires_grad=True)
+    m = Coords2RMSD()
+    r = m(X, torch.tensor(y, device=DEV), torch.tensor(n, device=DEV))
+    r.backward(torch.tensor(g, device=DEV))
+    torch.cuda.synchronize()
+    ut = m.UT.cpu().numpy()
+    assert np.abs(ut - Q.T[None]).max() <= 1e-8
+    assert np.abs(r.detach().cpu().numpy()).max() <= 1e-4
+    r64, ut64 = R.forward(x, y, n)
+    assert np.abs(ut - ut64).max() <= 1e-8
+    g64 = R.backward(x, y, n, g)
+    assert np.abs(X.grad.cpu().numpy() - g64).max() <= 1e-6 * max(np.abs(g64).max(), 1.0)
+
+
 def test_gradcheck_fp64():
     """The reference's backward test (test.py:67-83): torch.autograd.gradcheck
     on the Module, batch 1, up to 16 atoms, eps 1e-6, atol 1e-5, rtol 1e-5."""

@@ -12,9 +12,12 @@
 //     Acta Cryst. A61 (2005) 478-480).  F is symmetric, so every root is
 //     real and Newton from above the largest root decreases monotonically
 //     onto it;
-//   * q from a column of adj(F - mu I), the column of the largest diagonal
-//     entry.  For a simple eigenvalue mu = lambda, where adj(F - lambda I) =
-//     P'(lambda) q q^T.  When the largest eigenvalue is (nearly) double,
+//   * q from a column of adj(F - mu I), the column of the diagonal entry
+//     largest in magnitude.  For a simple eigenvalue mu = lambda, where
+//     adj(F - lambda I) = prod_{j != max} (lambda_j - lambda) q q^T
+//     = -P'(lambda) q q^T: the three factors are negative, so every diagonal
+//     entry -P'(lambda) q_i^2 is <= 0 and the most negative one belongs to
+//     the largest |q_i|.  When the largest eigenvalue is (nearly) double,
 //     P'(lambda) vanishes and that adjugate carries no direction; then mu =
 //     lambda (1 + 1e-8), where adj(F - mu I) = sum_k prod_{j!=k} (lambda_j -
 //     mu) q_k q_k^T is dominated by the top eigenspace (its terms are O(delta)
@@ -145,15 +148,17 @@ RMSD_HD Eigenpair max_eigenpair(const double F[16], double bound) {
     double A[16];
     RMSD_UNROLL
     for (int i = 0; i < 16; ++i) A[i] = F[i] - ((i % 5 == 0) ? mu : 0.0);
-    // adj(A) row i has diagonal entry minor(i, i) = P'(lambda) q_i^2: the row
-    // with the largest carries q best.
+    // adj(A) row i has diagonal entry minor(i, i) = -P'(lambda) q_i^2 <= 0
+    // (mu >= every eigenvalue, so each of the three factors lambda_j - mu is
+    // <= 0, in both branches): the most negative row carries q best; a row
+    // with q_i = 0 holds only rounding noise.
     const double d0 = minor3<0, 0>(A), d1 = minor3<1, 1>(A), d2 = minor3<2, 2>(A),
                  d3 = minor3<3, 3>(A);
     int r = 0;
     double best = d0;
-    if (d1 > best) { best = d1; r = 1; }
-    if (d2 > best) { best = d2; r = 2; }
-    if (d3 > best) r = 3;
+    if (d1 < best) { best = d1; r = 1; }
+    if (d2 < best) { best = d2; r = 2; }
+    if (d3 < best) r = 3;
     double w0, w1, w2, w3;
     switch (r) {
         case 0: cofactor_row<0>(A, w0, w1, w2, w3); break;

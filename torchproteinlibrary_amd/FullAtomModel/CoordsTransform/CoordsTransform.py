@@ -4,8 +4,8 @@ structures.
 Drop-in for TorchProteinLibrary/FullAtomModel/CoordsTransform/CoordsTransform.py
 (GPU path).  The names (``CoordsTranslate``, ``CoordsRotate``,
 ``Coords2Center`` and their ``...Function`` classes, ``getRotation``,
-``getRandomRotation``), the calls, the shapes and the error behaviour are the
-reference's:
+``getRandomRotation``, ``getBBox``, ``getRandomTranslation``), the calls, the
+shapes and the error behaviour are the reference's:
 
 * coordinates ``[B, 3*A]`` contiguous, float32 or float64, ``num_atoms`` int32
   ``[B]`` on the same device; ``T``/centre ``[B, 3]``, ``R`` ``[B, 3, 3]``;
@@ -20,10 +20,10 @@ What differs, by design: one thread per atom slot of one structure
 (neighbouring threads, neighbouring atoms) instead of one thread per structure
 and one block per atom index, and the centre summed in FP64 by a workgroup per
 structure; ``num_atoms`` outside ``[0, A]`` is clamped and reported; CPU
-tensors are rejected (the GPU path only).  ``getRotation`` and
-``getRandomRotation`` build rotations on the host, as the reference's do
-(Math/nUtil.cpp:60-79), with torch's random generator in place of the C
-library's.
+tensors are rejected (the GPU path only).  ``getRotation``,
+``getRandomRotation``, ``getBBox`` and ``getRandomTranslation`` are small
+host utilities in the reference (Math/nUtil.cpp:27-99) and stay host-side
+torch code here, with torch's random generator as in the reference.
 """
 from __future__ import annotations
 
@@ -38,7 +38,8 @@ from torch.nn.modules.module import Module
 from ..._native_xform import ext
 
 __all__ = ["CoordsTranslate", "CoordsTranslateFunction", "CoordsRotate", "CoordsRotateFunction",
-           "Coords2Center", "Coords2CenterFunction", "getRotation", "getRandomRotation"]
+           "Coords2Center", "Coords2CenterFunction", "getRotation", "getRandomRotation",
+           "getBBox", "getRandomTranslation"]
 
 _STATUS_COUNT = 2
 
@@ -178,3 +179,38 @@ def getRandomRotation(batch_size, generator=None):
     """Uniformly random rotations [B, 3, 3] (float64, CPU, like the reference's
     CoordsTransform.py:35-38)."""
     return getRotation(torch.rand(batch_size, 3, dtype=torch.float64, generator=generator))
+
+
+def getBBox(input_coords, num_atoms):
+    """Bounding box of each structure's first num_atoms atoms: (a, b), the
+    per-axis minimum and maximum, [B, 3] each, in the dtype of the
+    coordinates, on the CPU (reference CoordsTransform.py:12-22 and
+    Math/nUtil.cpp:27-57).  A structure with no atoms gets a = +inf,
+    b = -inf, the reference's starting values."""
+    if len(input_coords.size()) != 2:
+        raise ValueError('getBBox: ', 'Incorrect input size:', input_coords.size())
+    x = input_coords.detach().cpu()
+    B = x.size(0)
+    xyz = x.reshape(B, -1, 3)
+    n = num_atoms.detach().cpu().to(torch.int64).clamp(0, xyz.size(1))
+    valid = (torch.arange(xyz.size(1))[None, :] < n[:, None]).unsqueeze(-1)
+    inf = torch.tensor(float("inf"), dtype=x.dtype)
+    a = torch.where(valid, xyz, inf).amin(dim=1)
+    b = torch.where(valid, xyz, -inf).amax(dim=1)
+    return a, b
+
+
+def getRandomTranslation(a, b, volume_size, generator=None):
+    """Random translations [B, 3] that keep each box [a, b] inside a cubic
+    volume of side volume_size: per axis uniform in [-d, d] with
+    d = max(0, volume_size / 2 - (b - a) / 2) / 2, d in float32 as the
+    reference computes it (reference CoordsTransform.py:24-33,
+    Math/nUtil.cpp:88-99)."""
+    if len(a.size()) != 2:
+        raise ValueError('getRandomTranslation: ', 'Incorrect input size:', a.size())
+    a = a.detach().cpu()
+    b = b.detach().cpu()
+    half = torch.clamp(float(volume_size) / 2.0 - (b.to(torch.float64) - a.to(torch.float64)) / 2.0, min=0.0)
+    d = (half * 0.5).to(torch.float32).to(torch.float64)
+    u = torch.rand(a.size(0), 3, dtype=torch.float64, generator=generator)
+    return (u * (2.0 * d) - d).to(a.dtype)

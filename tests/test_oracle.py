@@ -98,6 +98,29 @@ def test_forward_matches_brute_force():
         assert np.all(x[b, 9 * n[b]:] == 0.0)
 
 
+def test_frames_match_brute_force():
+    """Per-atom frames A_i (rows of the 3x4 affine part) against the literal
+    4x4 products; column 0 of A_i is the unit bond vector ending at atom i
+    and the translation is atom i; zero past the chain."""
+    a, n, _ = _ragged(4, B=5, L=9)
+    fr = O.frames(a, n)
+    x = O.forward(a, n)
+    params = [(0, BETA["phi"], BOND["N-CA"]), (1, BETA["psi"], BOND["CA-C"]),
+              (2, BETA["omega"], BOND["C-N"])]
+    for b in range(len(n)):
+        A = np.eye(4)
+        for i in range(3 * n[b]):
+            if i > 0:
+                k, beta, d = params[(i - 1) % 3]
+                A = A @ _ry(beta) @ _tx(d) @ _rx(a[b, k, (i - 1) // 3])
+            np.testing.assert_allclose(fr[b, i], A[:3], rtol=0, atol=1e-12)
+            np.testing.assert_array_equal(fr[b, i, :, 3], x[b, 3 * i: 3 * i + 3])
+            if i > 0:
+                u = x[b, 3 * i: 3 * i + 3] - x[b, 3 * i - 3: 3 * i]
+                np.testing.assert_allclose(fr[b, i, :, 0], u / np.linalg.norm(u), rtol=0, atol=1e-12)
+        assert np.all(fr[b, 3 * n[b]:] == 0.0)
+
+
 # ------------------------------------------------------------ invariants
 
 def _atoms(x, n):

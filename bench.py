@@ -12,15 +12,24 @@ every step streams from HBM.
 Prints one JSON line.  ``value`` is residues/s (sum of chain lengths per
 second).  The line also carries:
 * ``roofline``: the slower of the two kernels, timed alone with HIP events on
-  the stream it runs on; algorithmic bytes per launch (``alg_bytes``) / average
-  duration against the MI355X 8 TB/s HBM peak, and the HBM bytes per launch the
-  counters measured (``profiles/traffic_<workload>.json``);
+  the stream it runs on; its compulsory bytes per launch (``alg_bytes``: the
+  dense-tensor API's reads and writes, nothing else) / average duration
+  against the MI355X 8 TB/s HBM peak, and ``traffic``, the HBM bytes per launch
+  that two ``rocprofv3 --pmc`` runs of this script's probe mode count in the
+  same run (FETCH_SIZE, then WRITE_SIZE, each scaled by what it counts for a
+  256 MiB device copy in the same run);
 * ``graph``: the same step replayed from a HIP graph that holds one forward +
   backward per buffer set;
 * ``plain``: the same step with the plain kernels (serial forward, O(L^2)
   backward) and the speedup of the scan path over it;
-* ``cpu_baseline``: the reference CPU algorithm on a bounded sample of the same
-  workload, on the host, one thread;
+* ``cpu_baseline``: the reference's own CPU code (built from its sources into
+  oracle/_ref) on its CPU configuration, B=32 chains of 200 residues, FP64,
+  one thread, one warm-up then the median of 5; with the host's CPU model;
+* ``cpu_path``: this repository's host path (libtpl_a2b_cpu.so) on the same
+  configuration, all host threads;
+* ``fit500``: the optimisation loop of docs/Examples/fitBackbone.py at B=512
+  (lengths 50-500), 1000 Adam steps in float32 and in float64: steps/s and
+  the final losses;
 * ``rmsd``: the Coords2RMSD loss (forward + backward) on the workload's
   backbone atoms against a moved, noisy copy, with its own roofline;
 * ``xform``: the coordinate transforms (translate, rotate, centre and its
@@ -49,20 +58,21 @@ MALL_BYTES = 256 << 20
 
 
 def alg_bytes(n, L, io_bytes=4):
-    """Compulsory HBM bytes of one forward and one backward launch.
+    """Compulsory HBM bytes of one forward and one backward launch: what the
+    layer's dense-tensor API must read and write, nothing else.
 
-    forward:  read angles (3 per residue), write coords (9 per slot incl.
-              padding), write the FP32 rounding residual of every coordinate
-              (9 halves per residue, the forward->backward state), read lengths;
-    backward: read coords and grad coords (9 + 9 per residue) and the residuals
-              (9 halves per residue), write grad angles (3 per slot incl.
-              padding), read lengths."""
+    forward:  read angles (3 per residue), write coords (9 per slot, padding
+              included), read lengths;
+    backward: read grad coords (9 per residue), re-read angles (3 per residue,
+              the minimal forward->backward state), write grad angles (3 per
+              slot, padding included), read lengths.
+    The checkpoint frames the forward saves (96 bytes per backward chunk) are
+    not credited."""
     import numpy as np
     B = len(n)
     S = int(np.sum(n))
-    res = 2 * 9 * S if io_bytes == 4 else 0
-    fwd = io_bytes * (3 * S + 9 * B * L) + res + 4 * B
-    bwd = io_bytes * (9 * S + 9 * S + 3 * B * L) + res + 4 * B
+    fwd = io_bytes * (3 * S + 9 * B * L) + 4 * B
+    bwd = io_bytes * (9 * S + 3 * S + 3 * B * L) + 4 * B
     return fwd, bwd
 
 
@@ -79,10 +89,13 @@ def parse():
     p.add_argument("--no-rmsd", action="store_true", help="skip the Coords2RMSD line")
     p.add_argument("--no-a2c", action="store_true", help="skip the Angles2Coords line")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-seconds", type=float, default=15.0)
-    p.add_argument("--config", default="", help="pin tile config 'waves,res_per_lane,max_wgs'")
-    p.add_argument("--calibrate-copy", action="store_true",
-                   help="also run 5 device copies of 256 MiB (counter calibration)")
+    p.add_argument("--cpu-seconds", type=float, default=15.0,
+                   help="time budget of the Angles2Coords CPU baseline")
+    p.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs")
+    p.add_argument("--no-fit", action="store_true", help="skip the fit500 line")
+    p.add_argument("--probe", action="store_true",
+                   help="(internal) launch each kernel --steps times and 5 device copies, for rocprofv3")
+    p.add_argument("--config", default="", help="pin config 'fwd_res_per_lane,bwd_res_per_lane'")
     p.add_argument("--fit", action="store_true",
                    help="run the fit500 Adam optimisation loop instead (own JSON line)")
     p.add_argument("--fit-steps", type=int, default=1000)
@@ -107,7 +120,7 @@ def time_kernels(sets, cfg, reps):
             if name == "forward":
                 ext.Angles2BackboneGPU_forward(A, X, N, st, sy, None, list(cfg))
             else:
-                ext.Angles2BackboneGPU_backward(GA, G, X, N, st, None, list(cfg))
+                ext.Angles2BackboneGPU_backward(GA, G, A, N, st, sy, None, list(cfg))
             if rec:
                 evs[i - 10][1].record(cur)
         torch.cuda.synchronize()
@@ -116,39 +129,182 @@ def time_kernels(sets, cfg, reps):
     return res
 
 
-def cpu_baseline(a, n, g, seconds):
-    """The reference CPU algorithm (FP64, single thread): its own kernels
-    compiled from the reference sources (oracle/_ref) if present, else this
-    repository's C restatement with the same derivative-tensor behaviour.
-    Chains are taken in workload order, one call per chain at the workload's
-    padded length, until the time budget is used."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline():
+    """The reference's CPU path on its CPU configuration (uniform200: B=32,
+    L=200, FP64): its own kernels compiled from its sources (oracle/_ref) if
+    present, else this repository's C restatement with the same
+    derivative-tensor behaviour.  One thread; buffers zero-filled inside the
+    timed call as the reference's Function does; one warm-up, then the median
+    of 5 forward+backward calls."""
     import numpy as np
     from oracle import oracle as O
+    from torchproteinlibrary_amd import synthetic
+    a, n, g = synthetic.workload("uniform200")
+    a, g = a.astype(np.float64), g.astype(np.float64)
     kind = "reference" if O.ref_available() else "port"
-    L = a.shape[2]
-    done_res, t_total, k = 0, 0.0, 0
-    while k < len(n) and t_total < seconds:
-        a1 = a[k:k + 1].astype(np.float64)
-        n1 = n[k:k + 1]
-        g1 = g[k:k + 1].astype(np.float64)
-        t0 = time.perf_counter()
+
+    def once():
         if kind == "reference":
-            O.ref_forward_backward(a1, n1, g1)
+            O.ref_forward_backward(a, n, g)
         else:
-            O.forward(a1, n1)
-            O.backward_dense(a1, n1, g1)
-        t_total += time.perf_counter() - t0
-        done_res += int(n1[0])
-        k += 1
+            O.forward(a, n)
+            O.backward_dense(a, n, g)
+    once()
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        once()
+        ts.append(time.perf_counter() - t0)
+    t = sorted(ts)[2]
     return {
-        "value": done_res / t_total,
+        "value": int(n.sum()) / t,
         "unit": "residues/s",
         "cores": 1,
         "kind": kind,
-        "sample": f"first {k} chains of the workload ({done_res} residues, L_max={L}), one "
-                  f"call per chain, FP64, dr_dangle materialised; {t_total:.1f} s on "
-                  f"{os.cpu_count()} visible host cores",
+        "sample": f"uniform200 (B=32, L=200, 6400 residues), FP64 forward+backward with the "
+                  f"O(L^2) derivative tensor zero-filled and reduced as the reference does; "
+                  f"median of 5 after a warm-up: {t:.3f} s per call",
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
     }
+
+
+def cpu_path_line():
+    """This repository's host path (libtpl_a2b_cpu.so through the layer, CPU
+    float64 tensors) on uniform200, forward+backward, all host threads (at
+    most 16): median of 20 after a warm-up."""
+    import numpy as np
+    import torch
+    from torchproteinlibrary_amd import synthetic
+    from torchproteinlibrary_amd.ReducedModel import Angles2Backbone
+    a, n, g = synthetic.workload("uniform200")
+    threads = min(16, os.cpu_count() or 1)
+    old = torch.get_num_threads()
+    torch.set_num_threads(threads)
+    try:
+        A = torch.tensor(a, dtype=torch.float64, requires_grad=True)
+        N = torch.tensor(n)
+        G = torch.tensor(g, dtype=torch.float64)
+        m = Angles2Backbone()
+        m(A, N).backward(G)
+        ts = []
+        for _ in range(20):
+            t0 = time.perf_counter()
+            m(A, N).backward(G)
+            ts.append(time.perf_counter() - t0)
+    finally:
+        torch.set_num_threads(old)
+    t = sorted(ts)[len(ts) // 2]
+    return {"value": int(n.sum()) / t, "unit": "residues/s", "threads": threads,
+            "ms_per_step": 1e3 * t,
+            "sample": "uniform200 (B=32, L=200), FP64, forward + O(L) adjoint backward"}
+
+
+def probe(args):
+    """Counter-probe mode (run under rocprofv3 by traffic()): each kernel alone
+    --steps times over rotating buffer sets, then 5 device copies of 256 MiB
+    for the counters' calibration."""
+    import numpy as np
+    import torch
+    from torchproteinlibrary_amd import synthetic
+    from torchproteinlibrary_amd._native import ext
+    dev = torch.device("cuda:0")
+    a, n, g = synthetic.workload(args.workload)
+    B, _, L = a.shape
+    cfg = [int(v) for v in args.config.split(",")]
+    st_b, sy_b = ext.workspace_bytes(False, B, L, cfg)
+    sets = []
+    for _ in range(6):
+        sets.append((torch.tensor(a, device=dev), torch.tensor(n, device=dev), torch.tensor(g, device=dev),
+                     torch.empty(B, 9 * L, device=dev), torch.empty(B, 3, L, device=dev),
+                     torch.empty(st_b, dtype=torch.uint8, device=dev)))
+    sy = torch.zeros(sy_b, dtype=torch.uint8, device=dev)
+    for A, N, G, X, GA, st in sets:
+        ext.Angles2BackboneGPU_forward(A, X, N, st, sy, None, cfg)
+    for i in range(args.steps):
+        A, N, G, X, GA, st = sets[i % 6]
+        ext.Angles2BackboneGPU_forward(A, X, N, st, sy, None, cfg)
+    for i in range(args.steps):
+        A, N, G, X, GA, st = sets[i % 6]
+        ext.Angles2BackboneGPU_backward(GA, G, A, N, st, sy, None, cfg)
+    src = torch.ones(256 << 20, dtype=torch.uint8, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print("probe done", flush=True)
+
+
+def _pmc_medians(outdir, counter):
+    """{kernel name: median counter value over its dispatches} from a
+    rocprofv3 --output-format csv run."""
+    import csv
+    import glob
+    import statistics
+    vals = {}
+    for f in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(f, newline="") as fh:
+            for row in csv.DictReader(fh):
+                if row.get("Counter_Name") != counter:
+                    continue
+                vals.setdefault(row.get("Kernel_Name", ""), []).append(float(row["Counter_Value"]))
+    return {k: statistics.median(v) for k, v in vals.items()}
+
+
+def traffic(args, cfg):
+    """HBM bytes per launch of the forward and backward kernels: two
+    rocprofv3 --pmc runs of the probe mode (FETCH_SIZE, then WRITE_SIZE, in
+    KiB per dispatch), each scaled by what it counts for a 256 MiB device copy
+    in the same run.  None if rocprofv3 is not there or a run fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            od = os.path.join(d, counter)
+            cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", od, "-o", "p", "--",
+                   sys.executable, os.path.abspath(__file__), "--probe", "--steps", "24",
+                   "--workload", args.workload, "--config", ",".join(str(c) for c in cfg)]
+            print(f"bench: counter run {counter}", file=sys.stderr, flush=True)
+            try:
+                r = subprocess.run(cmd, capture_output=True, text=True, timeout=120,
+                                   env=dict(os.environ, TMPDIR=d))
+            except subprocess.TimeoutExpired:
+                return None
+            if "probe done" not in r.stdout:
+                return None
+            med = _pmc_medians(od, counter)
+            fwd = [v for k, v in med.items() if "a2b_fwd_chunk" in k]
+            bwd = [v for k, v in med.items() if "a2b_bwd_chunk" in k]
+            cpy = [v for k, v in med.items() if "a2b_" not in k and ("copy" in k.lower() or "elementwise" in k.lower())]
+            if not (fwd and bwd and cpy):
+                return None
+            copy_kib = 262144.0
+            scale = copy_kib / max(cpy)
+            out[counter] = {"forward_kib": fwd[0], "backward_kib": bwd[0], "copy_kib": max(cpy), "scale": scale}
+    res = {"fetch_scale": out["FETCH_SIZE"]["scale"], "write_scale": out["WRITE_SIZE"]["scale"]}
+    for k in ("forward", "backward"):
+        res[k] = 1024.0 * (out["FETCH_SIZE"][k + "_kib"] * out["FETCH_SIZE"]["scale"] +
+                           out["WRITE_SIZE"][k + "_kib"] * out["WRITE_SIZE"]["scale"])
+        res[k + "_fetch_kib"] = out["FETCH_SIZE"][k + "_kib"]
+        res[k + "_write_kib"] = out["WRITE_SIZE"][k + "_kib"]
+    return res
 
 
 def fit_loop(steps: int):
@@ -447,6 +603,9 @@ def pdb_bench(files, atoms_per_file, with_cpu, cpu_seconds):
 
 def main():
     args = parse()
+    if args.probe:
+        probe(args)
+        return
     if args.fit:
         out = {"metric": "fit500 optimisation loop: Adam steps/s (fwd+bwd+update), B=512",
                "unit": "steps/s", "steps": args.fit_steps, "n_gpus": 1, "data": "synthetic",
@@ -498,7 +657,7 @@ def main():
         ksets.append((A.detach(), N, G, torch.empty(B, 9 * L, device=dev),
                       torch.empty(B, 3, L, device=dev),
                       torch.empty(max(st_b, 1), dtype=torch.uint8, device=dev),
-                      torch.empty(max(sy_b, 1), dtype=torch.uint8, device=dev)))
+                      torch.zeros(max(sy_b, 1), dtype=torch.uint8, device=dev)))
 
     m = Angles2Backbone()
     for A, N, G in gsets[:1]:  # one eager autograd step through the public API
@@ -512,14 +671,12 @@ def main():
     t_wall = wall_ms / 1e3
 
     kern = time_kernels(ksets, cfg, args.kernel_reps)
+    kern["forward"]["alg_bytes"], kern["backward"]["alg_bytes"] = fwd_b, bwd_b
+    for k in ("forward", "backward"):
+        kern[k]["alg_GBs"] = kern[k]["alg_bytes"] / (kern[k]["avg_ms"] * 1e-3) / 1e9
     dom = max(("forward", "backward"), key=lambda k: kern[k]["avg_ms"])
     dom_bytes = fwd_b if dom == "forward" else bwd_b
     achieved = dom_bytes / (kern[dom]["avg_ms"] * 1e-3) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get(dom)
 
     out = {
         "metric": METRIC,
@@ -540,7 +697,7 @@ def main():
             "max_length": L,
             "residues": residues,
             "io_dtype": "f32",
-            "tile": list(cfg),
+            "res_per_lane": {"forward": cfg[0], "backward": cfg[1]},
             "launch": "native loop, back-to-back fwd+bwd launches",
             "buffer_sets": sets_n,
             "step_alg_bytes": fwd_b + bwd_b,
@@ -554,7 +711,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
+            "traffic": None,
             "alg_bytes_per_launch": dom_bytes,
             "avg_ms": kern[dom]["avg_ms"],
             "kernels": kern,
@@ -591,16 +748,31 @@ def main():
         out["plain"] = {"value": pv, "ms_per_step": 1e3 * pt / args.plain_steps,
                         "speedup_scan_over_plain": value / pv}
 
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(a, n, g, args.cpu_seconds)
+    if not args.no_fit:
+        fit = fit_loop(1000)
+        out["fit500"] = {
+            "workload": "fit500: B=512, lengths uniform 50-500, 1000 Adam steps (lr 1e-2) of "
+                        "forward + loss + backward + update, eager, per-call checks off",
+            "steps_per_s_f32": fit["f32"]["steps_per_s"], "steps_per_s_f64": fit["f64"]["steps_per_s"],
+            "loss_start": fit["f64"]["loss_start"],
+            "loss_final_f32": fit["f32"]["loss_final"], "loss_final_f64": fit["f64"]["loss_final"],
+            "final_loss_rel_diff": abs(fit["f32"]["loss_final"] - fit["f64"]["loss_final"])
+            / max(fit["f64"]["loss_final"], 1e-30)}
 
-    if args.calibrate_copy:
-        src = torch.ones(256 << 20, dtype=torch.uint8, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_path"] = cpu_path_line()
+        out["cpu_path"]["speedup_over_reference_cpu"] = out["cpu_path"]["value"] / out["cpu_baseline"]["value"]
+
+    # Counted HBM bytes of the two kernels: rocprofv3 runs of the probe mode,
+    # after every other GPU measurement of this process.
+    if not args.no_traffic:
         torch.cuda.synchronize()
-        out["calibration_copy_bytes"] = src.numel()
+        tr = traffic(args, cfg)
+        if tr is not None:
+            out["roofline"]["traffic"] = tr[dom]
+            out["roofline"]["traffic_over_alg"] = tr[dom] / dom_bytes
+            out["roofline"]["counters"] = tr
 
     print(json.dumps(out), flush=True)
 

@@ -21,7 +21,7 @@
  * no atom.  The reference instead reads frames past the chain there.
  *
  * dtype is TPL_A2B_F32 or TPL_A2B_F64 for the angle/coordinate/gradient
- * arrays.  The scan kernels always compute in FP64.
+ * arrays.  The chunk kernels always compute in FP64.
  *
  * Streams: `stream` is a hipStream_t (NULL = the legacy default stream).  All
  * work is enqueued on it in stream order.  No call synchronises or allocates,
@@ -29,6 +29,9 @@
  *
  * `state` and `sync` buffers must start on a 256-byte boundary (hipMalloc and
  * torch allocations do); a misaligned one returns TPL_A2B_ERR_ARG.
+ *
+ * Lengths outside [0, angles_stride] are clamped on the device and reported
+ * in *status (the reference writes out of bounds there).
  *
  * Return value: TPL_A2B_OK (0) or an error code; tpl_a2b_strerror() names it.
  * Device-side conditions go to the optional `status` word as bits, because a
@@ -52,7 +55,7 @@ enum tpl_a2b_dtype { TPL_A2B_F32 = 0, TPL_A2B_F64 = 1 };
 enum tpl_a2b_error {
     TPL_A2B_OK = 0,
     TPL_A2B_ERR_ARG = 1,    /* null pointer, negative size or unknown dtype */
-    TPL_A2B_ERR_CONFIG = 2, /* unsupported tile configuration */
+    TPL_A2B_ERR_CONFIG = 2, /* unsupported chunk configuration */
     TPL_A2B_ERR_LAUNCH = 3, /* a kernel launch failed */
     TPL_A2B_ERR_HIP = 4,    /* another HIP runtime call failed */
     TPL_A2B_ERR_SIZE = 5    /* batch * angles_stride too large */
@@ -61,23 +64,22 @@ enum tpl_a2b_error {
 /* Bits OR-ed into *status by the kernels. */
 #define TPL_A2B_STATUS_NAN 1u    /* an output element is NaN */
 #define TPL_A2B_STATUS_LENGTH 2u /* some lengths[b] < 0 or > angles_stride (clamped) */
-#define TPL_A2B_STATUS_TIMEOUT 4u /* a look-back wait gave up (result invalid) */
+#define TPL_A2B_STATUS_TIMEOUT 4u /* a look-back wait gave up (result invalid; clear the
+                                     sync buffer before its next use) */
+#define TPL_A2B_STATUS_SYNC 8u    /* the sync buffer was not all zero at the call (result
+                                     invalid; clear it before its next use) */
 
-/* Launch shape of the scan kernels.  A forward tile is 64*waves*res_per_lane
- * consecutive residues of one chain; one workgroup scans one tile, and chains
- * longer than a tile are chained across workgroups by decoupled look-back.
- * Every lane keeps its residues' inputs and outputs in registers, which bounds
- * the tile: at most 1024 residues for TPL_A2B_F32 and 512 for TPL_A2B_F64
- * (larger tiles return TPL_A2B_ERR_CONFIG).  The backward holds more per
- * residue and runs one residue per lane with the same waves. */
+/* Launch shape of the chunk kernels.  A forward chunk is 64*fwd_res_per_lane
+ * consecutive residues of one chain and a backward chunk 64*bwd_res_per_lane;
+ * one 64-lane wave takes one chunk, and the chunks of one chain are chained
+ * by a decoupled look-back.  The backward chunk is also the checkpoint
+ * stride: the forward saves the FP64 frame at every backward-chunk boundary,
+ * so forward and backward of one step must use the same config. */
 typedef struct tpl_a2b_config {
-    int32_t waves;        /* 64-lane waves per workgroup: 1, 2 or 4 */
-    int32_t res_per_lane; /* residues composed serially per lane: 1, 2, 4 or 8 */
-    int32_t max_wgs;      /* forward grid cap (persistent workgroups walk the
-                             tiles; with one tile per chain each loads its next
-                             chain during the current one); 0 = one workgroup
-                             per tile.  The backward runs one workgroup per tile. */
-    int32_t flags;        /* reserved, must be 0 */
+    int32_t fwd_res_per_lane; /* 1, 2, 4 or 8 (TPL_A2B_F64: up to 4) */
+    int32_t bwd_res_per_lane; /* 1, 2 or 4 (TPL_A2B_F64: up to 2) */
+    int32_t flags;            /* reserved, must be 0 */
+    int32_t reserved;         /* reserved, must be 0 */
 } tpl_a2b_config;
 
 /* Human-readable name of an error code. */
@@ -87,53 +89,61 @@ const char *tpl_a2b_strerror(int code);
  * configuration is known. */
 int tpl_a2b_default_config(int batch, int angles_stride, tpl_a2b_config *cfg);
 
-/* Residues per tile of cfg, or -1 if cfg is invalid. */
-int64_t tpl_a2b_tile_residues(const tpl_a2b_config *cfg);
+/* Residues per forward (backward = 0) or backward (backward != 0) chunk of
+ * cfg, or -1 if cfg is invalid. */
+int64_t tpl_a2b_chunk_residues(const tpl_a2b_config *cfg, int backward);
 
-/* Bytes of the forward->backward state for this dtype and config.  For
- * TPL_A2B_F32 it holds, per output coordinate, the rounding residual of the
- * forward's FP64 value as a scaled half (2 bytes), so that the backward
- * rebuilds FP64-grade coordinates from the FP32 ones; when chains span
- * several backward tiles it also holds the backward's look-back words.  The
- * forward writes all of it; the backward leaves its look-back words as it
- * found them, so one state can serve repeated backward calls.  It replaces
- * the reference's per-atom 4x4 frame buffer A (Angles2Backbone.py:21) and is
- * 0 for TPL_A2B_F64 when every chain fits in one backward tile. */
+/* Bytes of the forward->backward state: one FP64 3x4 frame per backward chunk
+ * of every chain (96 bytes per 64*bwd_res_per_lane residues), the checkpoints
+ * from which the backward recomputes its frames.  It replaces the reference's
+ * per-atom 4x4 frame buffer A (Angles2Backbone.py:21, 192 bytes per residue
+ * in FP32). */
 size_t tpl_a2b_state_bytes(int dtype, int batch, int angles_stride, const tpl_a2b_config *cfg);
 
-/* Bytes of the forward's look-back scratch for one call (0 when every chain
- * fits in one forward tile).  The library clears what must start at zero
- * itself, with a kernel on `stream`; contents need not survive the call. */
+/* Bytes of the look-back words of forward and backward (per-chain tickets,
+ * done counters and flags, then the payload they guard).  The buffer must be
+ * all zero before its first use (tpl_a2b_sync_clear, or any zero fill); every
+ * call that ends without TPL_A2B_STATUS_TIMEOUT returns all it wrote to zero,
+ * so one buffer serves every later call on the same stream, of any shape and
+ * config that fits in it.  Calls that overlap in time (different streams)
+ * need different buffers.  A call that finds a ticket word out of range
+ * reports TPL_A2B_STATUS_SYNC and leaves that chain's outputs unwritten. */
 size_t tpl_a2b_sync_bytes(int dtype, int batch, int angles_stride, const tpl_a2b_config *cfg);
+
+/* Zeroes `bytes` of `sync` with a kernel on `stream` (capture-safe: no
+ * memset node).  bytes must be a multiple of 4. */
+int tpl_a2b_sync_clear(void *sync, size_t bytes, tpl_stream_t stream);
 
 /* Forward: angles -> coords.  Replaces Angles2BackboneGPU_forward
  * (Layers/ReducedModel/Angles2Backbone/angles2backbone_interface.cpp:7-26) and
  * its kernel computeCoordinatesBackbone
  * (Layers/ReducedModel/cBackboneProteinCUDAKernels.cu:6-41,164-167).
- * Writes `state` (tpl_a2b_state_bytes) for the backward; with state == NULL
- * (a forward that no backward follows) it skips that work. */
+ * Writes the checkpoints into `state` (tpl_a2b_state_bytes) for the backward;
+ * with state == NULL (a forward that no backward follows) it skips them. */
 int tpl_a2b_forward(int dtype, const void *angles, void *coords, const int32_t *lengths,
                     void *state, void *sync, uint32_t *status, int batch, int angles_stride,
                     const tpl_a2b_config *cfg, tpl_stream_t stream);
 
-/* Backward: d(loss)/d(coords) -> d(loss)/d(angles), O(L) reverse-scan
- * adjoint over the forward's coordinates.  Replaces Angles2BackboneGPU_backward
- * (angles2backbone_interface.cpp:27-56) and its kernels
+/* Backward: d(loss)/d(coords) -> d(loss)/d(angles), the O(L) reverse-scan
+ * adjoint.  Replaces Angles2BackboneGPU_backward
+ * (angles2backbone_interface.cpp:27-56: gradInput, gradOutput, input_angles,
+ * angles_length, A, dr_dangle) and its kernels
  * computeGradientsOptimizedBackbone + backwardFromCoordinatesBackbone
- * (cBackboneProteinCUDAKernels.cu:84-163,169-178).  Where the reference passes
- * the input angles and the saved frames A, this takes the forward's output
- * `coords` and `state` as the forward wrote them for the same lengths and cfg.
+ * (cBackboneProteinCUDAKernels.cu:84-163,169-178), argument for argument:
+ * `state` (the checkpoints the forward wrote for the same angles, lengths and
+ * cfg) stands where A stood, `sync` where the O(L^2) dr_dangle stood.
  * grad_angles is fully overwritten. */
-int tpl_a2b_backward(int dtype, void *grad_angles, const void *grad_coords, const void *coords,
-                     const int32_t *lengths, void *state, uint32_t *status, int batch,
-                     int angles_stride, const tpl_a2b_config *cfg, tpl_stream_t stream);
+int tpl_a2b_backward(int dtype, void *grad_angles, const void *grad_coords, const void *angles,
+                     const int32_t *lengths, const void *state, void *sync, uint32_t *status,
+                     int batch, int angles_stride, const tpl_a2b_config *cfg,
+                     tpl_stream_t stream);
 
 /* ---- plain form: the reference algorithm written as straightforward HIP ----
  * One thread per chain runs the serial recurrence and stores every atom's 3x4
  * frame.  The backward uses one thread per (chain, angle) with the per-pair
  * formula A_{m-1} dB_m A_m^{-1} x_i summed over downstream atoms (O(L^2) per
  * chain), without the dr_dangle tensor.  It computes in the I/O dtype.  It is
- * the comparison baseline for the scan kernels and an independent GPU check. */
+ * the comparison baseline for the chunk kernels and an independent GPU check. */
 
 /* Bytes of the per-atom frame buffer of the plain form. */
 size_t tpl_a2b_plain_frames_bytes(int dtype, int batch, int angles_stride);
@@ -155,7 +165,7 @@ int tpl_a2b_backward_plain(int dtype, void *grad_angles, const void *grad_coords
  * (median of `reps` runs after one warm-up, hipEvents on `stream`).  It
  * writes the fastest candidate to *best and each candidate's ms to
  * ms_per_candidate (may be NULL; invalid candidates get -1).  state and sync
- * must be sized for the largest candidate.  It synchronises the stream, so
+ * must be sized for the largest candidate (sync zeroed).  It synchronises the stream, so
  * it must not be called under stream capture.  It has no reference
  * counterpart: the reference has a single fixed launch shape. */
 int tpl_a2b_tune(int dtype, const void *angles, const int32_t *lengths,

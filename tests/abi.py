@@ -10,8 +10,8 @@ LIB = os.path.join(ROOT, "torchproteinlibrary_amd", "libtpl_a2b.so")
 
 
 class Config(ctypes.Structure):
-    _fields_ = [("waves", ctypes.c_int32), ("res_per_lane", ctypes.c_int32),
-                ("max_wgs", ctypes.c_int32), ("flags", ctypes.c_int32)]
+    _fields_ = [("fwd_res_per_lane", ctypes.c_int32), ("bwd_res_per_lane", ctypes.c_int32),
+                ("flags", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 def declared_functions():
@@ -25,17 +25,19 @@ def load():
     L.tpl_a2b_strerror.restype = ctypes.c_char_p
     L.tpl_a2b_strerror.argtypes = [i32]
     L.tpl_a2b_default_config.argtypes = [i32, i32, cfgp]
-    L.tpl_a2b_tile_residues.restype = ctypes.c_int64
-    L.tpl_a2b_tile_residues.argtypes = [cfgp]
+    L.tpl_a2b_chunk_residues.restype = ctypes.c_int64
+    L.tpl_a2b_chunk_residues.argtypes = [cfgp, i32]
+    L.tpl_a2b_sync_clear.argtypes = [vp, ctypes.c_size_t, vp]
     for f in (L.tpl_a2b_state_bytes, L.tpl_a2b_sync_bytes):
         f.restype = ctypes.c_size_t
         f.argtypes = [i32, i32, i32, cfgp]
     L.tpl_a2b_plain_frames_bytes.restype = ctypes.c_size_t
     L.tpl_a2b_plain_frames_bytes.argtypes = [i32, i32, i32]
     L.tpl_a2b_forward.argtypes = [i32, vp, vp, vp, vp, vp, vp, i32, i32, cfgp, vp]
-    L.tpl_a2b_backward.argtypes = [i32, vp, vp, vp, vp, vp, vp, i32, i32, cfgp, vp]
+    L.tpl_a2b_backward.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, i32, i32, cfgp, vp]
     L.tpl_a2b_forward_plain.argtypes = [i32, vp, vp, vp, vp, vp, i32, i32, vp]
     L.tpl_a2b_backward_plain.argtypes = [i32, vp, vp, vp, vp, vp, vp, i32, i32, vp]
     L.tpl_a2b_tune.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, i32, i32, cfgp, i32, i32,
                                cfgp, ctypes.POINTER(ctypes.c_float), vp]
     return L
+

@@ -15,7 +15,7 @@ def lib():
 
 def test_exports_every_declared_symbol(lib):
     names = abi.declared_functions()
-    assert len(names) == 11, names
+    assert len(names) == 12, names
     for name in names:
         assert hasattr(lib, name), f"{name} not exported by libtpl_a2b.so"
 
@@ -27,71 +27,83 @@ def test_strerror(lib):
     assert lib.tpl_a2b_strerror(99) == b"unknown error code"
 
 
+def _align(x):
+    return (x + 255) // 256 * 256
+
+
+def _state(B, L, rb):
+    # one FP64 3x4 frame (96 bytes) per backward chunk of 64*rb residues
+    return _align(B * -(-L // (64 * rb)) * 96)
+
+
+def _sync(B, L, rf, rb):
+    # per direction: tickets [B], done [B], flags [B][nc] (uint32), then the
+    # payload [B][nc] of 24 doubles (forward frames) or 12 (backward sums)
+    def region(nc, pay):
+        return _align(_align(8 * B + 4 * B * nc) + B * nc * pay * 8)
+    return region(-(-L // (64 * rf)), 24) + region(-(-L // (64 * rb)), 12)
+
+
 def test_config_validation_and_sizes(lib):
-    c = abi.Config(4, 4, 0, 0)
-    assert lib.tpl_a2b_tile_residues(ctypes.byref(c)) == 1024
-    for bad in [abi.Config(3, 1, 0, 0), abi.Config(8, 1, 0, 0), abi.Config(1, 3, 0, 0),
-                abi.Config(1, 1, -1, 0), abi.Config(1, 1, 0, 1)]:
-        assert lib.tpl_a2b_tile_residues(ctypes.byref(bad)) == -1
-    def align(x):
-        return (x + 255) // 256 * 256
-
-    def state(f32, B, L, ntb):
-        # FP32 residual halves [B][9][L rounded up to 8], then (ntb > 1) the
-        # backward payload (12 doubles per tile) + flags + tickets + done
-        res = align(B * 9 * ((L + 7) // 8 * 8) * 2) if f32 else 0
-        return res if ntb <= 1 else align(res + B * ntb * (96 + 4) + 8 * B)
-
-    def sync(B, ntf):  # flags + tickets (zeroed), then 24-double payload per tile
-        return 0 if ntf <= 1 else align(4 * (B * ntf + B)) + B * ntf * 24 * 8
-
-    # (4,4): forward tiles of 1024; the backward runs 1 residue per lane, so
-    # 256-residue tiles
-    assert lib.tpl_a2b_state_bytes(0, 1024, 1000, ctypes.byref(c)) == state(True, 1024, 1000, 4)
-    assert lib.tpl_a2b_sync_bytes(0, 1024, 1000, ctypes.byref(c)) == 0
-    assert lib.tpl_a2b_state_bytes(0, 64, 16384, ctypes.byref(c)) == state(True, 64, 16384, 64)
-    assert lib.tpl_a2b_sync_bytes(0, 64, 16384, ctypes.byref(c)) == sync(64, 16)
-    c2 = abi.Config(2, 4, 0, 0)  # forward 512, backward 128 residues
-    assert lib.tpl_a2b_state_bytes(0, 512, 500, ctypes.byref(c2)) == state(True, 512, 500, 4)
-    assert lib.tpl_a2b_sync_bytes(0, 512, 500, ctypes.byref(c2)) == 0
-    assert lib.tpl_a2b_state_bytes(1, 512, 500, ctypes.byref(c2)) == state(False, 512, 500, 4)
-    c1 = abi.Config(4, 1, 0, 0)  # 256 both ways: f64 needs no state
-    assert lib.tpl_a2b_state_bytes(1, 8, 256, ctypes.byref(c1)) == 0
-    assert lib.tpl_a2b_state_bytes(0, 8, 256, ctypes.byref(c1)) == state(True, 8, 256, 1)
-    assert lib.tpl_a2b_state_bytes(7, 512, 500, ctypes.byref(c2)) == 0  # unknown dtype
+    c = abi.Config(4, 2, 0, 0)
+    assert lib.tpl_a2b_chunk_residues(ctypes.byref(c), 0) == 256
+    assert lib.tpl_a2b_chunk_residues(ctypes.byref(c), 1) == 128
+    for bad in [abi.Config(3, 1, 0, 0), abi.Config(16, 1, 0, 0), abi.Config(1, 8, 0, 0),
+                abi.Config(1, 3, 0, 0), abi.Config(1, 1, 1, 0), abi.Config(1, 1, 0, 1)]:
+        assert lib.tpl_a2b_chunk_residues(ctypes.byref(bad), 0) == -1
+    for dt in (0, 1):
+        assert lib.tpl_a2b_state_bytes(dt, 1024, 1000, ctypes.byref(c)) == _state(1024, 1000, 2)
+        assert lib.tpl_a2b_sync_bytes(dt, 1024, 1000, ctypes.byref(c)) == _sync(1024, 1000, 4, 2)
+    c8 = abi.Config(8, 4, 0, 0)
+    assert lib.tpl_a2b_state_bytes(0, 64, 16384, ctypes.byref(c8)) == _state(64, 16384, 4)
+    assert lib.tpl_a2b_sync_bytes(0, 64, 16384, ctypes.byref(c8)) == _sync(64, 16384, 8, 4)
+    c1 = abi.Config(1, 1, 0, 0)
+    assert lib.tpl_a2b_state_bytes(0, 3, 37, ctypes.byref(c1)) == _state(3, 37, 1)
+    assert lib.tpl_a2b_state_bytes(7, 512, 500, ctypes.byref(c1)) == 0  # unknown dtype
+    assert lib.tpl_a2b_state_bytes(0, 0, 500, ctypes.byref(c1)) == 0
     assert lib.tpl_a2b_plain_frames_bytes(0, 2, 10) == 2 * 30 * 12 * 4
     assert lib.tpl_a2b_plain_frames_bytes(1, 2, 10) == 2 * 30 * 12 * 8
 
 
 def test_default_config(lib):
     c = abi.Config()
-    for L in (1, 200, 500, 1000, 16384):
+    for L in (1, 64, 200, 500, 1000, 16384):
         assert lib.tpl_a2b_default_config(8, L, ctypes.byref(c)) == 0
-        assert lib.tpl_a2b_tile_residues(ctypes.byref(c)) > 0
+        assert lib.tpl_a2b_chunk_residues(ctypes.byref(c), 0) > 0
+        assert c.fwd_res_per_lane <= 4 and c.bwd_res_per_lane <= 2  # valid for both dtypes
     assert lib.tpl_a2b_default_config(-1, 10, ctypes.byref(c)) == 1
 
 
 def test_argument_errors_return_before_launch(lib):
     c = abi.Config(1, 1, 0, 0)
     p = ctypes.c_void_p(0x1000)  # never dereferenced: every call below fails validation
-    assert lib.tpl_a2b_forward(7, p, p, p, None, None, None, 1, 1, ctypes.byref(c), None) == 1
-    assert lib.tpl_a2b_forward(0, None, p, p, None, None, None, 1, 1, ctypes.byref(c), None) == 1
-    assert lib.tpl_a2b_forward(0, p, p, p, None, None, None, -1, 1, ctypes.byref(c), None) == 1
+    q = ctypes.c_void_p(0x1001)  # misaligned state / sync
+    assert lib.tpl_a2b_forward(7, p, p, p, None, p, None, 1, 1, ctypes.byref(c), None) == 1
+    assert lib.tpl_a2b_forward(0, None, p, p, None, p, None, 1, 1, ctypes.byref(c), None) == 1
+    assert lib.tpl_a2b_forward(0, p, p, p, None, p, None, -1, 1, ctypes.byref(c), None) == 1
+    assert lib.tpl_a2b_forward(0, p, p, p, None, None, None, 1, 1, ctypes.byref(c), None) == 1  # no sync
+    assert lib.tpl_a2b_forward(0, p, p, p, q, p, None, 1, 1, ctypes.byref(c), None) == 1
+    assert lib.tpl_a2b_forward(0, p, p, p, p, q, None, 1, 1, ctypes.byref(c), None) == 1
     bad = abi.Config(3, 1, 0, 0)
-    assert lib.tpl_a2b_forward(0, p, p, p, None, None, None, 1, 1, ctypes.byref(bad), None) == 2
-    # tiles above the register-tile limit: 1024 residues (f32), 512 (f64)
-    big = abi.Config(4, 8, 0, 0)
-    assert lib.tpl_a2b_forward(0, p, p, p, p, p, None, 1, 1, ctypes.byref(big), None) == 2
-    f64big = abi.Config(4, 4, 0, 0)
-    assert lib.tpl_a2b_backward(1, p, p, p, p, p, None, 1, 1, ctypes.byref(f64big), None) == 2
-    # multi-tile forward without its scratch; backward without its state
-    assert lib.tpl_a2b_forward(0, p, p, p, None, None, None, 2, 1000, ctypes.byref(c), None) == 1
-    assert lib.tpl_a2b_backward(0, p, p, p, p, None, None, 2, 1000, ctypes.byref(c), None) == 1
-    assert lib.tpl_a2b_backward(0, None, p, p, p, None, None, 1, 10, ctypes.byref(c), None) == 1
+    assert lib.tpl_a2b_forward(0, p, p, p, None, p, None, 1, 1, ctypes.byref(bad), None) == 2
+    # register limits of FP64: forward chunks up to 4 residues per lane, backward 2
+    f64big = abi.Config(8, 1, 0, 0)
+    assert lib.tpl_a2b_forward(1, p, p, p, p, p, None, 1, 1, ctypes.byref(f64big), None) == 2
+    f64bwd = abi.Config(1, 4, 0, 0)
+    assert lib.tpl_a2b_backward(1, p, p, p, p, p, p, None, 1, 1, ctypes.byref(f64bwd), None) == 2
+    # backward without its state or its sync
+    assert lib.tpl_a2b_backward(0, p, p, p, p, None, p, None, 2, 1000, ctypes.byref(c), None) == 1
+    assert lib.tpl_a2b_backward(0, p, p, p, p, p, None, None, 2, 1000, ctypes.byref(c), None) == 1
+    assert lib.tpl_a2b_backward(0, None, p, p, p, p, p, None, 1, 10, ctypes.byref(c), None) == 1
     assert lib.tpl_a2b_forward_plain(0, p, p, p, None, None, 1, 1, None) == 1
     assert lib.tpl_a2b_backward_plain(5, p, p, p, p, p, None, 1, 1, None) == 1
+    assert lib.tpl_a2b_sync_clear(None, 256, None) == 1
+    assert lib.tpl_a2b_sync_clear(p, 6, None) == 1
+    assert lib.tpl_a2b_sync_clear(None, 0, None) == 0
+    # too large: 9 * L beyond int32
+    assert lib.tpl_a2b_forward(0, p, p, p, None, p, None, 1, 300_000_000, ctypes.byref(c), None) == 5
     # empty problems are no-ops
     assert lib.tpl_a2b_forward(0, None, None, None, None, None, None, 0, 5, ctypes.byref(c), None) == 0
-    assert lib.tpl_a2b_backward(0, None, None, None, None, None, None, 3, 0, ctypes.byref(c), None) == 0
+    assert lib.tpl_a2b_backward(0, None, None, None, None, None, None, None, 3, 0, ctypes.byref(c), None) == 0
     assert lib.tpl_a2b_tune(0, p, p, p, p, p, None, None, 1, 1, ctypes.byref(c), 0, 3,
                             ctypes.byref(c), None, None) == 1

@@ -17,7 +17,7 @@ def _p(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
-@pytest.mark.parametrize("cfg", [(4, 4, 0), (1, 1, 0), (2, 2, 5)])
+@pytest.mark.parametrize("cfg", [(4, 2), (1, 1), (2, 4)])
 def test_forward_backward_via_c_abi(cfg):
     lib = abi.load()
     a, n, g = synthetic.custom([300, 17, 0, 256, 1, 299], 300, seed=99)
@@ -26,21 +26,24 @@ def test_forward_backward_via_c_abi(cfg):
     N = torch.tensor(n, device=dev)
     G = torch.tensor(g, device=dev)
     B, _, L = a.shape
-    c = abi.Config(*cfg, 0)
+    c = abi.Config(*cfg, 0, 0)
     st = torch.empty(max(1, lib.tpl_a2b_state_bytes(0, B, L, ctypes.byref(c))), dtype=torch.uint8, device=dev)
-    sy = torch.empty(max(1, lib.tpl_a2b_sync_bytes(0, B, L, ctypes.byref(c))), dtype=torch.uint8, device=dev)
+    sy = torch.full((max(1, lib.tpl_a2b_sync_bytes(0, B, L, ctypes.byref(c))),), 7, dtype=torch.uint8,
+                    device=dev)
     X = torch.full((B, 9 * L), float("nan"), device=dev)
     GA = torch.full((B, 3, L), float("nan"), device=dev)
     status = torch.zeros(1, dtype=torch.int32, device=dev)
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert lib.tpl_a2b_sync_clear(_p(sy), sy.numel(), stream) == 0  # garbage -> zero
     rc = lib.tpl_a2b_forward(0, _p(A), _p(X), _p(N), _p(st), _p(sy), _p(status), B, L,
                              ctypes.byref(c), stream)
     assert rc == 0, lib.tpl_a2b_strerror(rc)
-    rc = lib.tpl_a2b_backward(0, _p(GA), _p(G), _p(X), _p(N), _p(st), _p(status), B, L,
+    rc = lib.tpl_a2b_backward(0, _p(GA), _p(G), _p(A), _p(N), _p(st), _p(sy), _p(status), B, L,
                               ctypes.byref(c), stream)
     assert rc == 0, lib.tpl_a2b_strerror(rc)
     torch.cuda.synchronize()
     assert status.item() == 0
+    assert int(sy.count_nonzero()) == 0  # the look-back buffer is all zero again
     x = X.cpu().numpy().astype(np.float64)
     ga = GA.cpu().numpy().astype(np.float64)
     x64 = O.forward(a.astype(np.float64), n)
@@ -72,10 +75,10 @@ def test_plain_via_c_abi_and_tuner():
     ms = (ctypes.c_float * 3)()
     st = torch.empty(max(lib.tpl_a2b_state_bytes(0, B, L, ctypes.byref(cands[i])) for i in range(2)),
                      dtype=torch.uint8, device=dev)
-    sy = torch.empty(max(1, max(lib.tpl_a2b_sync_bytes(0, B, L, ctypes.byref(cands[i])) for i in range(2))),
+    sy = torch.zeros(max(1, max(lib.tpl_a2b_sync_bytes(0, B, L, ctypes.byref(cands[i])) for i in range(2))),
                      dtype=torch.uint8, device=dev)
     rc = lib.tpl_a2b_tune(0, _p(A), _p(N), _p(G), _p(X), _p(GA), _p(st), _p(sy), B, L, cands, 3, 3,
                           ctypes.byref(best), ms, stream)
     assert rc == 0
     assert ms[0] > 0 and ms[1] > 0 and ms[2] == -1.0
-    assert (best.waves, best.res_per_lane) in [(1, 1), (4, 1)]
+    assert (best.fwd_res_per_lane, best.bwd_res_per_lane) in [(1, 1), (4, 1)]

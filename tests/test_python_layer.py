@@ -13,8 +13,10 @@ def test_native_extension_is_the_in_tree_build():
     assert os.path.samefile(_native.EXT_PATH, _build.ext_path())
 
 
-def test_cpu_tensors_rejected():
-    with pytest.raises(RuntimeError, match="GPU only"):
+def test_cpu_tensors_go_to_the_host_path_float64_only():
+    x = Angles2Backbone()(torch.zeros(2, 3, 4, dtype=torch.float64), torch.tensor([4, 2], dtype=torch.int32))
+    assert x.shape == (2, 36) and x.dtype == torch.float64
+    with pytest.raises(RuntimeError, match="float64"):
         Angles2Backbone()(torch.zeros(2, 3, 4), torch.tensor([4, 2], dtype=torch.int32))
 
 
@@ -24,29 +26,37 @@ def test_bad_form():
 
 
 def test_candidates_cover_the_length():
-    for L in (1, 30, 64, 200, 700, 1000, 16384):
-        for dt, cap in ((torch.float32, 1024), (torch.float64, 512)):
-            c = _tuning.candidates(L, dt, cus=256, batch=1024)
-            assert c and all(w in (1, 2, 4) and r in (1, 2, 4, 8) for w, r, _ in c)
-            assert all(64 * w * r <= cap for w, r, _ in c)
-            # persistent grids only where one tile holds a chain
-            assert all(g in (0, 512, 768, 1024) for _, _, g in c)
-            assert all(g == 0 or 64 * w * r >= L for w, r, g in c)
-    assert (4, 4, 768) in _tuning.candidates(1000, torch.float32, cus=256, batch=1024)
-    assert all(g == 0 for _, _, g in _tuning.candidates(1000, torch.float32))
-    assert all(g == 0 for _, _, g in _tuning.candidates(1000, torch.float32, cus=256, batch=512))
+    for L in (1, 30, 64, 65, 200, 700, 1000, 16384):
+        for dt, (fmax, bmax) in ((torch.float32, (8, 4)), (torch.float64, (4, 2))):
+            c = _tuning.candidates(L, dt)
+            assert c and all(f in (1, 2, 4, 8) and b in (1, 2, 4) for f, b in c)
+            assert all(f <= fmax and b <= bmax for f, b in c)
+            cap = 64
+            while cap < L:
+                cap *= 2
+            assert all(64 * f <= cap and 64 * b <= cap for f, b in c)
+    assert (4, 2) in _tuning.candidates(1000, torch.float32)
+    assert _tuning.candidates(30, torch.float32) == [(1, 1)]
+
+
+def test_length_bucket():
+    b = _tuning.length_bucket(np.array([30, 40, 100, 300, 1000, 1000, 999, 5]), 1000)
+    assert len(b) == 6  # [0,64) [64,128) [128,256) [256,512) [512,1024) [1024,...)
+    assert b == (3, 1, 0, 1, 3, 0)
+    assert _tuning.length_bucket(np.full(64, 700), 1000) == (0, 0, 0, 0, 8, 0)
+    assert _tuning.length_bucket(np.array([], np.int32), 10) == ()
 
 
 def test_workspace_bytes_binding():
-    # float64, one 256-residue tile per chain both ways: nothing to carry
-    assert _native.ext.workspace_bytes(True, 512, 200, [4, 1, 0]) == (0, 0)
-    # float32: 2 bytes of residual per coordinate even with one tile
-    st, sy = _native.ext.workspace_bytes(False, 512, 200, [4, 1, 0])
-    assert st >= 512 * 9 * 200 * 2 and sy == 0
-    st, sy = _native.ext.workspace_bytes(False, 64, 16384, [4, 4, 0])
-    assert st > 64 * 9 * 16384 * 2 + 64 * 32 * 96 and sy > 64 * 16 * 24 * 8
+    st, sy = _native.ext.workspace_bytes(False, 512, 200, [4, 1])
+    assert st >= 512 * 4 * 96 and sy > 0
+    assert _native.ext.workspace_bytes(True, 512, 200, [4, 1]) == (st, sy)  # dtype-independent
+    st, sy = _native.ext.workspace_bytes(False, 64, 16384, [4, 2])
+    assert st >= 64 * 128 * 96 and sy > 64 * 64 * 24 * 8
     with pytest.raises(RuntimeError):
-        _native.ext.workspace_bytes(False, 4, 4, [3, 1, 0])
+        _native.ext.workspace_bytes(False, 4, 4, [3, 1])
+    with pytest.raises(RuntimeError):
+        _native.ext.workspace_bytes(False, 4, 4, [4, 1, 0])
 
 
 def test_synthetic_workloads_are_deterministic():

@@ -18,11 +18,11 @@ the error behaviour are the reference's:
 What differs, by design:
 
 * the forward is a parallel FP64 prefix scan over per-residue rigid transforms
-  and the backward an O(L) reverse-scan adjoint over the forward's output
-  coordinates (no ``A``/``dr_dangle`` buffers: the reference's ``dr_dangle``
-  is O(B*L^2), 110 GB at B=1024, L=1000); for float32 the forward also keeps
-  2 bytes per coordinate of FP64 rounding residual for the backward, and only
-  when the input requires grad;
+  and the backward an O(L) reverse-scan adjoint that recomputes the frames
+  from the input angles and one FP64 checkpoint frame per 64-256 residues
+  saved by the forward (no ``A``/``dr_dangle`` buffers: the reference's ``A``
+  is 192 bytes per residue and its ``dr_dangle`` O(B*L^2), 110 GB at B=1024,
+  L=1000); the checkpoints are kept only when the input requires grad;
 * gradients of psi[len-1] and omega[len-1] are exactly 0 (the reference reads
   frames past the chain there);
 * lengths outside [0, L_max] are clamped on the device and reported (the
@@ -30,11 +30,18 @@ What differs, by design:
 * the NaN / length checks read one status word written by the kernels (one
   host sync, like the reference's ``output.sum()``); they are skipped while a
   stream is being captured, and ``TPL_A2B_CHECKS=0`` turns them off;
-* CPU tensors are rejected: this build is the GPU path only.
+* the look-back words of the kernels live in one zero-initialised buffer per
+  (device, stream), whose tickets, counters and flags every call returns to
+  zero;
+* CPU tensors (float64, as the reference's CPU path requires) go to a host
+  library: the same FP64 recurrence per chain and the O(L) adjoint from the
+  forward's coordinates, chains spread over torch.get_num_threads() threads,
+  where the reference builds an O(L^2) derivative tensor on one thread.
 """
 from __future__ import annotations
 
 import os
+import threading
 
 import torch
 from torch.autograd import Function
@@ -47,6 +54,7 @@ from ..._native import ext
 _STATUS_NAN = 1
 _STATUS_LENGTH = 2
 _STATUS_TIMEOUT = 4
+_STATUS_SYNC = 8
 
 
 def _checks_enabled() -> bool:
@@ -57,13 +65,15 @@ def _status_word(device):
     return torch.zeros(1, dtype=torch.int32, device=device) if _checks_enabled() else None
 
 
-def _raise_on_status(status, where: str) -> None:
+def _raise_on_status(status, where: str, sync=None) -> None:
     if status is None:
         return
     bits = int(status.item())
-    if bits & _STATUS_TIMEOUT:
-        raise RuntimeError("Angles2BackboneFunction: %s look-back wait timed out "
-                           "(result invalid)" % where)
+    if bits & (_STATUS_TIMEOUT | _STATUS_SYNC):
+        if sync is not None:
+            ext.sync_clear(sync)  # look-back words may be left set
+        what = "look-back wait timed out" if bits & _STATUS_TIMEOUT else "look-back words were not clean"
+        raise RuntimeError("Angles2BackboneFunction: %s %s (result invalid)" % (where, what))
     if bits & _STATUS_LENGTH:
         raise ValueError("Angles2BackboneFunction: angles_length outside [0, input.size(2)] "
                          "(clamped on the device)")
@@ -80,8 +90,28 @@ def _empty_bytes(n, device):
     return torch.empty(int(n), dtype=torch.uint8, device=device)
 
 
+_sync_lock = threading.Lock()
+_sync_bufs: dict = {}
+
+
+def _sync_buffer(device, nbytes: int):
+    """The look-back buffer of (device, current stream), zeroed when made;
+    every call returns its tickets, counters and flags to zero.  Grown (a new
+    zeroed buffer) when too small.  Calls on one stream are ordered, so they
+    can share it."""
+    stream = torch.cuda.current_stream(device)
+    key = (device.index, stream.cuda_stream)
+    with _sync_lock:
+        buf = _sync_bufs.get(key)
+        if buf is None or buf.numel() < nbytes:
+            buf = torch.zeros(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+            _sync_bufs[key] = buf
+    return buf
+
+
 class Angles2BackboneGPUFunction(Function):
-    """Backbone angles -> backbone coordinates (scan forward, O(L) adjoint backward).
+    """Backbone angles -> backbone coordinates (chunk scan forward, O(L)
+    adjoint backward from checkpoints).
 
     Reference: TorchProteinLibrary/ReducedModel/Angles2Backbone/Angles2Backbone.py:8-52.
     """
@@ -96,31 +126,68 @@ class Angles2BackboneGPUFunction(Function):
         st_bytes, sy_bytes = ext.workspace_bytes(input_angles.dtype == torch.float64, B, L,
                                                  list(cfg))
         state = _empty_bytes(st_bytes, dev) if ctx.needs_input_grad[0] else None
-        sync = _empty_bytes(sy_bytes, dev)
+        sync = _sync_buffer(dev, sy_bytes)
         status = _status_word(dev)
         ext.Angles2BackboneGPU_forward(input_angles, coords, angles_length, state, sync, status,
                                        list(cfg))
-        _raise_on_status(status, "forward")
+        _raise_on_status(status, "forward", sync)
         if state is not None:
-            ctx.save_for_backward(coords, angles_length, state)
+            ctx.save_for_backward(input_angles, angles_length, state)
         ctx.cfg = cfg
         return coords
 
     @staticmethod
     @once_differentiable
     def backward(ctx, gradOutput):
-        coords, angles_length, state = ctx.saved_tensors
+        input_angles, angles_length, state = ctx.saved_tensors
         gradOutput = gradOutput.contiguous()
-        if gradOutput.dtype != coords.dtype:
-            gradOutput = gradOutput.to(coords.dtype)
-        B, L = coords.shape[0], coords.shape[1] // 9
-        dev = coords.device
-        gradInput = torch.empty(B, 3, L, dtype=coords.dtype, device=dev)
+        if gradOutput.dtype != input_angles.dtype:
+            gradOutput = gradOutput.to(input_angles.dtype)
+        B, _, L = input_angles.shape
+        dev = input_angles.device
+        gradInput = torch.empty(B, 3, L, dtype=input_angles.dtype, device=dev)
+        _, sy_bytes = ext.workspace_bytes(input_angles.dtype == torch.float64, B, L, list(ctx.cfg))
+        sync = _sync_buffer(dev, sy_bytes)
         status = _status_word(dev)
-        ext.Angles2BackboneGPU_backward(gradInput, gradOutput, coords, angles_length, state,
-                                        status, list(ctx.cfg))
-        _raise_on_status(status, "backward")
+        ext.Angles2BackboneGPU_backward(gradInput, gradOutput, input_angles, angles_length, state,
+                                        sync, status, list(ctx.cfg))
+        _raise_on_status(status, "backward", sync)
         return gradInput, None
+
+
+class Angles2BackboneCPUFunction(Function):
+    """Backbone angles -> backbone coordinates on the host, float64.
+
+    Reference: TorchProteinLibrary/ReducedModel/Angles2Backbone/Angles2Backbone.py:54-93
+    (same names, float64 only, same error strings).
+    """
+
+    @staticmethod
+    def forward(ctx, input, angles_length):
+        _check_rank(input)
+        B, _, L = input.shape
+        coords = torch.empty(B, 9 * L, dtype=torch.double)
+        bits = ext.Angles2BackboneCPU_forward(input, coords, angles_length, torch.get_num_threads())
+        if bits & _STATUS_LENGTH:
+            raise ValueError("Angles2BackboneFunction: angles_length outside [0, input.size(2)] "
+                             "(clamped)")
+        if bits & _STATUS_NAN:
+            raise Exception('Angles2BackboneFunction: output_coords_cpu forward Nan')
+        ctx.save_for_backward(coords, angles_length)
+        return coords
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, gradOutput_cpu):
+        coords, angles_length = ctx.saved_tensors
+        gradOutput_cpu = gradOutput_cpu.contiguous().to(torch.double)
+        B, L = coords.shape[0], coords.shape[1] // 9
+        gradInput_cpu = torch.empty(B, 3, L, dtype=torch.double)
+        bits = ext.Angles2BackboneCPU_backward(gradInput_cpu, gradOutput_cpu, coords, angles_length,
+                                               torch.get_num_threads())
+        if bits & _STATUS_NAN:
+            raise Exception('Angles2BackboneFunction: gradInput_cpu backward Nan')
+        return gradInput_cpu, None
 
 
 class Angles2BackbonePlainGPUFunction(Function):
@@ -158,8 +225,9 @@ class Angles2BackbonePlainGPUFunction(Function):
 class Angles2Backbone(Module):
     """``Angles2Backbone()(input [B,3,L], angles_length int32 [B]) -> [B, 9L]``.
 
-    Reference: TorchProteinLibrary/ReducedModel/Angles2Backbone/Angles2Backbone.py:96-104.
-    ``form="plain"`` selects the baseline kernels instead of the scan.
+    Reference: TorchProteinLibrary/ReducedModel/Angles2Backbone/Angles2Backbone.py:96-104:
+    GPU tensors go to the chunk kernels, CPU tensors to the host library.
+    ``form="plain"`` selects the GPU baseline kernels instead of the scan.
     """
 
     def __init__(self, form: str = "scan"):
@@ -170,8 +238,9 @@ class Angles2Backbone(Module):
 
     def forward(self, input, angles_length):
         if not input.is_cuda:
-            raise RuntimeError("Angles2Backbone: this build runs on the GPU only; "
-                               "input must be a GPU tensor")
+            if self.form == "plain":
+                raise RuntimeError("Angles2Backbone: the plain form runs on the GPU only")
+            return Angles2BackboneCPUFunction.apply(input, angles_length)
         if self.form == "plain":
             return Angles2BackbonePlainGPUFunction.apply(input, angles_length)
         return Angles2BackboneGPUFunction.apply(input, angles_length)

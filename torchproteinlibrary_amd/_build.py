@@ -3,8 +3,10 @@
 * ``libtpl_a2b.so``  -- Angles2Backbone: HIP kernels + the C ABI of
   include/tpl_a2b.h, built by ``hipcc --offload-arch=gfx950`` (cross-compiles
   without a GPU).
+* ``libtpl_a2b_cpu.so`` -- Angles2Backbone on the host, FP64
+  (include/tpl_a2b_cpu.h), built by g++;
 * ``_ReducedModel<EXT_SUFFIX>`` -- the pybind11 torch extension that unpacks
-  tensors and calls that C ABI; links libtpl_a2b.so with an $ORIGIN rpath.
+  tensors and calls those C ABIs; links both libraries with an $ORIGIN rpath.
 * ``libtpl_rmsd.so`` and ``_RMSD<EXT_SUFFIX>`` -- the same pair for
   Coords2RMSD (include/tpl_rmsd.h).
 * ``libtpl_xform.so``, ``libtpl_a2c.so``, ``libtpl_pdb.so`` and
@@ -33,6 +35,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
 LIB_NAME = "libtpl_a2b.so"
+CPU_LIB_NAME = "libtpl_a2b_cpu.so"
 EXT_NAME = "_ReducedModel"
 RMSD_LIB_NAME = "libtpl_rmsd.so"
 RMSD_EXT_NAME = "_RMSD"
@@ -43,7 +46,7 @@ PDB_LIB_NAME = "libtpl_pdb.so"
 
 # ([(library, its HIP source)], extension, [its C++ sources])
 PARTS = [
-    ([(LIB_NAME, "a2b.hip")], EXT_NAME, ["a2b_ext.cpp"]),
+    ([(LIB_NAME, "a2b.hip"), (CPU_LIB_NAME, "a2b_cpu.cpp")], EXT_NAME, ["a2b_ext.cpp"]),
     ([(RMSD_LIB_NAME, "rmsd.hip")], RMSD_EXT_NAME, ["rmsd_ext.cpp"]),
     ([(XFORM_LIB_NAME, "xform.hip"), (A2C_LIB_NAME, "a2c.hip"), (PDB_LIB_NAME, "pdb.cpp")],
      XFORM_EXT_NAME, ["xform_ext.cpp", "a2c_ext.cpp", "pdb_ext.cpp"]),
@@ -78,8 +81,8 @@ def build_lib(verbose=True, name: str = LIB_NAME, source: str = "a2b.hip") -> st
     out = lib_path(name)
     tmp = _atomic_out(out)
     if source.endswith(".cpp"):  # host-only library
-        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Werror", "-I", INCLUDE,
-               os.path.join(CSRC, source), "-o", tmp]
+        cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-pthread", "-Wall", "-Werror",
+               "-I", INCLUDE, os.path.join(CSRC, source), "-o", tmp]
     else:
         cmd = [
             os.path.join(ROCM, "bin", "hipcc"), f"--offload-arch={ARCH}", "-O3", "-std=c++17",

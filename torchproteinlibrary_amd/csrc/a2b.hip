@@ -9,25 +9,27 @@
 // owns the three transforms (phi_r, psi_r, omega_r) that place CA_r, C_r and
 // N_{r+1}, so every lane reads its own angles and writes its own gradients.
 //
-// Scan kernels (a2b_fwd_scan / a2b_bwd_scan):
-//   * a tile is 64*W*R consecutive residues of one chain; lane l owns R of
-//     them and moves its own angles, coordinates and gradients between HBM
-//     and registers (whole runs in 16-byte accesses where aligned); lanes past
-//     the chain write their zeros first;
-//   * forward: lane l composes its R residues serially in FP64 (the structured
-//     3x4 step below), the 64 lane aggregates are scanned with DPP (row_shr
-//     1/2/4/8, row_bcast 15/31), the W wave aggregates through LDS, and tiles
-//     of one chain through a decoupled look-back on flags in global memory
-//     (tickets taken in chain order, so every tile waits only on tiles already
-//     being processed); FP32 outputs also keep each coordinate's rounding
-//     residual as a scaled half;
-//   * backward: the O(L) adjoint from the forward's coordinates (rebuilt to
-//     FP64 grade with the residuals),
+// Chunk kernels (a2b_fwd_chunk / a2b_bwd_chunk), FP64 arithmetic:
+//   * a chunk is 64*R consecutive residues of one chain and one wave takes
+//     one chunk; lane l owns R of them.  The slots of all chains form one
+//     flat list, chunk-major inside XCD stripes; slots wholly past a chain
+//     write its padding zeros, the others take the chain's chunks through a
+//     per-chain ticket, so work spreads over every SIMD whatever the mix of
+//     lengths;
+//   * forward: lane l composes its R residues serially (the structured 3x4
+//     step below), the 64 lane aggregates are scanned with DPP (row_shr
+//     1/2/4/8, row_bcast 15/31), and chunks of one chain are chained by a
+//     wave-parallel decoupled look-back (each lane polls one earlier chunk;
+//     the ordered product of their frames is another wave scan).  The frame
+//     at every backward-chunk boundary is saved as a checkpoint;
+//   * backward: the O(L) adjoint
 //       dL/dtheta_j = u_j . (T_j - x_j x F_j),
-//       F_j = sum_{i>j} g_i,  T_j = sum_{i>j} x_i x g_i,
-//     with u_j the unit bond vector ending at atom j: a reverse (suffix) sum of
-//     the 6-vectors (x_i x g_i, g_i) over lanes, waves and tiles.  No frame is
-//     recomputed.
+//       F_j = sum_{i>=j} g_i,  T_j = sum_{i>=j} x_i x g_i,
+//     with u_j the unit bond vector ending at atom j.  Each chunk recomputes
+//     its frames from its angles and its checkpoint (no coordinates or
+//     per-atom state are read back), sums (x x g, g) over its atoms, takes
+//     the suffix sums over lanes (DPP) and later chunks (look-back), and
+//     walks its atoms once more for the gradients.
 // Plain kernels (a2b_fwd_plain / a2b_bwd_plain) restate the reference GPU
 // algorithm without the O(L^2) derivative tensor: they are the baseline.
 #include <hip/hip_runtime.h>
@@ -260,24 +262,6 @@ __device__ __forceinline__ uint32_t atomic_inc(uint32_t* p) {
 
 constexpr uint32_t kFlagAggregate = 1, kFlagInclusive = 2;
 
-// Polls a look-back flag until it is set.  The wait is bounded: after
-// kSpinLimit polls (seconds, far beyond any legitimate wait) it reports
-// TPL_A2B_STATUS_TIMEOUT and returns 0 so the caller skips that predecessor
-// and the kernel still drains.
-constexpr int kSpinLimit = 1 << 21;
-__device__ __forceinline__ uint32_t wait_flag(const uint32_t* p, uint32_t* status) {
-    for (int spins = 0; spins < kSpinLimit; ++spins) {
-        const uint32_t f = flag_load(p);
-        if (f != 0) {
-            after_flag();
-            return f;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-    set_status(status, TPL_A2B_STATUS_TIMEOUT);
-    return 0;
-}
-
 // ------------------------------------------------------------------ lane I/O
 // A lane moves a run of N consecutive elements of T between global memory
 // and registers.  Elements at or past `cnt` read as 0 and are not written.  A
@@ -413,103 +397,10 @@ __device__ __forceinline__ void lane_store_zero(T* p, int cnt) {
     lane_store<T, N>(p, cnt, z);
 }
 
-// ------------------------------------------------------- FP32 output residual
-// With FP32 coordinates the forward also keeps, per coordinate, the rounding
-// residual of its FP64 value, r = x - (double)(float)x, as the half
-// h = RTZ_f16(r * 2^24), in nine planes per chain: coordinate m (0..8) of
-// residue r at res[(b*9 + m)*Lp + r], Lp = L rounded up to 8, so that the R
-// halves of a lane in one plane are contiguous and consecutive lanes are
-// adjacent.  The backward rebuilds x = (double)x32 + h * 2^-24,
-// within |x| * 2^-36 of the FP64 value (|x| < 2^16: h stays finite), so the
-// bond directions it takes from coordinate differences are FP64-grade; from
-// the FP32 coordinates alone they would carry ~1e-5 relative error.
-constexpr float kResScale = 0x1.0p24f, kResInv = 0x1.0p-24f;
-
-// Scaled residual of x against its FP32 rounding f.
-__device__ __forceinline__ float res_of(double x, float f) {
-    return (float)(x - (double)f) * kResScale;
-}
-__device__ __forceinline__ uint32_t res_pack(float r0, float r1) {
-    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(r0, r1));
-}
-__device__ __forceinline__ double res_apply(float x32, _Float16 h) {
-    return fma((double)((float)h), (double)kResInv, (double)x32);
-}
-
-// Look-back payload per (chain, tile): aggregate and inclusive value.  The
-// forward uses 12 doubles each (frames), the backward 6 (sums).
-constexpr int kFwdPayload = 2 * kF;
-constexpr int kBwdPayload = 12;
-
-struct ScanArgs {
-    const void* angles;    // forward input
-    const void* coords;    // backward input: the forward's output
-    void* out;             // coords (forward) / grad_angles (backward)
-    const void* grad;      // grad_coords (backward)
-    const int32_t* lengths;
-    uint32_t* status;
-    _Float16* res;         // [B][9][res_lp] residuals of FP32 coordinates, or null
-    int res_lp;            // halves per residual plane: L rounded up to 8
-    int B, L;
-    int ntf, trf;          // forward tiles per chain, residues per forward tile
-    int ntb, trb;          // backward tiles per chain, residues per backward tile
-    int bwhole;            // backward: one workgroup walks all tiles of a chain
-    // forward look-back (ntf > 1), in the per-call scratch, zeroed before launch
-    uint32_t* fflags;      // [B][ntf]
-    uint32_t* ftickets;    // [B]
-    double* fpay;          // [B][ntf][kFwdPayload]
-    // backward look-back (ntb > 1), in the state: zeroed by the forward,
-    // returned to zero by the backward
-    uint32_t* bflags;      // [B][ntb]
-    uint32_t* btickets;    // [B]
-    uint32_t* bdone;       // [B]
-    double* bpay;          // [B][ntb][kBwdPayload]
-};
-
-// The chain length as used: lengths outside [0, L] are clamped and reported.
-__device__ __forceinline__ int clamp_length(const ScanArgs& a, int n, bool report) {
-    if (n < 0 || n > a.L) {
-        if (report) set_status(a.status, TPL_A2B_STATUS_LENGTH);
-        n = n < 0 ? 0 : a.L;
-    }
-    return n;
-}
-
-// Work item -> tile of chain b = item % B.  With several tiles per chain, the
-// tile comes from the chain's ticket counter, so a tile is taken only after
-// every earlier ticket of its chain was taken by a running workgroup
-// (forward: ascending tiles; backward: descending).
-__device__ __forceinline__ void item_tile(int b, int nt, uint32_t* tickets, bool descending,
-                                          int* s_tile, int* t_out) {
-    int t = 0;
-    if (nt > 1) {
-        if (threadIdx.x == 0) *s_tile = (int)atomic_inc(tickets + b);
-        __syncthreads();
-        t = __builtin_amdgcn_readfirstlane(*s_tile);
-        if (descending) t = nt - 1 - t;
-    }
-    *t_out = t;
-}
-
-// E <- tile_pfx * wave_aggr[0] * ... * wave_aggr[wave-1] * E, built right to
-// left with in-place left products (E enters as the lane's exclusive prefix
-// within its wave).
-__device__ __forceinline__ void lane_prefix(const double (*s_tot)[kF], int wave,
-                                            const double* tile_pfx, double E[kF]) {
-    for (int w = wave - 1; w >= 0; --w) frame_lmul(s_tot[w], E);
-    if (tile_pfx) frame_lmul(tile_pfx, E);
-}
-
 __device__ __forceinline__ void affine(const double E[kF], double p0, double p1, double p2,
                                        double x[3]) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) x[i] = E[i] * p0 + E[3 + i] * p1 + E[6 + i] * p2 + E[9 + i];
-}
-
-__device__ __forceinline__ void cross(const double a[3], const double b[3], double c[3]) {
-    c[0] = a[1] * b[2] - a[2] * b[1];
-    c[1] = a[2] * b[0] - a[0] * b[2];
-    c[2] = a[0] * b[1] - a[1] * b[0];
 }
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -564,533 +455,583 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// ------------------------------------------------------------ forward scan
-// One tile = TR = 64*W*R residues of chain b; lane l owns residues
-// rl = r0 + l*R .. rl+R-1, i.e. chain atoms 3rl .. 3rl+3R-1 and output
-// floats 9rl .. 9rl+9R-1.  Steps:
-//   1. the lanes load their R angles per row straight into registers; waves
-//      wholly past the chain skip to step 5;
-//   2. pass 1: per lane, FP64 sincos and the serial product of its R residues;
-//      local positions of the lane's atoms 1 .. 3R-1 go to the lane's LDS slot;
-//   3. DPP scan over the wave, LDS over the waves, look-back over tiles;
-//   4. pass 2: atom 0 is the translation of the lane's global prefix frame E,
-//      atom a >= 1 is E p_a, written over p_a in LDS; FP32 outputs also write
-//      their residual planes;
-//   5. each wave stores its 9*64*R-float output span with 16-byte stores from
-//      LDS, zeros past the chain.
-// LB: chains span several tiles (look-back over tiles); AHEAD: one tile per
-// chain on a grid smaller than the batch (load the next chain ahead).  The
-// launch picks the variant, so each carries only the state it needs.
-template <typename T, int R, int MAXT, bool LB, bool AHEAD>
-__global__ void __launch_bounds__(MAXT) a2b_fwd_scan(ScanArgs a) {
-    extern __shared__ __align__(16) unsigned char s_dyn[];
-    constexpr int E9 = 9 * R;  // output floats per lane = the lane's LDS stride
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nw = blockDim.x >> 6;
-    const int TR = blockDim.x * R;
-    T* s_x = reinterpret_cast<T*>(s_dyn);
-    T* s_l = s_x + tid * E9;              // this lane's slot
-    const T* s_w = s_x + wave * 64 * E9;  // this wave's staging, in output order
-    __shared__ double s_tab[128];
-    __shared__ double s_tot[4][kF];
-    __shared__ double s_pfx[kF];
-    __shared__ double s_agg[kF];
-    __shared__ int s_tile;
-    const T* angles = static_cast<const T*>(a.angles);
-    T* coords = static_cast<T*>(a.out);
-    const int L = a.L;
-    for (int i = tid; i < 128; i += blockDim.x) s_tab[i] = kTabCS[i];
 
-    // With one tile per chain the items are independent, and a workgroup that
-    // walks several of them (a grid smaller than B) loads the next chain's
-    // length and angles while it works on the current one.  With several
-    // tiles per chain the tile comes from the chain's ticket at the start of
-    // the item, so nothing is loaded ahead.
-    const int total = a.B * a.ntf;
-    constexpr bool ahead = AHEAD && !LB;
-    T ang[3][R];  // this lane's angles of the current item (rows phi, psi, omega)
-    int n_ahead = 0;
-    auto load_ahead = [&](int it, T (&dst)[3][R]) {
-        const int nb = clamp_length(a, a.lengths[it % a.B], false);
-        const int c = min(max(nb - tid * R, 0), R);
-        if (wave * 64 * R < nb)
-            lane_load_rows<T, R>(angles + (size_t)(it % a.B) * 3 * L + tid * R, L, c, L - tid * R, dst);
-        return nb;
-    };
-    if (ahead && (int)blockIdx.x < total) n_ahead = load_ahead(blockIdx.x, ang);
+// ------------------------------------------------------------ chunk kernels
+// Work decomposition.  A chunk is C = 64*R consecutive residues of one chain
+// (lane l owns residues r0 + l*R .. r0 + l*R + R-1, i.e. chain atoms
+// 3(r0 + lR) .. 3(r0 + lR) + 3R - 1).  One workgroup of one wave takes one
+// slot (b, s), s < nc = ceil(L / C).  Slots are numbered so that
+//   * workgroup w runs on XCD w mod 8 (the dispatcher's round-robin) and
+//     takes the chains b = x, x + 8, x + 16, ... of stripe x = w mod 8: all
+//     slots of a chain share one XCD, and with it one L2 for the chain's
+//     look-back words;
+//   * inside a stripe, slot s of every chain comes before slot s + 1 of any
+//     chain (chunk-major), so the first waves are the ones with work and
+//     each XCD sees the same mix of lengths.
+// With n residues a chain has nreal = ceil(n / C) chunks.  Slots s >= nreal
+// only write the zeros of their span of the outputs.  The nreal slots below
+// take the chain's chunks from a per-chain ticket counter, ascending in the
+// forward and descending in the backward, so a chunk waits only on chunks
+// whose waves are already running, whatever order workgroups start in.  A
+// chain of one chunk takes no ticket and publishes nothing.
+//
+// Look-back words live in the caller's sync buffer and clean up after
+// themselves: each wave that took a ticket counts itself done after its last
+// flag access, and the last one of its chain returns the chain's ticket,
+// done counter and flags to zero.  The buffer is zeroed once when it is
+// allocated (tpl_a2b_sync_clear) and stays zero between calls.
 
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
-        __syncthreads();  // s_tab written; LDS of the previous tile no longer read
-        const int b = item % a.B;
-        int t = 0, n;
-        if constexpr (ahead) {
-            n = n_ahead;
-            if (tid == 0) clamp_length(a, a.lengths[b], true);  // reports a bad length
-        } else {
-            const int n_raw = a.lengths[b];  // issued before the ticket round trip
-            if constexpr (LB) item_tile(b, a.ntf, a.ftickets, false, &s_tile, &t);
-            n = clamp_length(a, n_raw, t == 0 && tid == 0);
-        }
-        const int r0 = t * TR;
-        const int rl = r0 + tid * R;
-        const int cnt = min(max(n - rl, 0), R);  // residues of the chain in this lane
+constexpr int kFwdPay = 2 * kF;  // per chunk: aggregate frame, inclusive frame
+constexpr int kBwdPay = 12;      // per chunk: aggregate (T, F), inclusive (T, F)
 
-        // Zero the backward's look-back words of the backward tiles inside
-        // this forward tile (the backward returns them to zero after use).
-        if (a.bflags) {
-            const int per = TR / a.trb;
-            for (int j = tid; j < per; j += blockDim.x) {
-                const int tb = t * per + j;
-                if (tb < a.ntb) word_store(a.bflags + (size_t)b * a.ntb + tb, 0u);
-            }
-            if (t == 0 && tid == 0) {
-                word_store(a.btickets + b, 0u);
-                word_store(a.bdone + b, 0u);
-            }
-        }
-
-        const int rw0 = r0 + wave * 64 * R;  // first residue of the wave
-        const bool wave_active = rw0 < n;    // uniform per wave
-        const int next = item + (int)gridDim.x;
-        T ang_next[3][R];
-        double E[kF];
-        if (wave_active) {
-            if (!ahead) lane_load_rows<T, R>(angles + (size_t)b * 3 * L + rl, L, cnt, L - rl, ang);
-            frame_identity(E);
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    double sn, cs;
-                    sincos_tab((double)ang[k][q], s_tab, &sn, &cs);
-                    if (k == 0) frame_step<0>(E, cs, sn);
-                    if (k == 1) frame_step<1>(E, cs, sn);
-                    if (k == 2) frame_step<2>(E, cs, sn);
-                    const int at = 3 * q + k + 1;  // lane atom placed by this step
-                    if (at < 3 * R) {
-                        s_l[3 * at] = (T)E[9];
-                        s_l[3 * at + 1] = (T)E[10];
-                        s_l[3 * at + 2] = (T)E[11];
-                    }
-                }
-            }
-        }
-        // The next item's inputs load during this item's scan and stores.
-        if (ahead && next < total) n_ahead = load_ahead(next, ang_next);
-        if (wave_active) {
-            wave_scan_frames(E);
-            if (lane == 63) {
-#pragma unroll
-                for (int e = 0; e < kF; ++e) s_tot[wave][e] = E[e];
-            }
-            wave_shift_frames(E);
-        } else if (lane == 63) {
-#pragma unroll
-            for (int e = 0; e < kF; ++e) s_tot[wave][e] = ident(e);
-        }
-        __syncthreads();
-        if constexpr (LB) {
-            if (tid == 0) {
-                const size_t s0 = (size_t)b * a.ntf;
-                uint32_t* fl = a.fflags + s0;
-                double* pay = a.fpay + s0 * kFwdPayload;
-                double* my = pay + (size_t)t * kFwdPayload;
-                double acc[kF];
-                {   // tile aggregate = product of the wave aggregates
-#pragma unroll
-                    for (int e = 0; e < kF; ++e) acc[e] = s_tot[nw - 1][e];
-                    for (int w = nw - 2; w >= 0; --w) frame_lmul(s_tot[w], acc);
-#pragma unroll
-                    for (int e = 0; e < kF; ++e) {
-                        s_agg[e] = acc[e];
-                        payload_store(my + (t == 0 ? kF : 0) + e, acc[e]);
-                    }
-                }
-                flag_store(fl + t, t == 0 ? kFlagInclusive : kFlagAggregate);
-                frame_identity(acc);  // becomes this tile's exclusive prefix
-                if (t > 0) {
-                    for (int j = t - 1; j >= 0; --j) {
-                        const uint32_t fj = wait_flag(fl + j, a.status);
-                        if (fj == 0) continue;
-                        const double* pj = pay + (size_t)j * kFwdPayload;
-                        const bool incl = (fj == kFlagInclusive);
-                        double v[kF];
-#pragma unroll
-                        for (int e = 0; e < kF; ++e) v[e] = payload_load(pj + (incl ? kF : 0) + e);
-                        frame_lmul(v, acc);
-                        if (incl) break;
-                    }
-                    double incl_v[kF];  // inclusive = prefix * aggregate
-                    frame_rmul_to(acc, s_agg, incl_v);
-#pragma unroll
-                    for (int e = 0; e < kF; ++e) payload_store(my + kF + e, incl_v[e]);
-                    flag_store(fl + t, kFlagInclusive);
-                }
-#pragma unroll
-                for (int e = 0; e < kF; ++e) s_pfx[e] = acc[e];
-            }
-            __syncthreads();
-        }
-        T* x_b = coords + (size_t)b * 9 * L;
-        const int g0 = 9 * rw0;  // the wave's output span [g0, g0 + 9*64*R) within [0, 9L)
-        const int wcount = min(9 * 64 * R, 9 * L - g0);
-        if (wave_active) {
-            if (cnt > 0) {
-                lane_prefix(s_tot, wave, LB ? s_pfx : nullptr, E);
-                // Pass 2, in place in the lane's slot.  Residuals go to plane
-                // m at residue rl + q as soon as a pair (q even, q + 1) is
-                // complete (one half when R = 1): the lane's R halves per
-                // plane are a whole run (rl + R <= Lp).
-                [[maybe_unused]] float rpend[9];
-                [[maybe_unused]] _Float16* rp = a.res + (size_t)b * 9 * a.res_lp + rl;
-                const bool has_res = a.res != nullptr;  // uniform
-                // Sum of the lane's coordinates: NaN or inf anywhere shows.
-                // Coordinates past the chain come from zero angles: finite.
-                float chk = 0.f;
-#pragma unroll
-                for (int at = 0; at < 3 * R; ++at) {
-                    double x[3];
-                    if (at == 0) {
-                        x[0] = E[9];
-                        x[1] = E[10];
-                        x[2] = E[11];
-                    } else {
-                        affine(E, (double)s_l[3 * at], (double)s_l[3 * at + 1], (double)s_l[3 * at + 2], x);
-                    }
-                    const int q = at / 3;
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const T o = (T)x[c];
-                        s_l[3 * at + c] = o;
-                        chk += (float)o;
-                        if constexpr (sizeof(T) == 4) {
-                            const int m = 3 * (at % 3) + c;  // plane
-                            const float r = res_of(x[c], o);
-                            if (has_res) {
-                                _Float16* d = rp + (size_t)m * a.res_lp + q;
-                                if constexpr (R == 1) {
-                                    *reinterpret_cast<uint16_t*>(d) = (uint16_t)(res_pack(r, 0.f) & 0xffffu);
-                                } else if (q % 2 == 0) {
-                                    rpend[m] = r;
-                                } else {
-                                    *reinterpret_cast<uint32_t*>(d - 1) = res_pack(rpend[m], r);
-                                }
-                            }
-                        }
-                    }
-                }
-                if (!__builtin_isfinite(chk)) set_status(a.status, TPL_A2B_STATUS_NAN);
-            }
-            wave_lds_sync();
-            wave_store_run<T>(x_b + g0, wcount, min(max(9 * n - g0, 0), wcount), s_w, lane);
-        } else if (wcount > 0) {
-            wave_store_run<T>(x_b + g0, wcount, 0, s_w, lane);
-        }
-        if (ahead) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-#pragma unroll
-                for (int q = 0; q < R; ++q) ang[k][q] = ang_next[k][q];
-        }
-    }
-}
-
-// The lane's 9R coordinates as loaded: values (xr) and, for FP32, their
-// residual halves (hr[9q + m] from plane m at residue rl + q; zero past the
-// chain).
-template <typename T, int R>
-struct LaneCoords {
-    T xr[9 * R];
-    _Float16 hr[sizeof(T) == 4 ? 9 * R : 1];
-
-    // x, g: the lane's first coordinate and coordinate gradient (into gr);
-    // rp: plane 0 of the chain; rl: the lane's first residue; room: residues
-    // of the row from rl.
-    __device__ __forceinline__ void load(const T* x, const T* g, T (&gr)[9 * R], const _Float16* rp,
-                                         int lp, int rl, int cnt, int room) {
-        lane_load2<T, 9 * R>(x, g, 9 * cnt, 9 * room, xr, gr);
-        if constexpr (sizeof(T) == 4) {
-            // One 2R-byte load per plane, no branch: the run at residue rl is
-            // 2R-aligned and inside the plane row for every lane of the row
-            // (rl + R <= Lp); lanes past the row read the row's last run.
-            // Halves past the chain are zeroed.
-            const _Float16* p0 = rp + min(rl, lp - R);
-#pragma unroll
-            for (int m = 0; m < 9; ++m) {
-                using V = typename Unit<2 * R>::type;
-                const V u = *reinterpret_cast<const V*>(p0 + (size_t)m * lp);
-                _Float16 h[R];
-                __builtin_memcpy(h, &u, sizeof(h));
-#pragma unroll
-                for (int q = 0; q < R; ++q) hr[9 * q + m] = q < cnt ? h[q] : (_Float16)0;
-            }
-        }
-    }
-    // FP64 coordinates of the lane's atom `at` (0 .. 3R-1).
-    __device__ __forceinline__ void atom(int at, double out[3]) const {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            if constexpr (sizeof(T) == 4)
-                out[c] = res_apply(xr[3 * at + c], hr[3 * at + c]);
-            else
-                out[c] = (double)xr[3 * at + c];
-        }
-    }
+struct ChunkArgs {
+    const void* angles;      // [B][3][L]
+    void* coords;            // forward output [B][9L]
+    const void* grad;        // backward input [B][9L]
+    void* grad_angles;       // backward output [B][3][L]
+    const int32_t* lengths;  // [B]
+    uint32_t* status;
+    double* ckpt;            // [B][nk][kF]: entry k is the frame of atom 3kK (N of
+                             // residue kK), written for 0 < kK < n; null: none
+    uint32_t* tickets;       // [B]
+    uint32_t* done;          // [B]
+    uint32_t* flags;         // [B][nc]
+    double* pay;             // [B][nc][kFwdPay] or [B][nc][kBwdPay]
+    int B, L;
+    int nc;                  // chunk slots per chain
+    int nk, K;               // checkpoints per chain, residues between checkpoints
+    int bx;                  // chains per stripe: ceil(B / 8)
 };
 
-// ----------------------------------------------------------- backward scan
-// The adjoint from the forward's coordinates.  Chain atom j >= 1 (atoms
-// N_0, CA_0, C_0, N_1, ...) is placed by the transform whose angle theta_j is
-// phi_r, psi_r, omega_r for j = 3r+1, 3r+2, 3r+3.  theta_j turns every later
-// atom about the bond axis u_j = (x_j - x_{j-1}) / d_j through x_j, so
-//   dL/dtheta_j = u_j . sum_{i>j} (x_i - x_j) x g_i = u_j . (T_j - x_j x F_j),
-//   T_j = sum_{i>j} x_i x g_i,  F_j = sum_{i>j} g_i,
-// a suffix sum of the 6-vectors (x_i x g_i, g_i).  Tiles of TR = 64*W*R
-// residues are taken from the last tile of each chain to the first; lane l
-// sums its own atoms 3rl .. 3rl+3R-1 and writes the gradients of its residues
-// (atoms 3rl+1 .. 3rl+3R, the last one being the next lane's first).  Steps:
-//   1. lanes wholly past the chain write their zero gradients at once; the
-//      others load their coordinates and coordinate gradients (and, for FP32,
-//      the residuals) into registers, zero past the chain, and keep them there
-//      until pass B;
-//   2. per lane, W = sum over its atoms of (x x g, g);
-//   3. suffix sums of W over lanes (DPP), waves (LDS) and tiles (reverse
-//      look-back) give C = the sum over all later atoms;
-//   4. per atom, from the lane's last down: the suffix (T_j, F_j) and the
-//      gradient; psi and omega of the last residue are 0.
-template <typename T, int R, int MAXT>
-__global__ void __launch_bounds__(MAXT) a2b_bwd_scan(ScanArgs a) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nw = blockDim.x >> 6;
-    const int TR = blockDim.x * R;
-    __shared__ double s_wsum[4][6];
-    __shared__ double s_carry[6];
-    __shared__ int s_tile;
-    const T* xin = static_cast<const T*>(a.coords);
-    const T* gin = static_cast<const T*>(a.grad);
-    T* gang = static_cast<T*>(a.out);
-    const int L = a.L;
-    constexpr double kInvD[3] = {1.0 / 1.460, 1.0 / 1.525, 1.0 / 1.330};
+// The chain length as used: lengths outside [0, L] are clamped and reported.
+__device__ __forceinline__ int clamp_length(const ChunkArgs& a, int n, bool report) {
+    if (n < 0 || n > a.L) {
+        if (report) set_status(a.status, TPL_A2B_STATUS_LENGTH);
+        n = n < 0 ? 0 : a.L;
+    }
+    return n;
+}
 
-    // Work items: with bwhole, a chain (its tiles walked from the last to the
-    // first, the sum over later tiles carried in registers); else a tile,
-    // chained to the later tiles of its chain by the reverse look-back.
-    const bool whole = a.bwhole != 0;
-    const int nitems = whole ? a.B : a.B * a.ntb;
-    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
-        __syncthreads();
-        const int b = item % a.B;
-        const int n_raw = a.lengths[b];  // issued before the ticket round trip
-        int t0 = 0;
-        if (!whole) item_tile(b, a.ntb, a.btickets, true, &s_tile, &t0);
-        const int n = clamp_length(a, n_raw, (whole || t0 == 0) && tid == 0);
-        double carry[6] = {0, 0, 0, 0, 0, 0};  // whole: sum over the later tiles
-        for (int step = 0; step < (whole ? a.ntb : 1); ++step) {
-            const int t = whole ? a.ntb - 1 - step : t0;
-            const int r0 = t * TR;
-            const int rl = r0 + tid * R;
-            const int cnt = min(max(n - rl, 0), R);
-            const int crow = min(max(L - rl, 0), R);
-            T* ga_l = gang + (size_t)b * 3 * L + rl;
-            if (cnt == 0 && crow > 0) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) lane_store_zero<T, R>(ga_l + (size_t)k * L, crow);
-            }
-            // A walked tile wholly past the chain only writes its zeros
-            // (uniform; it adds nothing to the carried sum).
-            if (whole && r0 >= n) continue;
-            if (whole) __syncthreads();  // s_wsum of the previous tile no longer read
-            // Issue priority by the tiles still to walk after this one: the
-            // longest chains are the kernel's critical path, and the waves of
-            // shorter ones have slack (measured 21.3 -> 20.0 us at B=1024,
-            // lengths 30-1000).
-            switch (min(whole ? t : 0, 3)) {
-                case 3: __builtin_amdgcn_s_setprio(3); break;
-                case 2: __builtin_amdgcn_s_setprio(2); break;
-                case 1: __builtin_amdgcn_s_setprio(1); break;
-                default: __builtin_amdgcn_s_setprio(0); break;
-            }
-            const bool wave_active = r0 + wave * 64 * R < n;  // uniform per wave
+// Slot of this workgroup; false for the padding slots of the last stripes.
+__device__ __forceinline__ bool chunk_slot(const ChunkArgs& a, int* b, int* s) {
+    const int w = (int)blockIdx.x;
+    const int j = w >> 3;
+    const int sl = j / a.bx;
+    *s = sl;
+    *b = (w & 7) + 8 * (j - sl * a.bx);
+    return *b < a.B;
+}
 
-            // Pass A: W = sum over the lane's atoms 0 .. 3R-1 (chain atoms
-            // 3rl .. 3rl+3R-1) of (x x g, g); zero past the chain.
-            const size_t o = (size_t)b * 9 * L + (size_t)9 * rl;
-            const size_t ro = (size_t)b * 9 * a.res_lp + (size_t)rl;  // plane 0, residue rl
-            double W[6] = {0, 0, 0, 0, 0, 0};
-            LaneCoords<T, R> xc;
-            T gr[9 * R];
-            if (wave_active) {
-                xc.load(xin + o, gin + o, gr, a.res ? a.res + (size_t)b * 9 * a.res_lp : nullptr,
-                        a.res_lp, rl, cnt, L - rl);
-#pragma unroll
-                for (int at = 0; at < 3 * R; ++at) {
-                    double x[3], xg[3];
-                    xc.atom(at, x);
-                    const double g[3] = {(double)gr[3 * at], (double)gr[3 * at + 1], (double)gr[3 * at + 2]};
-                    cross(x, g, xg);
-                    W[0] += xg[0]; W[1] += xg[1]; W[2] += xg[2];
-                    W[3] += g[0]; W[4] += g[1]; W[5] += g[2];
-                }
-            }
-            // C = sum over later lanes, waves and tiles.
-            double C[6];
-#pragma unroll
-            for (int e = 0; e < 6; ++e) C[e] = W[e];
-            if (wave_active) {
-                wave_scan_sum<6>(C);  // inclusive prefix over lanes
-#pragma unroll
-                for (int e = 0; e < 6; ++e) {
-                    const double tot = read_lane(C[e], 63);
-                    if (lane == 0) s_wsum[wave][e] = tot;
-                    C[e] = tot - C[e];
-                }
-            } else if (lane == 0) {
-#pragma unroll
-                for (int e = 0; e < 6; ++e) s_wsum[wave][e] = 0.0;
-            }
-            __syncthreads();
-            for (int w = wave + 1; w < nw; ++w) {
-#pragma unroll
-                for (int e = 0; e < 6; ++e) C[e] += s_wsum[w][e];
-            }
-            if (whole) {
-#pragma unroll
-                for (int e = 0; e < 6; ++e) C[e] += carry[e];
-                for (int w = 0; w < nw; ++w) {
-#pragma unroll
-                    for (int e = 0; e < 6; ++e) carry[e] += s_wsum[w][e];
-                }
-            } else if (a.ntb > 1) {  // then bflags, bpay, btickets, bdone are set (state)
-                if (tid == 0) {
-                    double tot[6] = {0, 0, 0, 0, 0, 0}, carry[6] = {0, 0, 0, 0, 0, 0};
-                    for (int w = 0; w < nw; ++w) {
-#pragma unroll
-                        for (int e = 0; e < 6; ++e) tot[e] += s_wsum[w][e];
-                    }
-                    const int ntv = (n + TR - 1) / TR;  // tiles holding residues
-                    if (t < ntv) {
-                        const size_t s0 = (size_t)b * a.ntb;
-                        uint32_t* fl = a.bflags + s0;
-                        double* pay = a.bpay + s0 * kBwdPayload;
-                        double* my = pay + (size_t)t * kBwdPayload;
-                        if (t == ntv - 1) {
-#pragma unroll
-                            for (int e = 0; e < 6; ++e) payload_store(my + 6 + e, tot[e]);
-                            flag_store(fl + t, kFlagInclusive);
-                        } else {
-#pragma unroll
-                            for (int e = 0; e < 6; ++e) payload_store(my + e, tot[e]);
-                            flag_store(fl + t, kFlagAggregate);
-                            for (int j = t + 1; j < ntv; ++j) {
-                                const uint32_t fj = wait_flag(fl + j, a.status);
-                                if (fj == 0) continue;
-                                const double* pj = pay + (size_t)j * kBwdPayload;
-                                const bool incl = (fj == kFlagInclusive);
-#pragma unroll
-                                for (int e = 0; e < 6; ++e) carry[e] += payload_load(pj + (incl ? 6 : 0) + e);
-                                if (incl) break;
-                            }
-#pragma unroll
-                            for (int e = 0; e < 6; ++e) payload_store(my + 6 + e, carry[e] + tot[e]);
-                            flag_store(fl + t, kFlagInclusive);
-                        }
-                    }
-#pragma unroll
-                    for (int e = 0; e < 6; ++e) s_carry[e] = carry[e];
-                }
-                __syncthreads();
-#pragma unroll
-                for (int e = 0; e < 6; ++e) C[e] += s_carry[e];
-            }
-            if (wave_active && cnt > 0) {
-                // Pass B, on the lane's atoms still in registers from pass A,
-                // plus chain atom 3(rl+R), the next lane's first, from memory.
-                const bool has_next = rl + R < n;
-                double xn[3], gn[3];  // chain atom 3(rl+R)
-                // Without a next atom the loads read the lane's own first atom
-                // (valid here, cnt > 0) and are discarded: no branch.
-                const size_t on = has_next ? o + 9 * R : o;
-                [[maybe_unused]] const size_t rn = has_next ? ro + R : ro;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const T xv = xin[on + c];
-                    const T gv = gin[on + c];
-                    double xd;
-                    if constexpr (sizeof(T) == 4)
-                        xd = res_apply((float)xv, a.res[rn + (size_t)c * a.res_lp]);
-                    else
-                        xd = (double)xv;
-                    xn[c] = has_next ? xd : 0.0;
-                    gn[c] = has_next ? (double)gv : 0.0;
-                }
-                // From the lane's atom 3R down: Xc = atom j, Xp = atom j-1, Xa =
-                // atom j+1, S = sum over atoms after j of (x x g, g).  Atom 3R
-                // belongs to the next lane, whose sum is inside C.
-                double S[6], Xc[3], Xp[3], Xa[3];
-                {
-                    double xg[3];
-                    cross(xn, gn, xg);
-                    S[0] = C[0] - xg[0]; S[1] = C[1] - xg[1]; S[2] = C[2] - xg[2];
-                    S[3] = C[3] - gn[0]; S[4] = C[4] - gn[1]; S[5] = C[5] - gn[2];
-                }
-#pragma unroll
-                for (int c = 0; c < 3; ++c) Xc[c] = xn[c];
-                xc.atom(3 * R - 1, Xp);
-                T out[3][R];
-#pragma unroll
-                for (int j = 3 * R; j >= 1; --j) {
-                    if (j < 3 * R) {
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) {
-                            Xa[c] = Xc[c];
-                            Xc[c] = Xp[c];
-                        }
-                        xc.atom(j - 1, Xp);
-                    }
-                    if (j == 3 * R - 1) {
-#pragma unroll
-                        for (int e = 0; e < 6; ++e) S[e] = C[e];
-                    } else if (j < 3 * R - 1) {
-                        const double g[3] = {(double)gr[3 * j + 3], (double)gr[3 * j + 4], (double)gr[3 * j + 5]};
-                        double xg[3];
-                        cross(Xa, g, xg);
-                        S[0] += xg[0]; S[1] += xg[1]; S[2] += xg[2];
-                        S[3] += g[0]; S[4] += g[1]; S[5] += g[2];
-                    }
-                    const int k = (j - 1) % 3, q = (j - 1) / 3;
-                    double u[3], xf[3];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) u[c] = (Xc[c] - Xp[c]) * kInvD[k];
-                    cross(Xc, S + 3, xf);
-                    const double v = u[0] * (S[0] - xf[0]) + u[1] * (S[1] - xf[1]) + u[2] * (S[2] - xf[2]);
-                    const int r = rl + q;
-                    const bool zero = (r >= n) || (r == n - 1 && k > 0);
-                    out[k][q] = zero ? (T)0 : (T)v;
-                }
-                bool bad = false;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-#pragma unroll
-                    for (int q = 0; q < R; ++q) bad |= (out[k][q] != out[k][q]);
-                    lane_store<T, R>(ga_l + (size_t)k * L, crow, out[k]);
-                }
-                if (bad) set_status(a.status, TPL_A2B_STATUS_NAN);
-            }
+__device__ __forceinline__ int take_ticket(uint32_t* t) {
+    uint32_t v = 0;
+    if (threadIdx.x == 0) v = atomic_inc(t);
+    return (int)__builtin_amdgcn_readfirstlane(v);
+}
 
-        }  // tiles of a whole chain
-        // Last tile of the chain to finish returns its look-back words to 0.
-        if (!whole && a.ntb > 1 && tid == 0) {
-            if (atomic_inc(a.bdone + b) == (uint32_t)(a.ntb - 1)) {
-                for (int j = 0; j < a.ntb; ++j) word_store(a.bflags + (size_t)b * a.ntb + j, 0u);
-                word_store(a.btickets + b, 0u);
-                word_store(a.bdone + b, 0u);
-            }
+// Called once by every wave that took a ticket of chain b, after its last
+// access to the chain's flags and payload.  The last of the nreal waves
+// returns the chain's words and payload (PAY doubles per chunk) to zero:
+// every other wave has finished with them, and the next call, of whatever
+// shape, finds the buffer as it was before this one (all zero).
+template <int PAY>
+__device__ __forceinline__ void chain_done(const ChunkArgs& a, int b, int nreal) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);  // this wave's flag stores have landed
+    uint32_t last = 0;
+    if (threadIdx.x == 0) last = atomic_inc(a.done + b) == (uint32_t)(nreal - 1) ? 1u : 0u;
+    if (__builtin_amdgcn_readfirstlane(last)) {
+        uint32_t* fl = a.flags + (size_t)b * a.nc;
+        double* pay = a.pay + (size_t)b * a.nc * PAY;
+        for (int j = threadIdx.x; j < nreal; j += 64) word_store(fl + j, 0u);
+        for (int j = threadIdx.x; j < nreal * PAY; j += 64) payload_store(pay + j, 0.0);
+        if (threadIdx.x == 0) {
+            word_store(a.tickets + b, 0u);
+            word_store(a.done + b, 0u);
         }
     }
 }
 
-__global__ void a2b_zero_words(uint32_t* p, int n) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+// The ticket of a wave of chain b (nreal chunks), or -1 if it is outside
+// [0, nreal): the sync buffer was not clean (TPL_A2B_STATUS_SYNC reported;
+// the wave then touches nothing of the chain).
+__device__ __forceinline__ int chain_ticket(const ChunkArgs& a, int b, int nreal) {
+    const int t = take_ticket(a.tickets + b);
+    if (t < 0 || t >= nreal) {
+        if (threadIdx.x == 0) set_status(a.status, TPL_A2B_STATUS_SYNC);
+        return -1;
+    }
+    return t;
+}
+
+// Lane 0 publishes the uniform values v[0..N) into payload slot p and then
+// sets the flag (payload first: readers load it only after seeing the flag).
+template <int N>
+__device__ __forceinline__ void publish(double* p, uint32_t* flag, const double (&v)[N], uint32_t f) {
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int e = 0; e < N; ++e) payload_store(p + e, v[e]);
+        flag_store(flag, f);
+    }
+}
+
+// Look-back in fixed groups of 64 chunks (0-63, 64-127, ...), so that every
+// sum and product is taken in the same order whatever the timing, and the
+// results are bitwise reproducible:
+//   * forward, chunk c of group g = c / 64:
+//       P_c = I(64g - 1) * A_{64g} * ... * A_{c-1},
+//     with A_j the aggregate frame of chunk j and I(64g - 1) the inclusive
+//     prefix of the previous group's last chunk (g > 0);
+//   * backward, chunk c of group g:
+//       S_c = A_{c+1} + ... + A_{64g+63} + I(64g + 64),
+//     with A_j chunk j's sum and I(64g + 64) the inclusive suffix of the next
+//     group's first chunk (where the chain has it).
+// Aggregates are published right after a chunk's own scan, so a chunk never
+// waits on a chain of look-backs inside its group; only the group-boundary
+// chunks publish inclusive values, and those form one chain per 64 chunks.
+// A lane of the window reads one chunk's aggregate; the window's product or
+// sum is one wave scan in lane order (identity or zero in unused lanes).
+
+// Waits until the flag of every valid lane is set (bounded); false after
+// kSpinLimit polls, with TPL_A2B_STATUS_TIMEOUT reported.
+constexpr int kSpinLimit = 1 << 21;
+__device__ __forceinline__ bool wait_window(const uint32_t* flag, bool valid, uint32_t* status) {
+    uint32_t f = 0;
+    for (int spins = 0; spins < kSpinLimit; ++spins) {
+        if (valid && f == 0) f = flag_load(flag);
+        if (__builtin_amdgcn_ballot_w64(valid && f == 0) == 0) {
+            after_flag();
+            return true;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if (threadIdx.x == 0) set_status(status, TPL_A2B_STATUS_TIMEOUT);
+    return false;
+}
+
+// Waits until one flag (read by every lane) is kFlagInclusive.
+__device__ __forceinline__ bool wait_inclusive(const uint32_t* flag, uint32_t* status) {
+    for (int spins = 0; spins < kSpinLimit; ++spins) {
+        if (__builtin_amdgcn_readfirstlane(flag_load(flag)) == kFlagInclusive) {
+            after_flag();
+            return true;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if (threadIdx.x == 0) set_status(status, TPL_A2B_STATUS_TIMEOUT);
+    return false;
+}
+
+// Forward: P <- the exclusive prefix of chunk c >= 1 of chain b (P enters
+// as I; uniform over the wave).
+__device__ void fwd_lookback(const ChunkArgs& a, int b, int c, double P[kF]) {
+    const int lane = threadIdx.x;
+    const uint32_t* fl = a.flags + (size_t)b * a.nc;
+    const double* pay = a.pay + (size_t)b * a.nc * kFwdPay;
+    const int lo = c & ~63;
+    if (lo > 0) {  // the previous group's inclusive prefix
+        if (!wait_inclusive(fl + lo - 1, a.status)) return;
+        const double* p = pay + (size_t)(lo - 1) * kFwdPay + kF;
+#pragma unroll
+        for (int e = 0; e < kF; ++e) P[e] = payload_load(p + e);
+    }
+    if (c == lo) return;
+    const int j = lo + lane;
+    const bool valid = j < c;
+    if (!wait_window(fl + (valid ? j : lo), valid, a.status)) return;
+    double v[kF];
+    if (valid) {
+        const double* p = pay + (size_t)j * kFwdPay;
+#pragma unroll
+        for (int e = 0; e < kF; ++e) v[e] = payload_load(p + e);
+    } else {
+#pragma unroll
+        for (int e = 0; e < kF; ++e) v[e] = ident(e);
+    }
+    wave_scan_frames(v);  // lane 63: A_lo * ... * A_{c-1}
+    double W[kF];
+#pragma unroll
+    for (int e = 0; e < kF; ++e) W[e] = read_lane(v[e], 63);
+    frame_rmul_to(P, W, v);  // v = P * W (P in registers, W uniform)
+#pragma unroll
+    for (int e = 0; e < kF; ++e) P[e] = v[e];
+}
+
+// Backward: carry <- the sum of (T, F) over the chunks after chunk c of
+// chain b (nreal chunks); carry enters as 0.
+__device__ void bwd_lookback(const ChunkArgs& a, int b, int c, int nreal, double carry[6]) {
+    const int lane = threadIdx.x;
+    const uint32_t* fl = a.flags + (size_t)b * a.nc;
+    const double* pay = a.pay + (size_t)b * a.nc * kBwdPay;
+    const int lo = c & ~63, hi = min(lo + 64, nreal);  // c's group, clipped to the chain
+    const int j = lo + lane;
+    const bool valid = j > c && j < hi;
+    if (!wait_window(fl + (valid ? j : c), valid, a.status)) return;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    if (valid) {
+        const double* p = pay + (size_t)j * kBwdPay;
+#pragma unroll
+        for (int e = 0; e < 6; ++e) v[e] = payload_load(p + e);
+    }
+    wave_scan_sum<6>(v);
+#pragma unroll
+    for (int e = 0; e < 6; ++e) carry[e] = read_lane(v[e], 63);
+    if (hi < nreal) {  // the next group's inclusive suffix
+        if (!wait_inclusive(fl + hi, a.status)) return;
+        const double* p = pay + (size_t)hi * kBwdPay + 6;
+#pragma unroll
+        for (int e = 0; e < 6; ++e) carry[e] += payload_load(p + e);
+    }
+}
+
+// ------------------------------------------------------------ forward chunk
+// Steps of one chunk (chunk c of chain b, first residue r0 = c*C):
+//   1. the lanes load their R angles per row straight into registers (zero
+//      past the chain); a slot guesses c = s and loads before its ticket
+//      returns, reloading in the rare case the guess was wrong;
+//   2. pass 1: per lane, FP64 sincos and the serial product of its R
+//      residues; positions of the lane's atoms 1 .. 3R-1 in the lane's own
+//      frame go to the lane's LDS slot as FP32 (|p| <= 3R * 1.53 A);
+//   3. DPP scan of the lane frames; look-back over earlier chunks (P);
+//   4. E = P * (lane's prefix in the wave) is the frame of the lane's first
+//      atom; lanes on a backward-chunk boundary save it as a checkpoint;
+//   5. pass 2: atom 0 is the translation of E, atom a >= 1 is E p_a, written
+//      over p_a in LDS;
+//   6. the wave stores its 9*C-float span with 16-byte stores from LDS,
+//      zeros past the chain.
+template <typename T, int R>
+__global__ void __launch_bounds__(64) a2b_fwd_chunk(ChunkArgs a) {
+    constexpr int C = 64 * R;
+    constexpr int E9 = 9 * R;  // output floats per lane = the lane's LDS stride
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    __shared__ double s_tab[128];
+    T* s_x = reinterpret_cast<T*>(s_dyn);
+    T* s_l = s_x + threadIdx.x * E9;
+    const int lane = threadIdx.x;
+    int b, s;
+    if (!chunk_slot(a, &b, &s)) return;
+    const int L = a.L;
+    const int n = clamp_length(a, a.lengths[b], s == 0 && lane == 0);
+    const int nreal = (n + C - 1) / C;
+    T* xrow = static_cast<T*>(a.coords) + (size_t)b * 9 * L;
+    if (s >= nreal) {  // wholly past the chain: its zeros
+        const int f0 = 9 * s * C;
+        wave_store_run<T>(xrow + f0, min(9 * C, 9 * L - f0), 0, s_x, lane);
+        return;
+    }
+    for (int i = lane; i < 128; i += 64) s_tab[i] = kTabCS[i];
+    const T* arow = static_cast<const T*>(a.angles) + (size_t)b * 3 * L;
+    T ang[3][R];
+    {
+        const int rl = s * C + lane * R;
+        lane_load_rows<T, R>(arow + rl, L, min(max(n - rl, 0), R), L - rl, ang);
+    }
+    const int c = nreal > 1 ? chain_ticket(a, b, nreal) : 0;
+    if (c < 0) return;
+    const int r0 = c * C;
+    const int rl = r0 + lane * R;
+    const int cnt = min(max(n - rl, 0), R);  // residues of the chain in this lane
+    if (c != s) lane_load_rows<T, R>(arow + rl, L, cnt, L - rl, ang);
+    __syncthreads();  // s_tab
+
+    // Pass 1.
+    double E[kF];
+    frame_identity(E);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double sn, cs;
+            sincos_tab((double)ang[k][q], s_tab, &sn, &cs);
+            if (k == 0) frame_step<0>(E, cs, sn);
+            if (k == 1) frame_step<1>(E, cs, sn);
+            if (k == 2) frame_step<2>(E, cs, sn);
+            const int at = 3 * q + k + 1;  // lane atom placed by this step
+            if (at < 3 * R) {
+                s_l[3 * at] = (T)E[9];
+                s_l[3 * at + 1] = (T)E[10];
+                s_l[3 * at + 2] = (T)E[11];
+            }
+        }
+    }
+    wave_scan_frames(E);
+    double agg[kF];  // the chunk's aggregate (uniform)
+#pragma unroll
+    for (int e = 0; e < kF; ++e) agg[e] = read_lane(E[e], 63);
+    wave_shift_frames(E);
+
+    // Chunks of one chain: a chunk's aggregate is read by the later chunks
+    // of its group; the last chunk of a group publishes its inclusive prefix
+    // instead, for the next group; the chain's last chunk publishes nothing.
+    if (nreal > 1) {
+        double* my = a.pay + ((size_t)b * a.nc + c) * kFwdPay;
+        uint32_t* myf = a.flags + (size_t)b * a.nc + c;
+        const bool last = c == nreal - 1;
+        const bool boundary = (c & 63) == 63;
+        if (!last && !boundary) publish<kF>(my, myf, agg, kFlagAggregate);
+        if (c > 0) {
+            double P[kF];
+            frame_identity(P);
+            fwd_lookback(a, b, c, P);
+            frame_lmul(P, E);
+            if (!last && boundary) {
+                double incl[kF];
+                frame_rmul_to(P, agg, incl);
+                publish<kF>(my + kF, myf, incl, kFlagInclusive);
+            }
+        }
+        chain_done<kFwdPay>(a, b, nreal);
+    }
+
+    // Checkpoint at a backward-chunk boundary inside the chain.
+    if (a.ckpt && rl > 0 && rl < n && (rl & (a.K - 1)) == 0) {
+        double* d = a.ckpt + ((size_t)b * a.nk + rl / a.K) * kF;
+#pragma unroll
+        for (int e = 0; e < kF; ++e) d[e] = E[e];
+    }
+
+    // Pass 2, in place in the lane's slot.
+    if (cnt > 0) {
+        // Sum of the lane's coordinates: NaN or inf anywhere shows.
+        // Coordinates past the chain come from zero angles: finite.
+        float chk = 0.f;
+#pragma unroll
+        for (int at = 0; at < 3 * R; ++at) {
+            double x[3];
+            if (at == 0) {
+                x[0] = E[9];
+                x[1] = E[10];
+                x[2] = E[11];
+            } else {
+                affine(E, (double)s_l[3 * at], (double)s_l[3 * at + 1], (double)s_l[3 * at + 2], x);
+            }
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                const T o = (T)x[cc];
+                s_l[3 * at + cc] = o;
+                chk += (float)o;
+            }
+        }
+        if (!__builtin_isfinite(chk)) set_status(a.status, TPL_A2B_STATUS_NAN);
+    }
+    wave_lds_sync();
+    const int g0 = 9 * r0;
+    const int wcount = min(9 * C, 9 * L - g0);
+    wave_store_run<T>(xrow + g0, wcount, min(max(9 * n - g0, 0), wcount), s_x, lane);
+}
+
+// ----------------------------------------------------------- backward chunk
+// Chain atom j >= 1 (atoms N_0, CA_0, C_0, N_1, ...) is placed by the
+// transform whose angle theta_j is phi_r, psi_r, omega_r for j = 3r+1, 3r+2,
+// 3r+3.  theta_j turns every later atom about the bond axis u_j (column 0 of
+// A_j) through x_j, so
+//   dL/dtheta_j = u_j . sum_{i>=j} (x_i - x_j) x g_i = u_j . (T_j - x_j x F_j),
+//   T_j = sum_{i>=j} x_i x g_i,  F_j = sum_{i>=j} g_i
+// (the i = j term is zero).  Steps of one chunk (C = 64R = the checkpoint
+// stride K):
+//   1. the lanes load their angles and the gradients of their 3R atoms;
+//   2. pass 1: per lane, sincos and the serial product of its residues from
+//      the identity, as in the forward; the lane keeps (cos, sin) of its 3R
+//      angles and sums, over its atoms a in the lane's own frame,
+//      M = sum p_a g_a^T and F = sum g_a;
+//   3. DPP scan of the lane frames; E = checkpoint * (lane's prefix) is the
+//      frame of the lane's first atom, and its sum over the lane's atoms is
+//      W = (eps : (R_E M) + t_E x F, F);
+//   4. suffix sums of W over later lanes (DPP) and later chunks (look-back);
+//   5. pass B: from E the lane steps through its 3R transforms again with the
+//      kept (cos, sin); at atom j it has x_j and u_j and the suffix
+//      S_j = sum over atoms i >= j, gives the gradient, and drops atom j's
+//      term from S.  psi and omega of the last residue are 0.
+template <typename T, int R>
+__global__ void __launch_bounds__(64) a2b_bwd_chunk(ChunkArgs a) {
+    constexpr int C = 64 * R;
+    __shared__ double s_tab[128];
+    const int lane = threadIdx.x;
+    int b, s;
+    if (!chunk_slot(a, &b, &s)) return;
+    const int L = a.L;
+    const int n = clamp_length(a, a.lengths[b], s == 0 && lane == 0);
+    const int nreal = (n + C - 1) / C;
+    T* grow = static_cast<T*>(a.grad_angles) + (size_t)b * 3 * L;
+    if (s >= nreal) {  // wholly past the chain: its zeros
+        const int rl = s * C + lane * R;
+        const int crow = min(max(L - rl, 0), R);
+        if (crow > 0) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) lane_store_zero<T, R>(grow + (size_t)k * L + rl, crow);
+        }
+        return;
+    }
+    for (int i = lane; i < 128; i += 64) s_tab[i] = kTabCS[i];
+    const T* arow = static_cast<const T*>(a.angles) + (size_t)b * 3 * L;
+    const T* gin = static_cast<const T*>(a.grad) + (size_t)b * 9 * L;
+    T ang[3][R];
+    T gr[9 * R];
+    auto load = [&](int cc) {
+        const int r = cc * C + lane * R;
+        const int k = min(max(n - r, 0), R);
+        lane_load_rows<T, R>(arow + r, L, k, L - r, ang);
+        lane_load<T, 9 * R>(gin + (size_t)9 * r, 9 * k, gr, 9 * (L - r));
+    };
+    const int guess = nreal - 1 - s;  // the chunk of this slot if tickets follow slot order
+    load(guess);
+    const int t = nreal > 1 ? chain_ticket(a, b, nreal) : 0;
+    if (t < 0) return;
+    const int c = nreal - 1 - t;
+    if (c != guess) load(c);
+    const int r0 = c * C;
+    const int rl = r0 + lane * R;
+    const int cnt = min(max(n - rl, 0), R);
+    double P[kF];  // checkpoint: frame of the chunk's first atom (uniform)
+    if (c == 0) {
+        frame_identity(P);
+    } else {
+        const double* ck = a.ckpt + ((size_t)b * a.nk + c) * kF;
+#pragma unroll
+        for (int e = 0; e < kF; ++e) P[e] = ck[e];
+    }
+    __syncthreads();  // s_tab
+
+    // Pass 1 in the lane's own frame.
+    double f[kF];
+    frame_identity(f);
+    double cs[3][R], sn[3][R];
+    double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // M[3m + k] = sum_a p_a[m] g_a[k]
+    double F[3] = {(double)gr[0], (double)gr[1], (double)gr[2]};  // atom 0 sits at p = 0
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            sincos_tab((double)ang[k][q], s_tab, &sn[k][q], &cs[k][q]);
+            if (k == 0) frame_step<0>(f, cs[k][q], sn[k][q]);
+            if (k == 1) frame_step<1>(f, cs[k][q], sn[k][q]);
+            if (k == 2) frame_step<2>(f, cs[k][q], sn[k][q]);
+            const int at = 3 * q + k + 1;
+            if (at < 3 * R) {
+                const double g0 = (double)gr[3 * at], g1 = (double)gr[3 * at + 1],
+                             g2 = (double)gr[3 * at + 2];
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    M[3 * m] = fma(f[9 + m], g0, M[3 * m]);
+                    M[3 * m + 1] = fma(f[9 + m], g1, M[3 * m + 1]);
+                    M[3 * m + 2] = fma(f[9 + m], g2, M[3 * m + 2]);
+                }
+                F[0] += g0;
+                F[1] += g1;
+                F[2] += g2;
+            }
+        }
+    }
+    wave_scan_frames(f);
+    wave_shift_frames(f);
+    frame_lmul(P, f);  // f = E, the frame of the lane's first atom
+
+    // W = sum over the lane's atoms of (x x g, g), x = R_E p + t_E:
+    // sum (R_E p) x g = eps : (R_E M), with (R_E M)[j][k] = sum_m R[j][m] M[m][k].
+    double W[6];
+    {
+        auto rm = [&](int j, int k) {
+            return f[j] * M[k] + f[3 + j] * M[3 + k] + f[6 + j] * M[6 + k];
+        };
+        W[0] = rm(1, 2) - rm(2, 1) + (f[10] * F[2] - f[11] * F[1]);
+        W[1] = rm(2, 0) - rm(0, 2) + (f[11] * F[0] - f[9] * F[2]);
+        W[2] = rm(0, 1) - rm(1, 0) + (f[9] * F[1] - f[10] * F[0]);
+        W[3] = F[0];
+        W[4] = F[1];
+        W[5] = F[2];
+    }
+    // S = sum over later lanes of the chunk, then over later chunks.
+    double S[6];
+#pragma unroll
+    for (int e = 0; e < 6; ++e) S[e] = W[e];
+    wave_scan_sum<6>(S);  // inclusive prefix over lanes
+    double tot[6];
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+        tot[e] = read_lane(S[e], 63);
+        S[e] = tot[e] - S[e];
+    }
+    // Chunks of one chain: a chunk's sum is read by the earlier chunks of
+    // its group; the first chunk of a group (after the first group) publishes
+    // its inclusive suffix instead, for the previous group; chunk 0 publishes
+    // nothing.
+    if (nreal > 1) {
+        double* my = a.pay + ((size_t)b * a.nc + c) * kBwdPay;
+        uint32_t* myf = a.flags + (size_t)b * a.nc + c;
+        const bool boundary = (c & 63) == 0;
+        if (!boundary) publish<6>(my, myf, tot, kFlagAggregate);
+        double carry[6] = {0, 0, 0, 0, 0, 0};
+        if (c < nreal - 1) bwd_lookback(a, b, c, nreal, carry);
+        if (boundary && c > 0) {
+            double incl[6];
+#pragma unroll
+            for (int e = 0; e < 6; ++e) incl[e] = carry[e] + tot[e];
+            publish<6>(my + 6, myf, incl, kFlagInclusive);
+        }
+#pragma unroll
+        for (int e = 0; e < 6; ++e) S[e] += carry[e];
+        chain_done<kBwdPay>(a, b, nreal);
+    }
+
+    if (cnt > 0) {
+        // S becomes the suffix from the lane's atom 1: add the lane's own
+        // atoms, drop atom 0.
+        {
+            const double g0 = (double)gr[0], g1 = (double)gr[1], g2 = (double)gr[2];
+            S[0] += W[0] - (f[10] * g2 - f[11] * g1);
+            S[1] += W[1] - (f[11] * g0 - f[9] * g2);
+            S[2] += W[2] - (f[9] * g1 - f[10] * g0);
+            S[3] += W[3] - g0;
+            S[4] += W[4] - g1;
+            S[5] += W[5] - g2;
+        }
+        T out[3][R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (k == 0) frame_step<0>(f, cs[k][q], sn[k][q]);
+                if (k == 1) frame_step<1>(f, cs[k][q], sn[k][q]);
+                if (k == 2) frame_step<2>(f, cs[k][q], sn[k][q]);
+                const int j = 3 * q + k + 1;  // lane atom j: x = t(f), u = column 0 of f
+                const double x0 = f[9], x1 = f[10], x2 = f[11];
+                const double v = f[0] * (S[0] - (x1 * S[5] - x2 * S[4])) +
+                                 f[1] * (S[1] - (x2 * S[3] - x0 * S[5])) +
+                                 f[2] * (S[2] - (x0 * S[4] - x1 * S[3]));
+                const int r = rl + q;
+                const bool zero = (r >= n) || (r == n - 1 && k > 0);
+                out[k][q] = zero ? (T)0 : (T)v;
+                if (j < 3 * R) {
+                    const double g0 = (double)gr[3 * j], g1 = (double)gr[3 * j + 1],
+                                 g2 = (double)gr[3 * j + 2];
+                    S[0] -= x1 * g2 - x2 * g1;
+                    S[1] -= x2 * g0 - x0 * g2;
+                    S[2] -= x0 * g1 - x1 * g0;
+                    S[3] -= g0;
+                    S[4] -= g1;
+                    S[5] -= g2;
+                }
+            }
+        }
+        bool bad = false;
+        const int crow = min(max(L - rl, 0), R);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) bad |= (out[k][q] != out[k][q]);
+            lane_store<T, R>(grow + (size_t)k * L + rl, crow, out[k]);
+        }
+        if (bad) set_status(a.status, TPL_A2B_STATUS_NAN);
+    } else {
+        const int crow = min(max(L - rl, 0), R);
+        if (crow > 0) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) lane_store_zero<T, R>(grow + (size_t)k * L + rl, crow);
+        }
+    }
+}
+
+__global__ void a2b_zero_words(uint32_t* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0u;
 }
 
@@ -1233,169 +1174,122 @@ __global__ void a2b_bwd_plain(T* grad_angles, const T* grad_coords, const T* ang
     if (sum != sum) set_status(status, TPL_A2B_STATUS_NAN);
 }
 
+
 // ------------------------------------------------------------- host side
 bool valid_config(const tpl_a2b_config* c) {
     if (!c) return false;
-    const int w = c->waves, r = c->res_per_lane;
-    const bool wok = (w == 1 || w == 2 || w == 4);
-    const bool rok = (r == 1 || r == 2 || r == 4 || r == 8);
-    return wok && rok && c->max_wgs >= 0 && c->flags == 0;
+    const int f = c->fwd_res_per_lane, b = c->bwd_res_per_lane;
+    const bool fok = (f == 1 || f == 2 || f == 4 || f == 8);
+    const bool bok = (b == 1 || b == 2 || b == 4);
+    return fok && bok && c->flags == 0 && c->reserved == 0;
 }
 
-// Largest tiles (each lane's runs stay in registers, no spills), and the
-// backward's residues per lane: one, so that a lane keeps its coordinates,
-// gradients and residuals in registers from pass A to pass B (about 100
-// VGPRs, four or more waves per SIMD).
-constexpr int kMaxTileF32 = 1024, kMaxTileF64 = 512;
-constexpr int kBwdR = 1;
-
+// Residues per lane the registers hold without spilling: the forward keeps R
+// angles per row and its frame, the backward also (cos, sin) of its 3R angles
+// and its 9R coordinate gradients.
 bool fits_dtype(int dtype, const tpl_a2b_config* c) {
-    const int tr = 64 * c->waves * c->res_per_lane;
-    return tr <= (dtype == TPL_A2B_F32 ? kMaxTileF32 : kMaxTileF64);
-}
-
-// Tiling of one call: forward tiles of trf = 64*W*R residues; backward tiles
-// of trb = 64*W*kBwdR residues, a divisor of trf.
-struct Tiling {
-    int trf, ntf, trb, ntb, rb;
-};
-
-Tiling tiling(int dtype, int L, const tpl_a2b_config* c) {
-    Tiling t{};
-    t.trf = 64 * c->waves * c->res_per_lane;
-    t.rb = kBwdR;
-    t.trb = 64 * c->waves * t.rb;
-    t.ntf = L <= 0 ? 0 : (L + t.trf - 1) / t.trf;
-    t.ntb = L <= 0 ? 0 : (L + t.trb - 1) / t.trb;
-    return t;
+    if (dtype == TPL_A2B_F32) return true;
+    return c->fwd_res_per_lane <= 4 && c->bwd_res_per_lane <= 2;
 }
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
-
-// state and sync regions start on 256-byte boundaries (the kernels move the
-// residual runs in aligned 4- to 16-byte units).
 bool misaligned(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 255) != 0; }
 
-// Per-call scratch of the forward's own look-back (ntf > 1): flags [B][ntf]
-// and tickets [B] (zeroed by a2b_zero_words before the forward), payload.
+int chunks(int L, int C) { return L <= 0 ? 0 : (L + C - 1) / C; }
+
+// Forward -> backward state: one FP64 frame per backward chunk of every chain
+// ([B][ncb][12] doubles; entry 0 of each chain, the identity, is not written).
+size_t state_total(int B, int L, const tpl_a2b_config* c) {
+    return align256((size_t)B * chunks(L, 64 * c->bwd_res_per_lane) * kF * sizeof(double));
+}
+
+// Sync: the forward's look-back words then the backward's, each as tickets
+// [B], done [B], flags [B][nc] (uint32), then payload [B][nc][pay] doubles.
+struct SyncRegion {
+    size_t tickets, done, flags, pay, end;
+};
+SyncRegion sync_region(size_t base, int B, int nc, int pay) {
+    SyncRegion r{};
+    r.tickets = base;
+    r.done = r.tickets + (size_t)B * 4;
+    r.flags = r.done + (size_t)B * 4;
+    r.pay = align256(r.flags + (size_t)B * nc * 4);
+    r.end = align256(r.pay + (size_t)B * nc * pay * sizeof(double));
+    return r;
+}
 struct SyncLayout {
-    size_t flags_off, tickets_off, words, pay_off, total;
+    SyncRegion fwd, bwd;
 };
-SyncLayout sync_layout(int B, int ntf) {
+SyncLayout sync_layout(int B, int L, const tpl_a2b_config* c) {
     SyncLayout s{};
-    if (ntf <= 1) return s;
-    const size_t tiles = (size_t)B * ntf;
-    s.flags_off = 0;
-    s.tickets_off = tiles * 4;
-    s.words = tiles + B;
-    s.pay_off = align256(s.words * 4);
-    s.total = s.pay_off + tiles * kFwdPayload * sizeof(double);
+    s.fwd = sync_region(0, B, chunks(L, 64 * c->fwd_res_per_lane), kFwdPay);
+    s.bwd = sync_region(s.fwd.end, B, chunks(L, 64 * c->bwd_res_per_lane), kBwdPay);
     return s;
 }
 
-// Forward -> backward state: FP32 coordinate residuals [B][9][Lp] halves,
-// Lp = L rounded up to 8 (FP32 only), then the backward's look-back words
-// and payload (ntb > 1).
-int res_lp(int L) { return (L + 7) & ~7; }
-
-struct StateLayout {
-    size_t res_off, pay_off, flags_off, tickets_off, done_off, total;
-};
-StateLayout state_layout(int dtype, int B, int L, int ntb) {
-    StateLayout s{};
-    s.res_off = 0;
-    const size_t res = dtype == TPL_A2B_F32 ? (size_t)B * 9 * res_lp(L) * 2 : 0;
-    s.total = align256(res);
-    if (ntb <= 1) return s;
-    const size_t tiles = (size_t)B * ntb;
-    s.pay_off = s.total;
-    s.flags_off = s.pay_off + tiles * kBwdPayload * sizeof(double);
-    s.tickets_off = s.flags_off + tiles * 4;
-    s.done_off = s.tickets_off + (size_t)B * 4;
-    s.total = align256(s.done_off + (size_t)B * 4);
-    return s;
-}
-
-using KernelFn = void (*)(ScanArgs);
-
-constexpr int kMaxThreads = 256;  // up to 4 waves per workgroup
-constexpr int kWholeMaxTiles = 4, kWholeMinChains = 512;
-
-// Forward variants: (LB, AHEAD) = (1, 0) several tiles per chain, (0, 0) one
-// tile per chain and one workgroup each, (0, 1) one tile per chain on a grid
-// smaller than the batch.
-template <typename T, int R>
-KernelFn pick_kernel(bool fwd, bool lb, bool ahead) {
-    if (!fwd) return a2b_bwd_scan<T, kBwdR, kMaxThreads>;
-    if (lb) return a2b_fwd_scan<T, R, kMaxThreads, true, false>;
-    return ahead ? a2b_fwd_scan<T, R, kMaxThreads, false, true>
-                 : a2b_fwd_scan<T, R, kMaxThreads, false, false>;
-}
+using KernelFn = void (*)(ChunkArgs);
 
 template <typename T>
-KernelFn pick_kernel(bool fwd, int r, bool lb, bool ahead) {
+KernelFn fwd_kernel(int r) {
     switch (r) {
-        case 1: return pick_kernel<T, 1>(fwd, lb, ahead);
-        case 2: return pick_kernel<T, 2>(fwd, lb, ahead);
-        case 4: return pick_kernel<T, 4>(fwd, lb, ahead);
-        default: return pick_kernel<T, 8>(fwd, lb, ahead);
+        case 1: return a2b_fwd_chunk<T, 1>;
+        case 2: return a2b_fwd_chunk<T, 2>;
+        case 4: return a2b_fwd_chunk<T, 4>;
+        default: return a2b_fwd_chunk<T, 8>;
     }
+}
+template <>
+KernelFn fwd_kernel<double>(int r) {
+    switch (r) {
+        case 1: return a2b_fwd_chunk<double, 1>;
+        case 2: return a2b_fwd_chunk<double, 2>;
+        default: return a2b_fwd_chunk<double, 4>;
+    }
+}
+template <typename T>
+KernelFn bwd_kernel(int r) {
+    switch (r) {
+        case 1: return a2b_bwd_chunk<T, 1>;
+        case 2: return a2b_bwd_chunk<T, 2>;
+        default: return a2b_bwd_chunk<T, 4>;
+    }
+}
+template <>
+KernelFn bwd_kernel<double>(int r) {
+    return r == 1 ? a2b_bwd_chunk<double, 1> : a2b_bwd_chunk<double, 2>;
 }
 
 int check_common(int dtype, int B, int L) {
     if (dtype != TPL_A2B_F32 && dtype != TPL_A2B_F64) return TPL_A2B_ERR_ARG;
     if (B < 0 || L < 0) return TPL_A2B_ERR_ARG;
+    // 9L and B*L index as int; the grid holds 8*ceil(B/8)*ceil(L/64) workgroups
     if ((int64_t)9 * L > INT32_MAX || (int64_t)B * L > INT32_MAX) return TPL_A2B_ERR_SIZE;
+    if ((int64_t)8 * ((B + 7) / 8) * chunks(L, 64) > INT32_MAX) return TPL_A2B_ERR_SIZE;
     return TPL_A2B_OK;
 }
 
-// state == nullptr (forward only): no residuals, no backward words.
-int launch_scan(bool fwd, int dtype, ScanArgs args, void* state, void* sync,
-                const tpl_a2b_config* cfg, hipStream_t stream) {
-    const Tiling tl = tiling(dtype, args.L, cfg);
-    args.trf = tl.trf;
-    args.ntf = tl.ntf;
-    args.trb = tl.trb;
-    args.ntb = tl.ntb;
-    args.res_lp = res_lp(args.L);
-    if (state) {
-        const StateLayout st = state_layout(dtype, args.B, args.L, tl.ntb);
-        char* base = static_cast<char*>(state);
-        if (dtype == TPL_A2B_F32) args.res = reinterpret_cast<_Float16*>(base + st.res_off);
-        if (tl.ntb > 1) {
-            args.bpay = reinterpret_cast<double*>(base + st.pay_off);
-            args.bflags = reinterpret_cast<uint32_t*>(base + st.flags_off);
-            args.btickets = reinterpret_cast<uint32_t*>(base + st.tickets_off);
-            args.bdone = reinterpret_cast<uint32_t*>(base + st.done_off);
-        }
-    }
-    if (fwd && tl.ntf > 1) {
-        const SyncLayout sl = sync_layout(args.B, tl.ntf);
-        char* base = static_cast<char*>(sync);
-        args.fflags = reinterpret_cast<uint32_t*>(base + sl.flags_off);
-        args.ftickets = reinterpret_cast<uint32_t*>(base + sl.tickets_off);
-        args.fpay = reinterpret_cast<double*>(base + sl.pay_off);
-        const int zb = (int)std::min<size_t>((sl.words + 255) / 256, 1024);
-        hipLaunchKernelGGL(a2b_zero_words, dim3(zb), dim3(256), 0, stream, args.fflags,
-                           (int)sl.words);
-        if (hipGetLastError() != hipSuccess) return TPL_A2B_ERR_LAUNCH;
-    }
-    // Backward: with few tiles per chain and enough chains to fill the device,
-    // one workgroup takes a whole chain (no tickets, no look-back waits);
-    // otherwise one workgroup per tile.
-    args.bwhole = (!fwd && tl.ntb > 1 && tl.ntb <= kWholeMaxTiles && args.B >= kWholeMinChains) ? 1 : 0;
-    const int nt = fwd ? tl.ntf : (args.bwhole ? 1 : tl.ntb);
-    int grid = args.B * nt;
-    if (fwd && cfg->max_wgs > 0) grid = std::min(grid, cfg->max_wgs);
-    const int threads = 64 * cfg->waves;
-    const int r = fwd ? cfg->res_per_lane : tl.rb;
-    const bool f32 = dtype == TPL_A2B_F32;
-    const bool lb = fwd && tl.ntf > 1;
-    const bool ahead = fwd && !lb && grid < args.B;
-    KernelFn k = f32 ? pick_kernel<float>(fwd, r, lb, ahead) : pick_kernel<double>(fwd, r, lb, ahead);
-    // The forward stages each lane's 9R coordinates in LDS, in output order.
-    const size_t lds = fwd ? (size_t)(9 * r) * (f32 ? 4 : 8) * threads : 0;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(threads), lds, stream, args);
+ChunkArgs base_args(int B, int L, int C, const tpl_a2b_config* cfg) {
+    ChunkArgs a{};
+    a.B = B;
+    a.L = L;
+    a.nc = chunks(L, C);
+    a.K = 64 * cfg->bwd_res_per_lane;
+    a.nk = chunks(L, a.K);
+    a.bx = (B + 7) / 8;
+    return a;
+}
+
+void bind_sync(ChunkArgs& a, void* sync, const SyncRegion& r) {
+    char* base = static_cast<char*>(sync);
+    a.tickets = reinterpret_cast<uint32_t*>(base + r.tickets);
+    a.done = reinterpret_cast<uint32_t*>(base + r.done);
+    a.flags = reinterpret_cast<uint32_t*>(base + r.flags);
+    a.pay = reinterpret_cast<double*>(base + r.pay);
+}
+
+int launch(KernelFn k, const ChunkArgs& a, size_t lds, hipStream_t stream) {
+    const dim3 grid((unsigned)(8 * a.bx * a.nc)), block(64);
+    hipLaunchKernelGGL(k, grid, block, lds, stream, a);
     return hipGetLastError() == hipSuccess ? TPL_A2B_OK : TPL_A2B_ERR_LAUNCH;
 }
 
@@ -1407,8 +1301,8 @@ extern "C" {
 const char* tpl_a2b_strerror(int code) {
     switch (code) {
         case TPL_A2B_OK: return "ok";
-        case TPL_A2B_ERR_ARG: return "invalid argument (null pointer, negative size or unknown dtype)";
-        case TPL_A2B_ERR_CONFIG: return "unsupported tile configuration";
+        case TPL_A2B_ERR_ARG: return "invalid argument (null pointer, negative size, misaligned buffer or unknown dtype)";
+        case TPL_A2B_ERR_CONFIG: return "unsupported chunk configuration";
         case TPL_A2B_ERR_LAUNCH: return "kernel launch failed";
         case TPL_A2B_ERR_HIP: return "HIP runtime call failed";
         case TPL_A2B_ERR_SIZE: return "batch * angles_stride too large";
@@ -1418,30 +1312,38 @@ const char* tpl_a2b_strerror(int code) {
 
 int tpl_a2b_default_config(int batch, int angles_stride, tpl_a2b_config* cfg) {
     if (!cfg || batch < 0 || angles_stride < 0) return TPL_A2B_ERR_ARG;
-    cfg->waves = 4;
-    cfg->max_wgs = 0;
+    cfg->fwd_res_per_lane = angles_stride <= 64 ? 1 : (angles_stride <= 128 ? 2 : 4);
+    cfg->bwd_res_per_lane = angles_stride <= 64 ? 1 : 2;
     cfg->flags = 0;
-    // One tile per chain up to 1024 residues, else 1024-residue tiles.
-    const int L = angles_stride;
-    cfg->res_per_lane = L <= 256 ? 1 : (L <= 512 ? 2 : 4);
+    cfg->reserved = 0;
     return TPL_A2B_OK;
 }
 
-int64_t tpl_a2b_tile_residues(const tpl_a2b_config* cfg) {
+int64_t tpl_a2b_chunk_residues(const tpl_a2b_config* cfg, int backward) {
     if (!valid_config(cfg)) return -1;
-    return (int64_t)64 * cfg->waves * cfg->res_per_lane;
+    return (int64_t)64 * (backward ? cfg->bwd_res_per_lane : cfg->fwd_res_per_lane);
 }
 
 size_t tpl_a2b_state_bytes(int dtype, int batch, int angles_stride, const tpl_a2b_config* cfg) {
     if (check_common(dtype, batch, angles_stride) || !valid_config(cfg)) return 0;
     if (batch == 0 || angles_stride == 0) return 0;
-    return state_layout(dtype, batch, angles_stride, tiling(dtype, angles_stride, cfg).ntb).total;
+    return state_total(batch, angles_stride, cfg);
 }
 
 size_t tpl_a2b_sync_bytes(int dtype, int batch, int angles_stride, const tpl_a2b_config* cfg) {
     if (check_common(dtype, batch, angles_stride) || !valid_config(cfg)) return 0;
     if (batch == 0 || angles_stride == 0) return 0;
-    return sync_layout(batch, tiling(dtype, angles_stride, cfg).ntf).total;
+    return sync_layout(batch, angles_stride, cfg).bwd.end;
+}
+
+int tpl_a2b_sync_clear(void* sync, size_t bytes, tpl_stream_t stream) {
+    if (bytes == 0) return TPL_A2B_OK;
+    if (!sync || misaligned(sync) || bytes % 4) return TPL_A2B_ERR_ARG;
+    const size_t words = bytes / 4;
+    const unsigned blocks = (unsigned)std::min<size_t>((words + 255) / 256, 1024);
+    hipLaunchKernelGGL(a2b_zero_words, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<uint32_t*>(sync), words);
+    return hipGetLastError() == hipSuccess ? TPL_A2B_OK : TPL_A2B_ERR_LAUNCH;
 }
 
 int tpl_a2b_forward(int dtype, const void* angles, void* coords, const int32_t* lengths,
@@ -1451,41 +1353,45 @@ int tpl_a2b_forward(int dtype, const void* angles, void* coords, const int32_t* 
     if (rc) return rc;
     if (!valid_config(cfg) || !fits_dtype(dtype, cfg)) return TPL_A2B_ERR_CONFIG;
     if (batch == 0 || angles_stride == 0) return TPL_A2B_OK;
-    if (!angles || !coords || !lengths) return TPL_A2B_ERR_ARG;
-    if (tpl_a2b_sync_bytes(dtype, batch, angles_stride, cfg) && !sync) return TPL_A2B_ERR_ARG;
+    if (!angles || !coords || !lengths || !sync) return TPL_A2B_ERR_ARG;
     if (misaligned(state) || misaligned(sync)) return TPL_A2B_ERR_ARG;
-    ScanArgs a{};
+    const int R = cfg->fwd_res_per_lane;
+    ChunkArgs a = base_args(batch, angles_stride, 64 * R, cfg);
     a.angles = angles;
-    a.out = coords;
+    a.coords = coords;
     a.lengths = lengths;
     a.status = status;
-    a.B = batch;
-    a.L = angles_stride;
-    void* st = tpl_a2b_state_bytes(dtype, batch, angles_stride, cfg) ? state : nullptr;
-    return launch_scan(true, dtype, a, st, sync, cfg, static_cast<hipStream_t>(stream));
+    a.ckpt = static_cast<double*>(state);
+    bind_sync(a, sync, sync_layout(batch, angles_stride, cfg).fwd);
+    const bool f32 = dtype == TPL_A2B_F32;
+    const size_t lds = (size_t)9 * 64 * R * (f32 ? 4 : 8);
+    return launch(f32 ? fwd_kernel<float>(R) : fwd_kernel<double>(R), a, lds,
+                  static_cast<hipStream_t>(stream));
 }
 
-int tpl_a2b_backward(int dtype, void* grad_angles, const void* grad_coords, const void* coords,
-                     const int32_t* lengths, void* state, uint32_t* status, int batch,
-                     int angles_stride, const tpl_a2b_config* cfg, tpl_stream_t stream) {
+int tpl_a2b_backward(int dtype, void* grad_angles, const void* grad_coords, const void* angles,
+                     const int32_t* lengths, const void* state, void* sync, uint32_t* status,
+                     int batch, int angles_stride, const tpl_a2b_config* cfg,
+                     tpl_stream_t stream) {
     int rc = check_common(dtype, batch, angles_stride);
     if (rc) return rc;
     if (!valid_config(cfg) || !fits_dtype(dtype, cfg)) return TPL_A2B_ERR_CONFIG;
     if (batch == 0 || angles_stride == 0) return TPL_A2B_OK;
-    if (!grad_angles || !grad_coords || !coords || !lengths) return TPL_A2B_ERR_ARG;
-    const bool need_state = tpl_a2b_state_bytes(dtype, batch, angles_stride, cfg) != 0;
-    if (need_state && !state) return TPL_A2B_ERR_ARG;
-    if (need_state && misaligned(state)) return TPL_A2B_ERR_ARG;
-    ScanArgs a{};
-    a.coords = coords;
-    a.out = grad_angles;
+    if (!grad_angles || !grad_coords || !angles || !lengths || !state || !sync)
+        return TPL_A2B_ERR_ARG;
+    if (misaligned(state) || misaligned(sync)) return TPL_A2B_ERR_ARG;
+    const int R = cfg->bwd_res_per_lane;
+    ChunkArgs a = base_args(batch, angles_stride, 64 * R, cfg);
+    a.angles = angles;
     a.grad = grad_coords;
+    a.grad_angles = grad_angles;
     a.lengths = lengths;
     a.status = status;
-    a.B = batch;
-    a.L = angles_stride;
-    return launch_scan(false, dtype, a, need_state ? state : nullptr, nullptr, cfg,
-                       static_cast<hipStream_t>(stream));
+    a.ckpt = const_cast<double*>(static_cast<const double*>(state));
+    bind_sync(a, sync, sync_layout(batch, angles_stride, cfg).bwd);
+    const bool f32 = dtype == TPL_A2B_F32;
+    return launch(f32 ? bwd_kernel<float>(R) : bwd_kernel<double>(R), a, 0,
+                  static_cast<hipStream_t>(stream));
 }
 
 size_t tpl_a2b_plain_frames_bytes(int dtype, int batch, int angles_stride) {
@@ -1558,17 +1464,25 @@ int tpl_a2b_tune(int dtype, const void* angles, const int32_t* lengths,
     int best_i = -1;
     rc = TPL_A2B_OK;
     std::vector<float> times;
+    size_t sync_max = 0;  // the caller sized `sync` for the largest candidate
+    for (int c = 0; c < n_candidates; ++c)
+        if (valid_config(candidates + c))
+            sync_max = std::max(sync_max, tpl_a2b_sync_bytes(dtype, batch, angles_stride, candidates + c));
     for (int c = 0; c < n_candidates && rc == TPL_A2B_OK; ++c) {
         const tpl_a2b_config* cf = candidates + c;
         float ms = -1.f;
         if (valid_config(cf) && fits_dtype(dtype, cf)) {
             times.clear();
+            // Every call leaves the look-back words zero; clearing them before
+            // each candidate keeps one failed candidate from affecting the next.
+            rc = tpl_a2b_sync_clear(sync, sync_max, stream);
+            if (rc) break;
             for (int it = 0; it <= reps && rc == TPL_A2B_OK; ++it) {
                 if (hipEventRecord(e0, s) != hipSuccess) { rc = TPL_A2B_ERR_HIP; break; }
                 rc = tpl_a2b_forward(dtype, angles, coords, lengths, state, sync, nullptr, batch,
                                      angles_stride, cf, stream);
                 if (rc) break;
-                rc = tpl_a2b_backward(dtype, grad_angles, grad_coords, coords, lengths, state,
+                rc = tpl_a2b_backward(dtype, grad_angles, grad_coords, angles, lengths, state, sync,
                                       nullptr, batch, angles_stride, cf, stream);
                 if (rc) break;
                 if (hipEventRecord(e1, s) != hipSuccess) { rc = TPL_A2B_ERR_HIP; break; }

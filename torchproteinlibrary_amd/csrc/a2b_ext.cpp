@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "tpl_a2b.h"
+#include "tpl_a2b_cpu.h"
 
 namespace {
 
@@ -36,19 +37,19 @@ int dtype_code(const torch::Tensor& t) {
 }
 
 tpl_a2b_config to_config(const std::vector<int64_t>& c) {
-    TORCH_CHECK(c.size() == 3, "config must be (waves, res_per_lane, max_wgs)");
+    TORCH_CHECK(c.size() == 2, "config must be (fwd_res_per_lane, bwd_res_per_lane)");
     tpl_a2b_config cfg{};
-    cfg.waves = (int32_t)c[0];
-    cfg.res_per_lane = (int32_t)c[1];
-    cfg.max_wgs = (int32_t)c[2];
+    cfg.fwd_res_per_lane = (int32_t)c[0];
+    cfg.bwd_res_per_lane = (int32_t)c[1];
     cfg.flags = 0;
-    TORCH_CHECK(tpl_a2b_tile_residues(&cfg) > 0, "unsupported config (waves=", c[0],
-                ", res_per_lane=", c[1], ", max_wgs=", c[2], ")");
+    cfg.reserved = 0;
+    TORCH_CHECK(tpl_a2b_chunk_residues(&cfg, 0) > 0, "unsupported config (fwd_res_per_lane=", c[0],
+                ", bwd_res_per_lane=", c[1], ")");
     return cfg;
 }
 
 std::vector<int64_t> from_config(const tpl_a2b_config& c) {
-    return {c.waves, c.res_per_lane, c.max_wgs};
+    return {c.fwd_res_per_lane, c.bwd_res_per_lane};
 }
 
 // Validates angles [B,3,L] + lengths int32 [B] on one device; returns (B, L).
@@ -63,6 +64,28 @@ std::pair<int, int> check_inputs(const torch::Tensor& angles, const torch::Tenso
     TORCH_CHECK(lengths.device() == angles.device(), "angles_length must be on the device of input_angles");
     TORCH_CHECK(angles.size(0) <= INT32_MAX && angles.size(2) <= INT32_MAX, "input too large");
     return {(int)angles.size(0), (int)angles.size(2)};
+}
+
+// The host path: CPU tensors, contiguous, float64 only (the reference's CPU
+// Function is float64-only, Angles2Backbone.py:64-65), lengths int32.
+std::pair<int, int> check_cpu_inputs(const torch::Tensor& angles, const torch::Tensor& lengths) {
+    TORCH_CHECK(!angles.is_cuda() && angles.is_contiguous(), "input_angles must be a contiguous CPU tensor");
+    TORCH_CHECK(angles.scalar_type() == torch::kDouble,
+                "Angles2Backbone on the CPU computes in float64: input must be float64, got ",
+                angles.scalar_type());
+    TORCH_CHECK(!lengths.is_cuda() && lengths.is_contiguous(), "angles_length must be a contiguous CPU tensor");
+    TORCH_CHECK(lengths.scalar_type() == torch::kInt, "angles_length must be int32");
+    TORCH_CHECK(angles.dim() == 3 && angles.size(1) == 3,
+                "input_angles must be [batch, 3, max_length], got ", angles.sizes());
+    TORCH_CHECK(lengths.dim() == 1 && lengths.size(0) == angles.size(0), "angles_length must be [batch]");
+    TORCH_CHECK(angles.size(0) <= INT32_MAX && angles.size(2) <= INT32_MAX, "input too large");
+    return {(int)angles.size(0), (int)angles.size(2)};
+}
+
+void check_cpu_out(const torch::Tensor& t, std::vector<int64_t> shape, const char* name) {
+    TORCH_CHECK(!t.is_cuda() && t.is_contiguous(), name, " must be a contiguous CPU tensor");
+    TORCH_CHECK(t.scalar_type() == torch::kDouble, name, " must be float64");
+    TORCH_CHECK(t.sizes() == torch::IntArrayRef(shape), name, " has shape ", t.sizes());
 }
 
 void check_out(const torch::Tensor& t, const torch::Tensor& like, std::vector<int64_t> shape,
@@ -98,45 +121,87 @@ hipStream_t stream_of(const torch::Tensor& t) {
 
 // Angles2BackboneGPU_forward (reference angles2backbone_interface.cpp:7-26):
 // (input_angles, output_coords, angles_length, A) with A replaced by the
-// forward->backward state (None: no backward follows), plus the look-back
-// scratch, optional status and config.
+// forward->backward checkpoints (None: no backward follows), plus the
+// look-back words (zero before the first call; left zero), optional status
+// and config.
 void forward(torch::Tensor input_angles, torch::Tensor output_coords,
              torch::Tensor angles_length, c10::optional<torch::Tensor> state, torch::Tensor sync,
              c10::optional<torch::Tensor> status, std::vector<int64_t> config) {
     auto [B, L] = check_inputs(input_angles, angles_length);
     check_out(output_coords, input_angles, {B, 9 * (int64_t)L}, "output_coords");
     const tpl_a2b_config cfg = to_config(config);
+    const int dt = dtype_code(input_angles);
     c10::DeviceGuard guard(input_angles.device());
     void* st = nullptr;
     if (state.has_value() && state->defined())
-        st = opt_ptr(*state, tpl_a2b_state_bytes(dtype_code(input_angles), B, L, &cfg), "state",
-                     input_angles);
-    void* sy = opt_ptr(sync, tpl_a2b_sync_bytes(dtype_code(input_angles), B, L, &cfg), "sync", input_angles);
-    check_rc(tpl_a2b_forward(dtype_code(input_angles), input_angles.data_ptr(),
-                             output_coords.data_ptr(), angles_length.data_ptr<int32_t>(), st, sy,
+        st = opt_ptr(*state, tpl_a2b_state_bytes(dt, B, L, &cfg), "state", input_angles);
+    void* sy = opt_ptr(sync, tpl_a2b_sync_bytes(dt, B, L, &cfg), "sync", input_angles);
+    check_rc(tpl_a2b_forward(dt, input_angles.data_ptr(), output_coords.data_ptr(),
+                             angles_length.data_ptr<int32_t>(), st, sy,
                              status_ptr(status, input_angles), B, L, &cfg,
                              stream_of(input_angles)),
              "Angles2BackboneGPU_forward");
 }
 
 // Angles2BackboneGPU_backward (reference angles2backbone_interface.cpp:27-56):
-// (gradInput, gradOutput, input_angles, angles_length, A, dr_dangle) with
-// input_angles -> the forward's output_coords, A -> the forward's state, and
-// no dr_dangle.
-void backward(torch::Tensor grad_input, torch::Tensor grad_output, torch::Tensor output_coords,
-              torch::Tensor angles_length, torch::Tensor state,
+// (gradInput, gradOutput, input_angles, angles_length, A, dr_dangle) with A ->
+// the forward's checkpoints and dr_dangle -> the look-back words.
+void backward(torch::Tensor grad_input, torch::Tensor grad_output, torch::Tensor input_angles,
+              torch::Tensor angles_length, torch::Tensor state, torch::Tensor sync,
               c10::optional<torch::Tensor> status, std::vector<int64_t> config) {
-    auto [B, L] = check_inputs(grad_input, angles_length);
-    check_out(grad_output, grad_input, {B, 9 * (int64_t)L}, "gradOutput");
-    check_out(output_coords, grad_input, {B, 9 * (int64_t)L}, "output_coords");
+    auto [B, L] = check_inputs(input_angles, angles_length);
+    check_out(grad_input, input_angles, {B, 3, L}, "gradInput");
+    check_out(grad_output, input_angles, {B, 9 * (int64_t)L}, "gradOutput");
     const tpl_a2b_config cfg = to_config(config);
-    c10::DeviceGuard guard(grad_input.device());
-    void* st = opt_ptr(state, tpl_a2b_state_bytes(dtype_code(grad_input), B, L, &cfg), "state", grad_input);
-    check_rc(tpl_a2b_backward(dtype_code(grad_input), grad_input.data_ptr(),
-                              grad_output.data_ptr(), output_coords.data_ptr(),
-                              angles_length.data_ptr<int32_t>(), st,
-                              status_ptr(status, grad_input), B, L, &cfg, stream_of(grad_input)),
+    const int dt = dtype_code(input_angles);
+    c10::DeviceGuard guard(input_angles.device());
+    void* st = opt_ptr(state, tpl_a2b_state_bytes(dt, B, L, &cfg), "state", input_angles);
+    void* sy = opt_ptr(sync, tpl_a2b_sync_bytes(dt, B, L, &cfg), "sync", input_angles);
+    check_rc(tpl_a2b_backward(dt, grad_input.data_ptr(), grad_output.data_ptr(),
+                              input_angles.data_ptr(), angles_length.data_ptr<int32_t>(), st, sy,
+                              status_ptr(status, input_angles), B, L, &cfg,
+                              stream_of(input_angles)),
              "Angles2BackboneGPU_backward");
+}
+
+void sync_clear(torch::Tensor sync) {
+    A2B_CHECK_GPU(sync);
+    c10::DeviceGuard guard(sync.device());
+    check_rc(tpl_a2b_sync_clear(sync.data_ptr(), (size_t)sync.nbytes() & ~size_t(3), stream_of(sync)),
+             "sync_clear");
+}
+
+// Angles2BackboneCPU_forward (reference angles2backbone_interface.cpp:60-80):
+// (input_angles, output_coords, angles_length) on the host, float64; the
+// reference's frame buffer A is not needed.  Returns the status bits.
+int64_t cpu_forward(torch::Tensor input_angles, torch::Tensor output_coords,
+                    torch::Tensor angles_length, int64_t threads) {
+    auto [B, L] = check_cpu_inputs(input_angles, angles_length);
+    check_cpu_out(output_coords, {B, 9 * (int64_t)L}, "output_coords");
+    uint32_t status = 0;
+    check_rc(tpl_a2b_cpu_forward(input_angles.data_ptr<double>(), output_coords.data_ptr<double>(),
+                                 angles_length.data_ptr<int32_t>(), &status, B, L, (int)threads),
+             "Angles2BackboneCPU_forward");
+    return status;
+}
+
+// Angles2BackboneCPU_backward (reference angles2backbone_interface.cpp:83-111):
+// (gradInput, gradOutput, output_coords, angles_length): the forward's
+// coordinates stand where input_angles and A stood; no dr_dangle.  Returns
+// the status bits.
+int64_t cpu_backward(torch::Tensor grad_input, torch::Tensor grad_output,
+                     torch::Tensor output_coords, torch::Tensor angles_length, int64_t threads) {
+    TORCH_CHECK(!grad_input.is_cuda(), "gradInput must be a CPU tensor");
+    TORCH_CHECK(grad_input.dim() == 3, "gradInput must be [batch, 3, max_length]");
+    auto [B, L] = check_cpu_inputs(grad_input, angles_length);
+    check_cpu_out(grad_output, {B, 9 * (int64_t)L}, "gradOutput");
+    check_cpu_out(output_coords, {B, 9 * (int64_t)L}, "output_coords");
+    uint32_t status = 0;
+    check_rc(tpl_a2b_cpu_backward(grad_input.data_ptr<double>(), grad_output.data_ptr<double>(),
+                                  output_coords.data_ptr<double>(),
+                                  angles_length.data_ptr<int32_t>(), &status, B, L, (int)threads),
+             "Angles2BackboneCPU_backward");
+    return status;
 }
 
 void forward_plain(torch::Tensor input_angles, torch::Tensor output_coords,
@@ -221,7 +286,8 @@ std::tuple<std::vector<int64_t>, std::vector<double>> tune(
 // Times `steps` back-to-back steps (forward then backward launch) on the
 // current stream, rotating over the buffer sets, after `warmup` untimed steps.
 // Each set is [angles, lengths, grad_coords, coords, grad_angles, state, sync]
-// (or, for plain=true, [..., frames, unused]).  Returns (event ms, wall ms).
+// (or, for plain=true, [..., frames, unused]); sync must be zero before the
+// first call (the calls leave it zero).  Returns (event ms, wall ms).
 std::tuple<double, double> bench_steps(std::vector<std::vector<torch::Tensor>> sets,
                                        std::vector<int64_t> config, int64_t steps,
                                        int64_t warmup, bool plain) {
@@ -246,9 +312,9 @@ std::tuple<double, double> bench_steps(std::vector<std::vector<torch::Tensor>> s
             rc = tpl_a2b_forward(dt, s[0].data_ptr(), s[3].data_ptr(), s[1].data_ptr<int32_t>(),
                                  s[5].data_ptr(), s[6].data_ptr(), nullptr, B, L, &cfg, stream);
             if (rc == TPL_A2B_OK)
-                rc = tpl_a2b_backward(dt, s[4].data_ptr(), s[2].data_ptr(), s[3].data_ptr(),
-                                      s[1].data_ptr<int32_t>(), s[5].data_ptr(), nullptr, B, L,
-                                      &cfg, stream);
+                rc = tpl_a2b_backward(dt, s[4].data_ptr(), s[2].data_ptr(), s[0].data_ptr(),
+                                      s[1].data_ptr<int32_t>(), s[5].data_ptr(), s[6].data_ptr(),
+                                      nullptr, B, L, &cfg, stream);
         }
         check_rc(rc, "bench_steps");
     };
@@ -322,9 +388,9 @@ std::tuple<double, double, int64_t> bench_graph(std::vector<std::vector<torch::T
             rc = tpl_a2b_forward(dt, s[0].data_ptr(), s[3].data_ptr(), s[1].data_ptr<int32_t>(),
                                  s[5].data_ptr(), s[6].data_ptr(), nullptr, B, L, &cfg, cs);
             if (rc == TPL_A2B_OK)
-                rc = tpl_a2b_backward(dt, s[4].data_ptr(), s[2].data_ptr(), s[3].data_ptr(),
-                                      s[1].data_ptr<int32_t>(), s[5].data_ptr(), nullptr, B, L,
-                                      &cfg, cs);
+                rc = tpl_a2b_backward(dt, s[4].data_ptr(), s[2].data_ptr(), s[0].data_ptr(),
+                                      s[1].data_ptr<int32_t>(), s[5].data_ptr(), s[6].data_ptr(),
+                                      nullptr, B, L, &cfg, cs);
             if (rc != TPL_A2B_OK) break;
         }
         hipGraph_t g = nullptr;
@@ -376,11 +442,14 @@ std::tuple<double, double, int64_t> bench_graph(std::vector<std::vector<torch::T
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.doc() = "Angles2Backbone for MI355X: tensor glue over the tpl_a2b C ABI";
-    m.def("Angles2BackboneGPU_forward", &forward, "Angles2Backbone gpu forward (scan)");
+    m.def("Angles2BackboneGPU_forward", &forward, "Angles2Backbone gpu forward (chunk scan)");
     m.def("Angles2BackboneGPU_backward", &backward, "Angles2Backbone gpu backward (O(L) adjoint)");
+    m.def("Angles2BackboneCPU_forward", &cpu_forward, "Angles2Backbone cpu forward (float64)");
+    m.def("Angles2BackboneCPU_backward", &cpu_backward, "Angles2Backbone cpu backward (float64, O(L) adjoint)");
     m.def("Angles2BackboneGPU_forward_plain", &forward_plain, "plain serial forward");
     m.def("Angles2BackboneGPU_backward_plain", &backward_plain, "plain O(L^2) backward");
     m.def("workspace_bytes", &workspace_bytes, "(state_bytes, sync_bytes) for a config");
+    m.def("sync_clear", &sync_clear, "zero a look-back buffer on the current stream");
     m.def("plain_frames_bytes", &plain_frames_bytes, "frame buffer bytes of the plain form");
     m.def("default_config", &default_config, "untuned config for (batch, max_length)");
     m.def("tune", &tune, "time candidate configs; returns (best, ms per candidate)");

@@ -173,8 +173,10 @@ __global__ void __launch_bounds__(kThreads) rmsd_fwd(const T* __restrict__ x, co
     rmsd_math::u_from_q(eig, U);
     // r2 - 2|lambda| is >= 0 in exact arithmetic; for (nearly) identical
     // structures its FP64 rounding (~1e-16 r2) can make it negative, which
-    // the reference's formula turns into NaN.  Clamped at 0 here.
-    const double rmsd = sqrt((fmax(r2 - 2.0 * fabs(lam), 0.0) + kEpsLimit) / nd);
+    // the reference's formula turns into NaN.  Clamped at 0 here; a NaN (from
+    // NaN coordinates) stays NaN and is reported.
+    const double dev = r2 - 2.0 * fabs(lam);
+    const double rmsd = sqrt(((dev < 0.0 ? 0.0 : dev) + kEpsLimit) / nd);
     rmsd_out[b] = (T)rmsd;
 #pragma unroll
     for (int j = 0; j < 3; ++j)

@@ -239,35 +239,45 @@ def probe(args):
     for i in range(args.steps):
         A, N, G, X, GA, st = sets[i % 6]
         ext.Angles2BackboneGPU_backward(GA, G, A, N, st, sy, None, cfg)
-    src = torch.ones(256 << 20, dtype=torch.uint8, device=dev)
+    # Calibration, the last 5 dispatches: an elementwise kernel that reads
+    # 256 MiB and writes 256 MiB (float32, 64 Mi elements).
+    src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
+    torch.cuda.synchronize()
     for _ in range(5):
-        dst.copy_(src)
+        torch.add(src, 1.0, out=dst)
     torch.cuda.synchronize()
     print("probe done", flush=True)
 
 
-def _pmc_medians(outdir, counter):
-    """{kernel name: median counter value over its dispatches} from a
-    rocprofv3 --output-format csv run."""
+def _pmc_medians(outdir, counter, last=5):
+    """({kernel name: median counter value over its dispatches}, median over
+    the last `last` dispatches) from a rocprofv3 --output-format csv run."""
     import csv
     import glob
     import statistics
-    vals = {}
+    rows = []
     for f in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
         with open(f, newline="") as fh:
             for row in csv.DictReader(fh):
-                if row.get("Counter_Name") != counter:
-                    continue
-                vals.setdefault(row.get("Kernel_Name", ""), []).append(float(row["Counter_Value"]))
-    return {k: statistics.median(v) for k, v in vals.items()}
+                if row.get("Counter_Name") == counter:
+                    rows.append((int(row.get("Dispatch_Id", 0)), row.get("Kernel_Name", ""),
+                                 float(row["Counter_Value"])))
+    rows.sort()
+    vals = {}
+    for _, k, v in rows:
+        vals.setdefault(k, []).append(v)
+    tail = [v for _, _, v in rows[-last:]]
+    return ({k: statistics.median(v) for k, v in vals.items()},
+            statistics.median(tail) if len(tail) == last else None)
 
 
 def traffic(args, cfg):
     """HBM bytes per launch of the forward and backward kernels: two
     rocprofv3 --pmc runs of the probe mode (FETCH_SIZE, then WRITE_SIZE, in
-    KiB per dispatch), each scaled by what it counts for a 256 MiB device copy
-    in the same run.  None if rocprofv3 is not there or a run fails."""
+    KiB per dispatch), each scaled by what it counts for the probe's last
+    dispatches, an elementwise kernel that reads and writes 256 MiB.  None if
+    rocprofv3 is not there or a run fails."""
     import shutil
     import subprocess
     import tempfile
@@ -289,15 +299,13 @@ def traffic(args, cfg):
                 return None
             if "probe done" not in r.stdout:
                 return None
-            med = _pmc_medians(od, counter)
+            med, calib = _pmc_medians(od, counter)
             fwd = [v for k, v in med.items() if "a2b_fwd_chunk" in k]
             bwd = [v for k, v in med.items() if "a2b_bwd_chunk" in k]
-            cpy = [v for k, v in med.items() if "a2b_" not in k and ("copy" in k.lower() or "elementwise" in k.lower())]
-            if not (fwd and bwd and cpy):
+            if not (fwd and bwd and calib):
                 return None
-            copy_kib = 262144.0
-            scale = copy_kib / max(cpy)
-            out[counter] = {"forward_kib": fwd[0], "backward_kib": bwd[0], "copy_kib": max(cpy), "scale": scale}
+            scale = 262144.0 / calib  # the calibration kernel reads and writes 256 MiB
+            out[counter] = {"forward_kib": fwd[0], "backward_kib": bwd[0], "copy_kib": calib, "scale": scale}
     res = {"fetch_scale": out["FETCH_SIZE"]["scale"], "write_scale": out["WRITE_SIZE"]["scale"]}
     for k in ("forward", "backward"):
         res[k] = 1024.0 * (out["FETCH_SIZE"][k + "_kib"] * out["FETCH_SIZE"]["scale"] +

@@ -100,8 +100,9 @@ int64_t tpl_a2b_chunk_residues(const tpl_a2b_config *cfg, int backward);
  * in FP32). */
 size_t tpl_a2b_state_bytes(int dtype, int batch, int angles_stride, const tpl_a2b_config *cfg);
 
-/* Bytes of the look-back words of forward and backward (per-chain tickets,
- * done counters and flags, then the payload they guard).  The buffer must be
+/* Bytes of the look-back words of forward and backward (per-chain tickets
+ * and done counters, then one record of 64-bit words per chunk, each word
+ * marking itself written).  The buffer must be
  * all zero before its first use (tpl_a2b_sync_clear, or any zero fill); every
  * call that ends without TPL_A2B_STATUS_TIMEOUT returns all it wrote to zero,
  * so one buffer serves every later call on the same stream, of any shape and

@@ -37,11 +37,11 @@ def _state(B, L, rb):
 
 
 def _sync(B, L, rf, rb):
-    # per direction: tickets [B], done [B], flags [B][nc] (uint32), then the
-    # payload [B][nc] of 24 doubles (forward frames) or 12 (backward sums)
+    # per direction: tickets [B], done [B] (uint32), then one record per
+    # chunk [B][nc] of 12 64-bit words (forward frames) or 6 (backward sums)
     def region(nc, pay):
-        return _align(_align(8 * B + 4 * B * nc) + B * nc * pay * 8)
-    return region(-(-L // (64 * rf)), 24) + region(-(-L // (64 * rb)), 12)
+        return _align(_align(8 * B) + B * nc * pay * 8)
+    return region(-(-L // (64 * rf)), 12) + region(-(-L // (64 * rb)), 6)
 
 
 def test_config_validation_and_sizes(lib):

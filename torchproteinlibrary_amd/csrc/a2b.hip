@@ -117,7 +117,6 @@ __device__ __forceinline__ void frame_rmul_to(const double f[kF], const double* 
 }
 
 
-// ------------------------------------------------------------ cross-lane ops
 __device__ __forceinline__ double dbl_from(int lo, int hi) {
     return __hiloint2double(hi, lo);
 }
@@ -221,46 +220,29 @@ __device__ __forceinline__ void set_status(uint32_t* status, uint32_t bits) {
 // acquire fence writes back or invalidates the whole L2 of the XCD (the L2s of
 // the eight XCDs are not coherent with each other), which the streaming
 // output of every other workgroup makes expensive.  So the look-back never
-// fences at agent scope:
-//   * flags, tickets and payload words move with relaxed agent-scope atomic
-//     loads/stores, which are coherent at device scope by themselves;
-//   * a publisher orders its payload stores before the flag store with
-//     s_waitcnt vmcnt(0) (vector memory operations complete), plus a
-//     workgroup-scope fence that stops the compiler from reordering;
-//   * a reader waits for its flag load the same way before loading payload.
+// fences at agent scope: tickets, done counters and record words move with
+// relaxed agent-scope atomics, which are coherent at device scope by
+// themselves, and a record needs no order between its words (each word
+// shows by itself whether it is written).  The one ordering point, a wave's
+// record stores before its done count, waits with s_waitcnt vmcnt(0)
+// (vector memory operations complete), plus a workgroup-scope fence that
+// stops the compiler from reordering.
 constexpr int kWaitVmcnt0 = 0 | (0x7 << 4) | (0xf << 8);  // vmcnt(0); expcnt, lgkmcnt: no wait
 
-__device__ __forceinline__ uint32_t flag_load(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ void word_store(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// Publishes flag value v after every earlier store of this thread completed.
-__device__ __forceinline__ void flag_store(uint32_t* p, uint32_t v) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-    word_store(p, v);
+__device__ __forceinline__ void record_store(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// After a flag was seen set: later loads are issued after the flag load.
-__device__ __forceinline__ void after_flag() {
-    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-__device__ __forceinline__ void payload_store(double* p, double v) {
-    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p),
-                       static_cast<unsigned long long>(__double_as_longlong(v)), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double payload_load(const double* p) {
-    return __longlong_as_double(static_cast<long long>(__hip_atomic_load(
-        reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+__device__ __forceinline__ uint64_t record_load(const uint64_t* p) {
+    return __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint32_t atomic_inc(uint32_t* p) {
     return __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-
-constexpr uint32_t kFlagAggregate = 1, kFlagInclusive = 2;
 
 // ------------------------------------------------------------------ lane I/O
 // A lane moves a run of N consecutive elements of T between global memory
@@ -460,14 +442,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Work decomposition.  A chunk is C = 64*R consecutive residues of one chain
 // (lane l owns residues r0 + l*R .. r0 + l*R + R-1, i.e. chain atoms
 // 3(r0 + lR) .. 3(r0 + lR) + 3R - 1).  One workgroup of one wave takes one
-// slot (b, s), s < nc = ceil(L / C).  Slots are numbered so that
-//   * workgroup w runs on XCD w mod 8 (the dispatcher's round-robin) and
-//     takes the chains b = x, x + 8, x + 16, ... of stripe x = w mod 8: all
-//     slots of a chain share one XCD, and with it one L2 for the chain's
-//     look-back words;
-//   * inside a stripe, slot s of every chain comes before slot s + 1 of any
-//     chain (chunk-major), so the first waves are the ones with work and
-//     each XCD sees the same mix of lengths.
+// slot (b, s), s < nc = ceil(L / C).  Workgroup w runs on XCD w mod 8 (the
+// dispatcher's round-robin) and takes a chain b = x, x + 8, x + 16, ... of
+// stripe x = w mod 8, so all slots of a chain share one XCD; chunk_slot
+// below says which slot.
 // With n residues a chain has nreal = ceil(n / C) chunks.  Slots s >= nreal
 // only write the zeros of their span of the outputs.  The nreal slots below
 // take the chain's chunks from a per-chain ticket counter, ascending in the
@@ -475,14 +453,23 @@ __device__ __forceinline__ void wave_lds_sync() {
 // whose waves are already running, whatever order workgroups start in.  A
 // chain of one chunk takes no ticket and publishes nothing.
 //
-// Look-back words live in the caller's sync buffer and clean up after
-// themselves: each wave that took a ticket counts itself done after its last
-// flag access, and the last one of its chain returns the chain's ticket,
-// done counter and flags to zero.  The buffer is zeroed once when it is
-// allocated (tpl_a2b_sync_clear) and stays zero between calls.
+// Look-back records (a chunk's aggregate, or a group boundary's inclusive
+// value; one record per chunk) live in the caller's sync buffer and carry
+// their own "written" mark: a record word is the FP64 bit pattern of its
+// value with bit 0 set, so no written word is zero, and a reader that sees
+// every word of a record nonzero has the whole record (each word is written
+// once per call and moves as one 64-bit access).  Setting bit 0 changes a
+// value by at most one unit in the last place, the same way on every run.
+// So a publisher issues its stores and goes on, and a reader polls the
+// record itself: one memory round trip between them, and no flag.
+// The buffer cleans up after itself: each wave that took a ticket counts
+// itself done after its last record access, and the last one of its chain
+// returns the chain's ticket, done counter and records to zero.  The buffer
+// is zeroed once when it is allocated (tpl_a2b_sync_clear) and stays zero
+// between calls.
 
-constexpr int kFwdPay = 2 * kF;  // per chunk: aggregate frame, inclusive frame
-constexpr int kBwdPay = 12;      // per chunk: aggregate (T, F), inclusive (T, F)
+constexpr int kFwdPay = kF;  // per chunk: aggregate or inclusive frame
+constexpr int kBwdPay = 6;   // per chunk: aggregate or inclusive (T, F)
 
 struct ChunkArgs {
     const void* angles;      // [B][3][L]
@@ -495,31 +482,115 @@ struct ChunkArgs {
                              // residue kK), written for 0 < kK < n; null: none
     uint32_t* tickets;       // [B]
     uint32_t* done;          // [B]
-    uint32_t* flags;         // [B][nc]
-    double* pay;             // [B][nc][kFwdPay] or [B][nc][kBwdPay]
+    uint64_t* pay;           // [B][nc][kFwdPay] or [B][nc][kBwdPay] record words
     int B, L;
     int nc;                  // chunk slots per chain
     int nk, K;               // checkpoints per chain, residues between checkpoints
     int bx;                  // chains per stripe: ceil(B / 8)
 };
 
-// The chain length as used: lengths outside [0, L] are clamped and reported.
-__device__ __forceinline__ int clamp_length(const ChunkArgs& a, int n, bool report) {
-    if (n < 0 || n > a.L) {
-        if (report) set_status(a.status, TPL_A2B_STATUS_LENGTH);
-        n = n < 0 ? 0 : a.L;
-    }
-    return n;
+// The slot of this workgroup.  Stripe x = blockIdx.x mod 8 holds the chains
+// b = x + 8i, i < m (m = the stripe's chains); position j = blockIdx.x / 8.
+//   * Positions j < T, T = sum over the stripe of nreal_i, are the chunks
+//     that hold residues, in rows: row r lists, in stripe order, the chains
+//     with nreal_i > r (cnt_r of them), and a wave of row r takes its chain's
+//     r-th chunk in processing order.
+//   * Positions T <= j < m*nc are the padding slots, in rows as well: row r
+//     lists the chains with nreal_i <= r, whose slot r is past the chain.
+// So the waves with work come first in every stripe, and the dispatcher,
+// placing workgroups in order, spreads them over every SIMD; only then come
+// the waves that write zeros.  The row counts are popcounts of ballots over
+// the stripe's lengths (read once per wave, at most kMaxStripe of them) and
+// the row is found by a scalar walk over them, so this costs one load round
+// and about 10 instructions per row.  With more than kMaxRows rows or
+// kMaxStripe chains, the slots go in slot order instead (row r, chain i),
+// the chains of row r rotated by r * kRowTurn so that one chain's chunks do
+// not follow each other at the period of the dispatcher's placement.
+constexpr int kStripeLanes = 8;  // chains per lane
+constexpr int kMaxStripe = 64 * kStripeLanes;
+constexpr int kMaxRows = 32;
+constexpr int kRowTurn = 37;
+
+struct Slot {
+    int b;      // chain
+    int r;      // row: chunk rank in processing order (real) or padding slot
+    int n;      // chain length, clamped to [0, L]
+    int nreal;  // chunks that hold residues
+    bool real;
+    bool bad;   // lengths[b] was outside [0, L]
+};
+
+__device__ __forceinline__ Slot make_slot(const ChunkArgs& a, int C, int b, int r) {
+    const int raw = a.lengths[b];
+    const int n = raw < 0 ? 0 : (raw > a.L ? a.L : raw);
+    const int nreal = (n + C - 1) / C;
+    return Slot{b, r, n, nreal, r < nreal, raw != n};
 }
 
-// Slot of this workgroup; false for the padding slots of the last stripes.
-__device__ __forceinline__ bool chunk_slot(const ChunkArgs& a, int* b, int* s) {
-    const int w = (int)blockIdx.x;
-    const int j = w >> 3;
-    const int sl = j / a.bx;
-    *s = sl;
-    *b = (w & 7) + 8 * (j - sl * a.bx);
-    return *b < a.B;
+__device__ __forceinline__ bool chunk_slot(const ChunkArgs& a, int C, Slot* out) {
+    const int lane = threadIdx.x;
+    const int x = (int)(blockIdx.x & 7), j = (int)(blockIdx.x >> 3);
+    const int m = a.B > x ? (a.B - x + 7) / 8 : 0;
+    if (j >= m * a.nc) return false;
+    if (m > kMaxStripe || a.nc > kMaxRows) {  // slot order, rows rotated
+        const int r = j / m;
+        const int i = (j - r * m + (int)(((int64_t)r * kRowTurn) % m)) % m;
+        *out = make_slot(a, C, x + 8 * i, r);
+        return true;
+    }
+    const int kk = (m + 63) / 64;  // lengths per lane in use (uniform)
+    int nr[kStripeLanes];
+#pragma unroll
+    for (int k = 0; k < kStripeLanes; ++k) {
+        const int i = lane + 64 * k;
+        int n = 0;
+        if (k < kk && i < m) {
+            n = a.lengths[x + 8 * i];
+            n = n < 0 ? 0 : (n > a.L ? a.L : n);
+        }
+        nr[k] = (n + C - 1) / C;
+    }
+    // cnt_r = #{i : nreal_i > r}; real rows first, then padding rows of
+    // m - cnt_r slots each.
+    int row_start = 0, r = -1, start = 0;
+    bool real = true;
+    for (int pass = 0; pass < 2 && r < 0; ++pass) {
+        for (int rr = 0; rr < a.nc; ++rr) {
+            int cnt = 0;
+#pragma unroll
+            for (int k = 0; k < kStripeLanes; ++k)
+                if (k < kk) cnt += __builtin_popcountll(__builtin_amdgcn_ballot_w64(nr[k] > rr));
+            const int size = pass == 0 ? cnt : m - cnt;
+            if (j < row_start + size) {
+                r = rr;
+                start = row_start;
+                real = pass == 0;
+                break;
+            }
+            row_start += size;
+        }
+    }
+    int q = j - start;  // rank within the row
+    int b = x;
+    bool found = false;
+#pragma unroll
+    for (int k = 0; k < kStripeLanes; ++k) {
+        if (k >= kk) break;
+        const bool in = (lane + 64 * k < m) && (real ? nr[k] > r : nr[k] <= r);
+        const uint64_t mask = __builtin_amdgcn_ballot_w64(in);
+        const int cnt = __builtin_popcountll(mask);
+        if (!found && q < cnt) {
+            const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+            const uint64_t hit = __builtin_amdgcn_ballot_w64(in && below == q);
+            b = x + 8 * (__builtin_ctzll(hit) + 64 * k);
+            found = true;
+        } else if (!found) {
+            q -= cnt;
+        }
+    }
+    *out = make_slot(a, C, b, r);
+    return true;
 }
 
 __device__ __forceinline__ int take_ticket(uint32_t* t) {
@@ -529,21 +600,25 @@ __device__ __forceinline__ int take_ticket(uint32_t* t) {
 }
 
 // Called once by every wave that took a ticket of chain b, after its last
-// access to the chain's flags and payload.  The last of the nreal waves
-// returns the chain's words and payload (PAY doubles per chunk) to zero:
-// every other wave has finished with them, and the next call, of whatever
-// shape, finds the buffer as it was before this one (all zero).
-template <int PAY>
-__device__ __forceinline__ void chain_done(const ChunkArgs& a, int b, int nreal) {
+// access to the chain's records, in two halves so that the counter's round
+// trip overlaps the wave's own stores: chain_done_issue counts the wave done
+// (after its record stores have landed) and returns the counter's old value,
+// chain_done_finish lets the last of the nreal waves return the chain's
+// records (PAY words per chunk), ticket and counter to zero: every other wave
+// has finished with them, and the next call, of whatever shape, finds the
+// buffer as it was before this one (all zero).
+__device__ __forceinline__ uint32_t chain_done_issue(const ChunkArgs& a, int b) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);  // this wave's flag stores have landed
-    uint32_t last = 0;
-    if (threadIdx.x == 0) last = atomic_inc(a.done + b) == (uint32_t)(nreal - 1) ? 1u : 0u;
-    if (__builtin_amdgcn_readfirstlane(last)) {
-        uint32_t* fl = a.flags + (size_t)b * a.nc;
-        double* pay = a.pay + (size_t)b * a.nc * PAY;
-        for (int j = threadIdx.x; j < nreal; j += 64) word_store(fl + j, 0u);
-        for (int j = threadIdx.x; j < nreal * PAY; j += 64) payload_store(pay + j, 0.0);
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);  // this wave's record stores have landed
+    uint32_t old = 0;
+    if (threadIdx.x == 0) old = atomic_inc(a.done + b);
+    return old;
+}
+template <int PAY>
+__device__ __forceinline__ void chain_done_finish(const ChunkArgs& a, int b, int nreal, uint32_t old) {
+    if (__builtin_amdgcn_readfirstlane(old) == (uint32_t)(nreal - 1)) {
+        uint64_t* pay = a.pay + (size_t)b * a.nc * PAY;
+        for (int j = threadIdx.x; j < nreal * PAY; j += 64) record_store(pay + j, 0ull);
         if (threadIdx.x == 0) {
             word_store(a.tickets + b, 0u);
             word_store(a.done + b, 0u);
@@ -563,15 +638,36 @@ __device__ __forceinline__ int chain_ticket(const ChunkArgs& a, int b, int nreal
     return t;
 }
 
-// Lane 0 publishes the uniform values v[0..N) into payload slot p and then
-// sets the flag (payload first: readers load it only after seeing the flag).
+// Lane 0 publishes the uniform values v[0..N) as the record at p.
 template <int N>
-__device__ __forceinline__ void publish(double* p, uint32_t* flag, const double (&v)[N], uint32_t f) {
+__device__ __forceinline__ void publish(uint64_t* p, const double (&v)[N]) {
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int e = 0; e < N; ++e) payload_store(p + e, v[e]);
-        flag_store(flag, f);
+        for (int e = 0; e < N; ++e)
+            record_store(p + e, static_cast<uint64_t>(__double_as_longlong(v[e])) | 1ull);
     }
+}
+
+// Waits until the record at p (read by every valid lane) is written and
+// returns it in v (bounded: false after kSpinLimit polls, with
+// TPL_A2B_STATUS_TIMEOUT reported).  Invalid lanes read nothing.
+constexpr int kSpinLimit = 1 << 21;
+template <int N>
+__device__ __forceinline__ bool wait_record(const uint64_t* p, bool valid, double (&v)[N],
+                                            uint32_t* status) {
+    for (int spins = 0; spins < kSpinLimit; ++spins) {
+        bool ok = true;
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < N; ++e) v[e] = __longlong_as_double(static_cast<long long>(record_load(p + e)));
+#pragma unroll
+            for (int e = 0; e < N; ++e) ok = ok && __double_as_longlong(v[e]) != 0;
+        }
+        if (__builtin_amdgcn_ballot_w64(valid && !ok) == 0) return true;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if (threadIdx.x == 0) set_status(status, TPL_A2B_STATUS_TIMEOUT);
+    return false;
 }
 
 // Look-back in fixed groups of 64 chunks (0-63, 64-127, ...), so that every
@@ -587,71 +683,39 @@ __device__ __forceinline__ void publish(double* p, uint32_t* flag, const double 
 //     group's first chunk (where the chain has it).
 // Aggregates are published right after a chunk's own scan, so a chunk never
 // waits on a chain of look-backs inside its group; only the group-boundary
-// chunks publish inclusive values, and those form one chain per 64 chunks.
-// A lane of the window reads one chunk's aggregate; the window's product or
-// sum is one wave scan in lane order (identity or zero in unused lanes).
-
-// Waits until the flag of every valid lane is set (bounded); false after
-// kSpinLimit polls, with TPL_A2B_STATUS_TIMEOUT reported.
-constexpr int kSpinLimit = 1 << 21;
-__device__ __forceinline__ bool wait_window(const uint32_t* flag, bool valid, uint32_t* status) {
-    uint32_t f = 0;
-    for (int spins = 0; spins < kSpinLimit; ++spins) {
-        if (valid && f == 0) f = flag_load(flag);
-        if (__builtin_amdgcn_ballot_w64(valid && f == 0) == 0) {
-            after_flag();
-            return true;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-    if (threadIdx.x == 0) set_status(status, TPL_A2B_STATUS_TIMEOUT);
-    return false;
-}
-
-// Waits until one flag (read by every lane) is kFlagInclusive.
-__device__ __forceinline__ bool wait_inclusive(const uint32_t* flag, uint32_t* status) {
-    for (int spins = 0; spins < kSpinLimit; ++spins) {
-        if (__builtin_amdgcn_readfirstlane(flag_load(flag)) == kFlagInclusive) {
-            after_flag();
-            return true;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-    if (threadIdx.x == 0) set_status(status, TPL_A2B_STATUS_TIMEOUT);
-    return false;
-}
+// chunks publish inclusive values (in the record their aggregate would use:
+// no chunk reads a boundary's aggregate), and those form one chain per 64
+// chunks.  Lane k of the window reads one chunk's aggregate; the window's
+// product or sum is one wave scan in lane order (identity or zero in unused
+// lanes), skipped when the window is one chunk.
 
 // Forward: P <- the exclusive prefix of chunk c >= 1 of chain b (P enters
 // as I; uniform over the wave).
 __device__ void fwd_lookback(const ChunkArgs& a, int b, int c, double P[kF]) {
     const int lane = threadIdx.x;
-    const uint32_t* fl = a.flags + (size_t)b * a.nc;
-    const double* pay = a.pay + (size_t)b * a.nc * kFwdPay;
+    const uint64_t* pay = a.pay + (size_t)b * a.nc * kFwdPay;
     const int lo = c & ~63;
     if (lo > 0) {  // the previous group's inclusive prefix
-        if (!wait_inclusive(fl + lo - 1, a.status)) return;
-        const double* p = pay + (size_t)(lo - 1) * kFwdPay + kF;
+        double I[kF];
+        if (!wait_record<kF>(pay + (size_t)(lo - 1) * kFwdPay, true, I, a.status)) return;
 #pragma unroll
-        for (int e = 0; e < kF; ++e) P[e] = payload_load(p + e);
+        for (int e = 0; e < kF; ++e) P[e] = I[e];
     }
     if (c == lo) return;
     const int j = lo + lane;
     const bool valid = j < c;
-    if (!wait_window(fl + (valid ? j : lo), valid, a.status)) return;
-    double v[kF];
-    if (valid) {
-        const double* p = pay + (size_t)j * kFwdPay;
-#pragma unroll
-        for (int e = 0; e < kF; ++e) v[e] = payload_load(p + e);
-    } else {
+    double v[kF] = {};
+    if (!wait_record<kF>(pay + (size_t)(valid ? j : lo) * kFwdPay, valid, v, a.status)) return;
+    if (!valid) {
 #pragma unroll
         for (int e = 0; e < kF; ++e) v[e] = ident(e);
     }
-    wave_scan_frames(v);  // lane 63: A_lo * ... * A_{c-1}
+    if (c - lo > 1) wave_scan_frames(v);  // lane 63: A_lo * ... * A_{c-1}
+    const int src = c - lo > 1 ? 63 : 0;  // a window of one chunk: lane 0
     double W[kF];
 #pragma unroll
-    for (int e = 0; e < kF; ++e) W[e] = read_lane(v[e], 63);
-    frame_rmul_to(P, W, v);  // v = P * W (P in registers, W uniform)
+    for (int e = 0; e < kF; ++e) W[e] = read_lane(v[e], src);
+    frame_rmul_to(P, W, v);  // v = P * W
 #pragma unroll
     for (int e = 0; e < kF; ++e) P[e] = v[e];
 }
@@ -660,26 +724,21 @@ __device__ void fwd_lookback(const ChunkArgs& a, int b, int c, double P[kF]) {
 // chain b (nreal chunks); carry enters as 0.
 __device__ void bwd_lookback(const ChunkArgs& a, int b, int c, int nreal, double carry[6]) {
     const int lane = threadIdx.x;
-    const uint32_t* fl = a.flags + (size_t)b * a.nc;
-    const double* pay = a.pay + (size_t)b * a.nc * kBwdPay;
+    const uint64_t* pay = a.pay + (size_t)b * a.nc * kBwdPay;
     const int lo = c & ~63, hi = min(lo + 64, nreal);  // c's group, clipped to the chain
-    const int j = lo + lane;
-    const bool valid = j > c && j < hi;
-    if (!wait_window(fl + (valid ? j : c), valid, a.status)) return;
-    double v[6] = {0, 0, 0, 0, 0, 0};
-    if (valid) {
-        const double* p = pay + (size_t)j * kBwdPay;
+    const int j = c + 1 + lane;  // lane k reads chunk c + 1 + k
+    const bool valid = j < hi;
+    double v[6] = {};  // zero in unused lanes
+    if (!wait_record<6>(pay + (size_t)(valid ? j : c) * kBwdPay, valid, v, a.status)) return;
+    if (hi - c - 1 > 1) wave_scan_sum<6>(v);  // lane 63: the window's sum
+    const int src = hi - c - 1 > 1 ? 63 : 0;   // a window of one chunk: lane 0
 #pragma unroll
-        for (int e = 0; e < 6; ++e) v[e] = payload_load(p + e);
-    }
-    wave_scan_sum<6>(v);
-#pragma unroll
-    for (int e = 0; e < 6; ++e) carry[e] = read_lane(v[e], 63);
+    for (int e = 0; e < 6; ++e) carry[e] = read_lane(v[e], src);
     if (hi < nreal) {  // the next group's inclusive suffix
-        if (!wait_inclusive(fl + hi, a.status)) return;
-        const double* p = pay + (size_t)hi * kBwdPay + 6;
+        double in[6];
+        if (!wait_record<6>(pay + (size_t)hi * kBwdPay, true, in, a.status)) return;
 #pragma unroll
-        for (int e = 0; e < 6; ++e) carry[e] += payload_load(p + e);
+        for (int e = 0; e < 6; ++e) carry[e] += in[e];
     }
 }
 
@@ -707,13 +766,12 @@ __global__ void __launch_bounds__(64) a2b_fwd_chunk(ChunkArgs a) {
     T* s_x = reinterpret_cast<T*>(s_dyn);
     T* s_l = s_x + threadIdx.x * E9;
     const int lane = threadIdx.x;
-    int b, s;
-    if (!chunk_slot(a, &b, &s)) return;
-    const int L = a.L;
-    const int n = clamp_length(a, a.lengths[b], s == 0 && lane == 0);
-    const int nreal = (n + C - 1) / C;
+    Slot sl;
+    if (!chunk_slot(a, C, &sl)) return;
+    const int L = a.L, b = sl.b, n = sl.n, nreal = sl.nreal, s = sl.r;
+    if (sl.bad && s == 0 && lane == 0) set_status(a.status, TPL_A2B_STATUS_LENGTH);
     T* xrow = static_cast<T*>(a.coords) + (size_t)b * 9 * L;
-    if (s >= nreal) {  // wholly past the chain: its zeros
+    if (!sl.real) {  // slot s is wholly past the chain: its zeros
         const int f0 = 9 * s * C;
         wave_store_run<T>(xrow + f0, min(9 * C, 9 * L - f0), 0, s_x, lane);
         return;
@@ -763,11 +821,10 @@ __global__ void __launch_bounds__(64) a2b_fwd_chunk(ChunkArgs a) {
     // of its group; the last chunk of a group publishes its inclusive prefix
     // instead, for the next group; the chain's last chunk publishes nothing.
     if (nreal > 1) {
-        double* my = a.pay + ((size_t)b * a.nc + c) * kFwdPay;
-        uint32_t* myf = a.flags + (size_t)b * a.nc + c;
+        uint64_t* my = a.pay + ((size_t)b * a.nc + c) * kFwdPay;
         const bool last = c == nreal - 1;
         const bool boundary = (c & 63) == 63;
-        if (!last && !boundary) publish<kF>(my, myf, agg, kFlagAggregate);
+        if (!last && !boundary) publish<kF>(my, agg);
         if (c > 0) {
             double P[kF];
             frame_identity(P);
@@ -776,10 +833,9 @@ __global__ void __launch_bounds__(64) a2b_fwd_chunk(ChunkArgs a) {
             if (!last && boundary) {
                 double incl[kF];
                 frame_rmul_to(P, agg, incl);
-                publish<kF>(my + kF, myf, incl, kFlagInclusive);
+                publish<kF>(my, incl);
             }
         }
-        chain_done<kFwdPay>(a, b, nreal);
     }
 
     // Checkpoint at a backward-chunk boundary inside the chain.
@@ -813,10 +869,12 @@ __global__ void __launch_bounds__(64) a2b_fwd_chunk(ChunkArgs a) {
         }
         if (!__builtin_isfinite(chk)) set_status(a.status, TPL_A2B_STATUS_NAN);
     }
+    const uint32_t done_old = nreal > 1 ? chain_done_issue(a, b) : 0u;
     wave_lds_sync();
     const int g0 = 9 * r0;
     const int wcount = min(9 * C, 9 * L - g0);
     wave_store_run<T>(xrow + g0, wcount, min(max(9 * n - g0, 0), wcount), s_x, lane);
+    if (nreal > 1) chain_done_finish<kFwdPay>(a, b, nreal, done_old);
 }
 
 // ----------------------------------------------------------- backward chunk
@@ -846,13 +904,12 @@ __global__ void __launch_bounds__(64) a2b_bwd_chunk(ChunkArgs a) {
     constexpr int C = 64 * R;
     __shared__ double s_tab[128];
     const int lane = threadIdx.x;
-    int b, s;
-    if (!chunk_slot(a, &b, &s)) return;
-    const int L = a.L;
-    const int n = clamp_length(a, a.lengths[b], s == 0 && lane == 0);
-    const int nreal = (n + C - 1) / C;
+    Slot sl;
+    if (!chunk_slot(a, C, &sl)) return;
+    const int L = a.L, b = sl.b, n = sl.n, nreal = sl.nreal, s = sl.r;
+    if (sl.bad && s == 0 && lane == 0) set_status(a.status, TPL_A2B_STATUS_LENGTH);
     T* grow = static_cast<T*>(a.grad_angles) + (size_t)b * 3 * L;
-    if (s >= nreal) {  // wholly past the chain: its zeros
+    if (!sl.real) {  // slot s is wholly past the chain: its zeros
         const int rl = s * C + lane * R;
         const int crow = min(max(L - rl, 0), R);
         if (crow > 0) {
@@ -955,23 +1012,22 @@ __global__ void __launch_bounds__(64) a2b_bwd_chunk(ChunkArgs a) {
     // its inclusive suffix instead, for the previous group; chunk 0 publishes
     // nothing.
     if (nreal > 1) {
-        double* my = a.pay + ((size_t)b * a.nc + c) * kBwdPay;
-        uint32_t* myf = a.flags + (size_t)b * a.nc + c;
+        uint64_t* my = a.pay + ((size_t)b * a.nc + c) * kBwdPay;
         const bool boundary = (c & 63) == 0;
-        if (!boundary) publish<6>(my, myf, tot, kFlagAggregate);
+        if (!boundary) publish<6>(my, tot);
         double carry[6] = {0, 0, 0, 0, 0, 0};
         if (c < nreal - 1) bwd_lookback(a, b, c, nreal, carry);
         if (boundary && c > 0) {
             double incl[6];
 #pragma unroll
             for (int e = 0; e < 6; ++e) incl[e] = carry[e] + tot[e];
-            publish<6>(my + 6, myf, incl, kFlagInclusive);
+            publish<6>(my, incl);
         }
 #pragma unroll
         for (int e = 0; e < 6; ++e) S[e] += carry[e];
-        chain_done<kBwdPay>(a, b, nreal);
     }
 
+    const uint32_t done_old = nreal > 1 ? chain_done_issue(a, b) : 0u;
     if (cnt > 0) {
         // S becomes the suffix from the lane's atom 1: add the lane's own
         // atoms, drop atom 0.
@@ -1028,6 +1084,7 @@ __global__ void __launch_bounds__(64) a2b_bwd_chunk(ChunkArgs a) {
             for (int k = 0; k < 3; ++k) lane_store_zero<T, R>(grow + (size_t)k * L + rl, crow);
         }
     }
+    if (nreal > 1) chain_done_finish<kBwdPay>(a, b, nreal, done_old);
 }
 
 __global__ void a2b_zero_words(uint32_t* p, size_t n) {
@@ -1204,17 +1261,16 @@ size_t state_total(int B, int L, const tpl_a2b_config* c) {
 }
 
 // Sync: the forward's look-back words then the backward's, each as tickets
-// [B], done [B], flags [B][nc] (uint32), then payload [B][nc][pay] doubles.
+// [B], done [B] (uint32), then records [B][nc][pay] (64-bit words).
 struct SyncRegion {
-    size_t tickets, done, flags, pay, end;
+    size_t tickets, done, pay, end;
 };
 SyncRegion sync_region(size_t base, int B, int nc, int pay) {
     SyncRegion r{};
     r.tickets = base;
     r.done = r.tickets + (size_t)B * 4;
-    r.flags = r.done + (size_t)B * 4;
-    r.pay = align256(r.flags + (size_t)B * nc * 4);
-    r.end = align256(r.pay + (size_t)B * nc * pay * sizeof(double));
+    r.pay = align256(r.done + (size_t)B * 4);
+    r.end = align256(r.pay + (size_t)B * nc * pay * sizeof(uint64_t));
     return r;
 }
 struct SyncLayout {
@@ -1283,8 +1339,7 @@ void bind_sync(ChunkArgs& a, void* sync, const SyncRegion& r) {
     char* base = static_cast<char*>(sync);
     a.tickets = reinterpret_cast<uint32_t*>(base + r.tickets);
     a.done = reinterpret_cast<uint32_t*>(base + r.done);
-    a.flags = reinterpret_cast<uint32_t*>(base + r.flags);
-    a.pay = reinterpret_cast<double*>(base + r.pay);
+    a.pay = reinterpret_cast<uint64_t*>(base + r.pay);
 }
 
 int launch(KernelFn k, const ChunkArgs& a, size_t lds, hipStream_t stream) {

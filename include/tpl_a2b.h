@@ -58,7 +58,8 @@ enum tpl_a2b_error {
     TPL_A2B_ERR_CONFIG = 2, /* unsupported chunk configuration */
     TPL_A2B_ERR_LAUNCH = 3, /* a kernel launch failed */
     TPL_A2B_ERR_HIP = 4,    /* another HIP runtime call failed */
-    TPL_A2B_ERR_SIZE = 5    /* batch * angles_stride too large */
+    TPL_A2B_ERR_SIZE = 5,   /* batch * angles_stride too large */
+    TPL_A2B_ERR_SCRATCH = 6 /* too large for tpl_a2b_forward_noalloc's scratch */
 };
 
 /* Bits OR-ed into *status by the kernels. */
@@ -138,6 +139,26 @@ int tpl_a2b_backward(int dtype, void *grad_angles, const void *grad_coords, cons
                      const int32_t *lengths, const void *state, void *sync, uint32_t *status,
                      int batch, int angles_stride, const tpl_a2b_config *cfg,
                      tpl_stream_t stream);
+
+/* Forward that allocates nothing and takes no workspace.  It has no
+ * counterpart in the reference.  It is for callers that cannot allocate or
+ * own a workspace: code under graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  Same as tpl_a2b_forward with state == NULL (no
+ * backward follows), but its look-back words live in a device array inside
+ * this library, tpl_a2b_noalloc_max_bytes() bytes (4 MiB) per GPU, held until
+ * the process ends.  A problem whose forward look-back words exceed it
+ * (8 bytes per chain plus 96 per chunk of 64 * fwd_res_per_lane residues,
+ * each part rounded up to 256 bytes)
+ * returns TPL_A2B_ERR_SCRATCH.  Every call on one GPU shares that array, so
+ * calls that overlap in time on one GPU (different streams, or different
+ * threads without ordering) race and must not be made.  Each call first
+ * clears the words it uses, with a kernel on `stream`. */
+int tpl_a2b_forward_noalloc(int dtype, const void *angles, void *coords, const int32_t *lengths,
+                            uint32_t *status, int batch, int angles_stride,
+                            const tpl_a2b_config *cfg, tpl_stream_t stream);
+
+/* Bytes of tpl_a2b_forward_noalloc's built-in scratch. */
+size_t tpl_a2b_noalloc_max_bytes(void);
 
 /* ---- plain form: the reference algorithm written as straightforward HIP ----
  * One thread per chain runs the serial recurrence and stores every atom's 3x4

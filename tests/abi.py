@@ -28,6 +28,9 @@ def load():
     L.tpl_a2b_chunk_residues.restype = ctypes.c_int64
     L.tpl_a2b_chunk_residues.argtypes = [cfgp, i32]
     L.tpl_a2b_sync_clear.argtypes = [vp, ctypes.c_size_t, vp]
+    L.tpl_a2b_forward_noalloc.argtypes = [i32, vp, vp, vp, vp, i32, i32, cfgp, vp]
+    L.tpl_a2b_noalloc_max_bytes.restype = ctypes.c_size_t
+    L.tpl_a2b_noalloc_max_bytes.argtypes = []
     for f in (L.tpl_a2b_state_bytes, L.tpl_a2b_sync_bytes):
         f.restype = ctypes.c_size_t
         f.argtypes = [i32, i32, i32, cfgp]

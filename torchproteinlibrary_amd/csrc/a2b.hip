@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "sincos_tab.h"
@@ -1342,6 +1343,32 @@ void bind_sync(ChunkArgs& a, void* sync, const SyncRegion& r) {
     a.pay = reinterpret_cast<uint64_t*>(base + r.pay);
 }
 
+// The forward's look-back words for callers that cannot allocate
+// (tpl_a2b_forward_noalloc): a file-scope device array, one copy per GPU,
+// held until the process ends.  Host code must not pass `scratch` itself to
+// a kernel (in host code the name is a stand-in, not a device address): it
+// uses the address hipGetSymbolAddress returns on each device, taken once
+// per device with that device current.
+constexpr size_t kScratchBytes = 4u << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, or null on failure.
+void* scratch_address() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
 int launch(KernelFn k, const ChunkArgs& a, size_t lds, hipStream_t stream) {
     const dim3 grid((unsigned)(8 * a.bx * a.nc)), block(64);
     hipLaunchKernelGGL(k, grid, block, lds, stream, a);
@@ -1361,6 +1388,9 @@ const char* tpl_a2b_strerror(int code) {
         case TPL_A2B_ERR_LAUNCH: return "kernel launch failed";
         case TPL_A2B_ERR_HIP: return "HIP runtime call failed";
         case TPL_A2B_ERR_SIZE: return "batch * angles_stride too large";
+        case TPL_A2B_ERR_SCRATCH:
+            return "problem too large for the built-in scratch of tpl_a2b_forward_noalloc "
+                   "(tpl_a2b_noalloc_max_bytes)";
         default: return "unknown error code";
     }
 }
@@ -1446,6 +1476,40 @@ int tpl_a2b_backward(int dtype, void* grad_angles, const void* grad_coords, cons
     bind_sync(a, sync, sync_layout(batch, angles_stride, cfg).bwd);
     const bool f32 = dtype == TPL_A2B_F32;
     return launch(f32 ? bwd_kernel<float>(R) : bwd_kernel<double>(R), a, 0,
+                  static_cast<hipStream_t>(stream));
+}
+
+size_t tpl_a2b_noalloc_max_bytes(void) { return kScratchBytes; }
+
+int tpl_a2b_forward_noalloc(int dtype, const void* angles, void* coords, const int32_t* lengths,
+                            uint32_t* status, int batch, int angles_stride,
+                            const tpl_a2b_config* cfg, tpl_stream_t stream) {
+    int rc = check_common(dtype, batch, angles_stride);
+    if (rc) return rc;
+    if (!valid_config(cfg) || !fits_dtype(dtype, cfg)) return TPL_A2B_ERR_CONFIG;
+    if (batch == 0 || angles_stride == 0) return TPL_A2B_OK;
+    if (!angles || !coords || !lengths) return TPL_A2B_ERR_ARG;
+    const SyncRegion r = sync_layout(batch, angles_stride, cfg).fwd;
+    if (r.end > kScratchBytes) return TPL_A2B_ERR_SCRATCH;
+    void* base = scratch_address();
+    if (!base) return TPL_A2B_ERR_HIP;
+    // Tickets, done counters and records must start at zero.  Every call
+    // leaves them zero unless it timed out, and a different shape lays them
+    // out elsewhere, so the words this call uses are cleared first, with a
+    // kernel (capture-safe).
+    rc = tpl_a2b_sync_clear(base, r.end, stream);
+    if (rc) return rc;
+    const int R = cfg->fwd_res_per_lane;
+    ChunkArgs a = base_args(batch, angles_stride, 64 * R, cfg);
+    a.angles = angles;
+    a.coords = coords;
+    a.lengths = lengths;
+    a.status = status;
+    a.ckpt = nullptr;
+    bind_sync(a, base, r);
+    const bool f32 = dtype == TPL_A2B_F32;
+    const size_t lds = (size_t)9 * 64 * R * (f32 ? 4 : 8);
+    return launch(f32 ? fwd_kernel<float>(R) : fwd_kernel<double>(R), a, lds,
                   static_cast<hipStream_t>(stream));
 }
 

@@ -103,29 +103,40 @@ def parse():
 
 
 def time_kernels(sets, cfg, reps):
-    """Average duration of the forward and of the backward launch alone, HIP
-    events on torch's current stream (the stream the extension launches on)."""
+    """Duration of the forward and of the backward launch, HIP events on
+    torch's current stream (the stream the extension launches on):
+    avg_ms = one event pair around `reps` back-to-back launches of that kernel
+    alone (rotating buffer sets), divided by reps; isolated_median_ms = the
+    median over launches each bracketed by its own event pair, which adds the
+    event round trip to every launch (an empty kernel measures about 6 us
+    that way against 3 us back to back on MI355X)."""
     import torch
     from torchproteinlibrary_amd._native import ext
     cur = torch.cuda.current_stream()
     res = {}
     for name in ("forward", "backward"):
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-               for _ in range(reps)]
-        for i in range(reps + 10):
+        def launch(i):
             A, N, G, X, GA, st, sy = sets[i % len(sets)]
-            rec = i >= 10
-            if rec:
-                evs[i - 10][0].record(cur)
             if name == "forward":
                 ext.Angles2BackboneGPU_forward(A, X, N, st, sy, None, list(cfg))
             else:
                 ext.Angles2BackboneGPU_backward(GA, G, A, N, st, sy, None, list(cfg))
-            if rec:
-                evs[i - 10][1].record(cur)
+        for i in range(10):
+            launch(i)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(cur)
+        for i in range(reps):
+            launch(i)
+        e1.record(cur)
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(reps)]
+        for i in range(reps):
+            evs[i][0].record(cur)
+            launch(i)
+            evs[i][1].record(cur)
         torch.cuda.synchronize()
         ms = sorted(a.elapsed_time(b) for a, b in evs)
-        res[name] = {"avg_ms": sum(ms) / len(ms), "median_ms": ms[len(ms) // 2]}
+        res[name] = {"avg_ms": e0.elapsed_time(e1) / reps, "isolated_median_ms": ms[len(ms) // 2]}
     return res
 
 
@@ -272,6 +283,24 @@ def _pmc_medians(outdir, counter, last=5):
             statistics.median(tail) if len(tail) == last else None)
 
 
+def _trace_avg_ms(outdir):
+    """{"forward": ms, "backward": ms}: mean end - start of the chunk kernels'
+    dispatches in a rocprofv3 --kernel-trace csv run."""
+    import csv
+    import glob
+    durs = {"forward": [], "backward": []}
+    for f in glob.glob(os.path.join(outdir, "**", "*kernel_trace.csv"), recursive=True):
+        with open(f, newline="") as fh:
+            for row in csv.DictReader(fh):
+                name = row.get("Kernel_Name", "")
+                k = "forward" if "a2b_fwd_chunk" in name else ("backward" if "a2b_bwd_chunk" in name else None)
+                if k:
+                    durs[k].append((int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) * 1e-6)
+    if not (durs["forward"] and durs["backward"]):
+        return None
+    return {k: sum(v) / len(v) for k, v in durs.items()}
+
+
 def traffic(args, cfg):
     """HBM bytes per launch of the forward and backward kernels: two
     rocprofv3 --pmc runs of the probe mode (FETCH_SIZE, then WRITE_SIZE, in
@@ -306,7 +335,23 @@ def traffic(args, cfg):
                 return None
             scale = 262144.0 / calib  # the calibration kernel reads and writes 256 MiB
             out[counter] = {"forward_kib": fwd[0], "backward_kib": bwd[0], "copy_kib": calib, "scale": scale}
+        # Kernel durations of the same probe run, from the kernel trace.
+        od = os.path.join(d, "trace")
+        cmd = [prof, "--kernel-trace", "--output-format", "csv", "-d", od, "-o", "p", "--",
+               sys.executable, os.path.abspath(__file__), "--probe", "--steps", "24",
+               "--workload", args.workload, "--config", ",".join(str(c) for c in cfg)]
+        print("bench: kernel trace run", file=sys.stderr, flush=True)
+        durations = None
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=120,
+                               env=dict(os.environ, TMPDIR=d))
+            if "probe done" in r.stdout:
+                durations = _trace_avg_ms(od)
+        except subprocess.TimeoutExpired:
+            pass
     res = {"fetch_scale": out["FETCH_SIZE"]["scale"], "write_scale": out["WRITE_SIZE"]["scale"]}
+    if durations:
+        res["rocprof_avg_ms"] = durations
     for k in ("forward", "backward"):
         res[k] = 1024.0 * (out["FETCH_SIZE"][k + "_kib"] * out["FETCH_SIZE"]["scale"] +
                            out["WRITE_SIZE"][k + "_kib"] * out["WRITE_SIZE"]["scale"])
@@ -781,6 +826,8 @@ def main():
             out["roofline"]["traffic"] = tr[dom]
             out["roofline"]["traffic_over_alg"] = tr[dom] / dom_bytes
             out["roofline"]["counters"] = tr
+            if "rocprof_avg_ms" in tr:
+                out["roofline"]["rocprof_avg_ms"] = tr["rocprof_avg_ms"][dom]
 
     print(json.dumps(out), flush=True)
 

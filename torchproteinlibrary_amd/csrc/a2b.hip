@@ -445,8 +445,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 // 3(r0 + lR) .. 3(r0 + lR) + 3R - 1).  One workgroup of one wave takes one
 // slot (b, s), s < nc = ceil(L / C).  Workgroup w runs on XCD w mod 8 (the
 // dispatcher's round-robin) and takes a chain b = x, x + 8, x + 16, ... of
-// stripe x = w mod 8, so all slots of a chain share one XCD; chunk_slot
-// below says which slot.
+// stripe x = w mod 8, so all slots of a chain share one XCD; slot_at below
+// says which slots.
 // With n residues a chain has nreal = ceil(n / C) chunks.  Slots s >= nreal
 // only write the zeros of their span of the outputs.  The nreal slots below
 // take the chain's chunks from a per-chain ticket counter, ascending in the
@@ -490,23 +490,25 @@ struct ChunkArgs {
     int bx;                  // chains per stripe: ceil(B / 8)
 };
 
-// The slot of this workgroup.  Stripe x = blockIdx.x mod 8 holds the chains
-// b = x + 8i, i < m (m = the stripe's chains); position j = blockIdx.x / 8.
+
+// Positions.  Stripe x = blockIdx.x mod 8 holds the chains b = x + 8i, i < m
+// (m = the stripe's chains) and m*nc positions j; workgroup w takes position
+// w / 8 of stripe w mod 8.
 //   * Positions j < T, T = sum over the stripe of nreal_i, are the chunks
 //     that hold residues, in rows: row r lists, in stripe order, the chains
-//     with nreal_i > r (cnt_r of them), and a wave of row r takes its chain's
-//     r-th chunk in processing order.
+//     with nreal_i > r (cnt_r of them), and row r's position of a chain
+//     takes its r-th chunk in processing order.
 //   * Positions T <= j < m*nc are the padding slots, in rows as well: row r
 //     lists the chains with nreal_i <= r, whose slot r is past the chain.
-// So the waves with work come first in every stripe, and the dispatcher,
-// placing workgroups in order, spreads them over every SIMD; only then come
-// the waves that write zeros.  The row counts are popcounts of ballots over
-// the stripe's lengths (read once per wave, at most kMaxStripe of them) and
-// the row is found by a scalar walk over them, so this costs one load round
-// and about 10 instructions per row.  With more than kMaxRows rows or
-// kMaxStripe chains, the slots go in slot order instead (row r, chain i),
-// the chains of row r rotated by r * kRowTurn so that one chain's chunks do
-// not follow each other at the period of the dispatcher's placement.
+// So the work comes first in every stripe, spread over every SIMD by the
+// dispatcher, and the zeros after it.  The row counts are popcounts of
+// ballots over the stripe's chunk counts (read once per wave, at most
+// kMaxStripe of them) and a position's row is found by a scalar walk over
+// them, about 10 instructions per row.
+// With more than kMaxRows rows or kMaxStripe chains, positions go in slot
+// order instead (row r, chain i), the chains of row r rotated by
+// r * kRowTurn so that one chain's chunks do not follow each other at the
+// period of the dispatcher's placement.
 constexpr int kStripeLanes = 8;  // chains per lane
 constexpr int kMaxStripe = 64 * kStripeLanes;
 constexpr int kMaxRows = 32;
@@ -528,28 +530,39 @@ __device__ __forceinline__ Slot make_slot(const ChunkArgs& a, int C, int b, int 
     return Slot{b, r, n, nreal, r < nreal, raw != n};
 }
 
-__device__ __forceinline__ bool chunk_slot(const ChunkArgs& a, int C, Slot* out) {
+struct Stripe {
+    int x, m;     // stripe and its chain count
+    int kk;       // chunk counts per lane in use; 0: slot order
+    int nr[kStripeLanes];  // chunk count of chain x + 8 (lane + 64k)
+};
+
+// False if the stripe has no chains.
+__device__ __forceinline__ bool stripe_init(const ChunkArgs& a, int C, Stripe& st) {
     const int lane = threadIdx.x;
-    const int x = (int)(blockIdx.x & 7), j = (int)(blockIdx.x >> 3);
-    const int m = a.B > x ? (a.B - x + 7) / 8 : 0;
-    if (j >= m * a.nc) return false;
-    if (m > kMaxStripe || a.nc > kMaxRows) {  // slot order, rows rotated
-        const int r = j / m;
-        const int i = (j - r * m + (int)(((int64_t)r * kRowTurn) % m)) % m;
-        *out = make_slot(a, C, x + 8 * i, r);
-        return true;
-    }
-    const int kk = (m + 63) / 64;  // lengths per lane in use (uniform)
-    int nr[kStripeLanes];
+    st.x = (int)(blockIdx.x & 7);
+    st.m = a.B > st.x ? (a.B - st.x + 7) / 8 : 0;
+    st.kk = (st.m <= kMaxStripe && a.nc <= kMaxRows) ? (st.m + 63) / 64 : 0;
 #pragma unroll
     for (int k = 0; k < kStripeLanes; ++k) {
         const int i = lane + 64 * k;
         int n = 0;
-        if (k < kk && i < m) {
-            n = a.lengths[x + 8 * i];
+        if (k < st.kk && i < st.m) {
+            n = a.lengths[st.x + 8 * i];
             n = n < 0 ? 0 : (n > a.L ? a.L : n);
         }
-        nr[k] = (n + C - 1) / C;
+        st.nr[k] = (n + C - 1) / C;
+    }
+    return st.m > 0;
+}
+
+// The slot at position j < m * nc of the stripe.
+__device__ __forceinline__ Slot slot_at(const ChunkArgs& a, int C, const Stripe& st, int j) {
+    const int lane = threadIdx.x;
+    const int m = st.m, kk = st.kk;
+    if (kk == 0) {  // slot order, rows rotated
+        const int r = j / m;
+        const int i = (j - r * m + (int)(((int64_t)r * kRowTurn) % m)) % m;
+        return make_slot(a, C, st.x + 8 * i, r);
     }
     // cnt_r = #{i : nreal_i > r}; real rows first, then padding rows of
     // m - cnt_r slots each.
@@ -560,7 +573,7 @@ __device__ __forceinline__ bool chunk_slot(const ChunkArgs& a, int C, Slot* out)
             int cnt = 0;
 #pragma unroll
             for (int k = 0; k < kStripeLanes; ++k)
-                if (k < kk) cnt += __builtin_popcountll(__builtin_amdgcn_ballot_w64(nr[k] > rr));
+                if (k < kk) cnt += __builtin_popcountll(__builtin_amdgcn_ballot_w64(st.nr[k] > rr));
             const int size = pass == 0 ? cnt : m - cnt;
             if (j < row_start + size) {
                 r = rr;
@@ -572,26 +585,25 @@ __device__ __forceinline__ bool chunk_slot(const ChunkArgs& a, int C, Slot* out)
         }
     }
     int q = j - start;  // rank within the row
-    int b = x;
+    int b = st.x;
     bool found = false;
 #pragma unroll
     for (int k = 0; k < kStripeLanes; ++k) {
         if (k >= kk) break;
-        const bool in = (lane + 64 * k < m) && (real ? nr[k] > r : nr[k] <= r);
+        const bool in = (lane + 64 * k < m) && (real ? st.nr[k] > r : st.nr[k] <= r);
         const uint64_t mask = __builtin_amdgcn_ballot_w64(in);
         const int cnt = __builtin_popcountll(mask);
         if (!found && q < cnt) {
             const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
             const uint64_t hit = __builtin_amdgcn_ballot_w64(in && below == q);
-            b = x + 8 * (__builtin_ctzll(hit) + 64 * k);
+            b = st.x + 8 * (__builtin_ctzll(hit) + 64 * k);
             found = true;
         } else if (!found) {
             q -= cnt;
         }
     }
-    *out = make_slot(a, C, b, r);
-    return true;
+    return make_slot(a, C, b, r);
 }
 
 __device__ __forceinline__ int take_ticket(uint32_t* t) {
@@ -758,26 +770,28 @@ __device__ void bwd_lookback(const ChunkArgs& a, int b, int c, int nreal, double
 //      over p_a in LDS;
 //   6. the wave stores its 9*C-float span with 16-byte stores from LDS,
 //      zeros past the chain.
+// One slot of the forward; s_tab is filled on the wave's first chunk (tab).
 template <typename T, int R>
-__global__ void __launch_bounds__(64) a2b_fwd_chunk(ChunkArgs a) {
+__device__ __forceinline__ void fwd_slot(const ChunkArgs& a, const Slot& sl, T* s_x, double* s_tab,
+                                         bool& tab) {
     constexpr int C = 64 * R;
     constexpr int E9 = 9 * R;  // output floats per lane = the lane's LDS stride
-    extern __shared__ __align__(16) unsigned char s_dyn[];
-    __shared__ double s_tab[128];
-    T* s_x = reinterpret_cast<T*>(s_dyn);
     T* s_l = s_x + threadIdx.x * E9;
     const int lane = threadIdx.x;
-    Slot sl;
-    if (!chunk_slot(a, C, &sl)) return;
     const int L = a.L, b = sl.b, n = sl.n, nreal = sl.nreal, s = sl.r;
     if (sl.bad && s == 0 && lane == 0) set_status(a.status, TPL_A2B_STATUS_LENGTH);
     T* xrow = static_cast<T*>(a.coords) + (size_t)b * 9 * L;
-    if (!sl.real) {  // slot s is wholly past the chain: its zeros
-        const int f0 = 9 * s * C;
-        wave_store_run<T>(xrow + f0, min(9 * C, 9 * L - f0), 0, s_x, lane);
+    if (!sl.real) {  // past the chain: the first such slot writes all its zeros
+        if (s == nreal) {
+            const int f0 = 9 * s * C;
+            wave_store_run<T>(xrow + f0, 9 * L - f0, 0, s_x, lane);
+        }
         return;
     }
-    for (int i = lane; i < 128; i += 64) s_tab[i] = kTabCS[i];
+    const bool fill = !tab;
+    if (fill) {
+        for (int i = lane; i < 128; i += 64) s_tab[i] = kTabCS[i];
+    }
     const T* arow = static_cast<const T*>(a.angles) + (size_t)b * 3 * L;
     T ang[3][R];
     {
@@ -790,7 +804,10 @@ __global__ void __launch_bounds__(64) a2b_fwd_chunk(ChunkArgs a) {
     const int rl = r0 + lane * R;
     const int cnt = min(max(n - rl, 0), R);  // residues of the chain in this lane
     if (c != s) lane_load_rows<T, R>(arow + rl, L, cnt, L - rl, ang);
-    __syncthreads();  // s_tab
+    if (fill) {
+        __syncthreads();  // s_tab
+        tab = true;
+    }
 
     // Pass 1.
     double E[kF];
@@ -878,6 +895,19 @@ __global__ void __launch_bounds__(64) a2b_fwd_chunk(ChunkArgs a) {
     if (nreal > 1) chain_done_finish<kFwdPay>(a, b, nreal, done_old);
 }
 
+template <typename T, int R>
+__global__ void __launch_bounds__(64) a2b_fwd_chunk(ChunkArgs a) {
+    constexpr int C = 64 * R;
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    __shared__ double s_tab[128];
+    T* s_x = reinterpret_cast<T*>(s_dyn);
+    Stripe st;
+    const int j = (int)(blockIdx.x >> 3);
+    if (!stripe_init(a, C, st) || j >= st.m * a.nc) return;
+    bool tab = false;
+    fwd_slot<T, R>(a, slot_at(a, C, st, j), s_x, s_tab, tab);
+}
+
 // ----------------------------------------------------------- backward chunk
 // Chain atom j >= 1 (atoms N_0, CA_0, C_0, N_1, ...) is placed by the
 // transform whose angle theta_j is phi_r, psi_r, omega_r for j = 3r+1, 3r+2,
@@ -900,69 +930,96 @@ __global__ void __launch_bounds__(64) a2b_fwd_chunk(ChunkArgs a) {
 //      kept (cos, sin); at atom j it has x_j and u_j and the suffix
 //      S_j = sum over atoms i >= j, gives the gradient, and drops atom j's
 //      term from S.  psi and omega of the last residue are 0.
+// One slot of the backward; s_tab is filled on the wave's first chunk (tab).
+// s_cs [6R][64]: the lanes' (cos, sin) from pass 1 for pass B, element-major
+// so that a wave's access is 64 consecutive doubles.
 template <typename T, int R>
-__global__ void __launch_bounds__(64) a2b_bwd_chunk(ChunkArgs a) {
+__device__ __forceinline__ void bwd_slot(const ChunkArgs& a, const Slot& sl, double* s_tab, double* s_cs,
+                                         bool& tab) {
     constexpr int C = 64 * R;
-    __shared__ double s_tab[128];
     const int lane = threadIdx.x;
-    Slot sl;
-    if (!chunk_slot(a, C, &sl)) return;
     const int L = a.L, b = sl.b, n = sl.n, nreal = sl.nreal, s = sl.r;
     if (sl.bad && s == 0 && lane == 0) set_status(a.status, TPL_A2B_STATUS_LENGTH);
     T* grow = static_cast<T*>(a.grad_angles) + (size_t)b * 3 * L;
-    if (!sl.real) {  // slot s is wholly past the chain: its zeros
-        const int rl = s * C + lane * R;
-        const int crow = min(max(L - rl, 0), R);
-        if (crow > 0) {
+    if (!sl.real) {  // past the chain: the first such slot writes all its zeros
+        if (s == nreal) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) lane_store_zero<T, R>(grow + (size_t)k * L + rl, crow);
+            for (int k = 0; k < 3; ++k)
+                wave_store_run<T>(grow + (size_t)k * L + s * C, L - s * C, 0, static_cast<const T*>(nullptr), lane);
         }
         return;
     }
-    for (int i = lane; i < 128; i += 64) s_tab[i] = kTabCS[i];
+    const bool fill = !tab;
+    if (fill) {
+        for (int i = lane; i < 128; i += 64) s_tab[i] = kTabCS[i];
+    }
     const T* arow = static_cast<const T*>(a.angles) + (size_t)b * 3 * L;
     const T* gin = static_cast<const T*>(a.grad) + (size_t)b * 9 * L;
     T ang[3][R];
     T gr[9 * R];
+    // A lane wholly past the chain reads nothing: whole-run reads there
+    // would fetch the padding of every chain's last chunk, about half a chunk
+    // of gradients and angles per chain (+45% over the valid inputs at
+    // B=1024 with lengths 30..1000).
     auto load = [&](int cc) {
         const int r = cc * C + lane * R;
         const int k = min(max(n - r, 0), R);
-        lane_load_rows<T, R>(arow + r, L, k, L - r, ang);
-        lane_load<T, 9 * R>(gin + (size_t)9 * r, 9 * k, gr, 9 * (L - r));
+        if (k > 0) {
+            lane_load_rows<T, R>(arow + r, L, k, L - r, ang);
+            lane_load<T, 9 * R>(gin + (size_t)9 * r, 9 * k, gr, 9 * (L - r));
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+                for (int i = 0; i < R; ++i) ang[kk][i] = (T)0;
+#pragma unroll
+            for (int i = 0; i < 9 * R; ++i) gr[i] = (T)0;
+        }
+    };
+    double P[kF];  // checkpoint: frame of the chunk's first atom (uniform)
+    auto load_ckpt = [&](int cc) {
+        if (cc == 0) {
+            frame_identity(P);
+        } else {
+            const double* ck = a.ckpt + ((size_t)b * a.nk + cc) * kF;
+#pragma unroll
+            for (int e = 0; e < kF; ++e) P[e] = ck[e];
+        }
     };
     const int guess = nreal - 1 - s;  // the chunk of this slot if tickets follow slot order
     load(guess);
+    load_ckpt(guess);
     const int t = nreal > 1 ? chain_ticket(a, b, nreal) : 0;
     if (t < 0) return;
     const int c = nreal - 1 - t;
-    if (c != guess) load(c);
+    if (c != guess) {
+        load(c);
+        load_ckpt(c);
+    }
     const int r0 = c * C;
     const int rl = r0 + lane * R;
     const int cnt = min(max(n - rl, 0), R);
-    double P[kF];  // checkpoint: frame of the chunk's first atom (uniform)
-    if (c == 0) {
-        frame_identity(P);
-    } else {
-        const double* ck = a.ckpt + ((size_t)b * a.nk + c) * kF;
-#pragma unroll
-        for (int e = 0; e < kF; ++e) P[e] = ck[e];
+    if (fill) {
+        __syncthreads();  // s_tab
+        tab = true;
     }
-    __syncthreads();  // s_tab
 
     // Pass 1 in the lane's own frame.
     double f[kF];
     frame_identity(f);
-    double cs[3][R], sn[3][R];
     double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // M[3m + k] = sum_a p_a[m] g_a[k]
     double F[3] = {(double)gr[0], (double)gr[1], (double)gr[2]};  // atom 0 sits at p = 0
 #pragma unroll
     for (int q = 0; q < R; ++q) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            sincos_tab((double)ang[k][q], s_tab, &sn[k][q], &cs[k][q]);
-            if (k == 0) frame_step<0>(f, cs[k][q], sn[k][q]);
-            if (k == 1) frame_step<1>(f, cs[k][q], sn[k][q]);
-            if (k == 2) frame_step<2>(f, cs[k][q], sn[k][q]);
+            double sn, cs;
+            sincos_tab((double)ang[k][q], s_tab, &sn, &cs);
+            s_cs[(6 * q + 2 * k) * 64 + lane] = cs;
+            s_cs[(6 * q + 2 * k + 1) * 64 + lane] = sn;
+            if (k == 0) frame_step<0>(f, cs, sn);
+            if (k == 1) frame_step<1>(f, cs, sn);
+            if (k == 2) frame_step<2>(f, cs, sn);
             const int at = 3 * q + k + 1;
             if (at < 3 * R) {
                 const double g0 = (double)gr[3 * at], g1 = (double)gr[3 * at + 1],
@@ -1046,9 +1103,11 @@ __global__ void __launch_bounds__(64) a2b_bwd_chunk(ChunkArgs a) {
         for (int q = 0; q < R; ++q) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                if (k == 0) frame_step<0>(f, cs[k][q], sn[k][q]);
-                if (k == 1) frame_step<1>(f, cs[k][q], sn[k][q]);
-                if (k == 2) frame_step<2>(f, cs[k][q], sn[k][q]);
+                const double cs = s_cs[(6 * q + 2 * k) * 64 + lane];
+                const double sn = s_cs[(6 * q + 2 * k + 1) * 64 + lane];
+                if (k == 0) frame_step<0>(f, cs, sn);
+                if (k == 1) frame_step<1>(f, cs, sn);
+                if (k == 2) frame_step<2>(f, cs, sn);
                 const int j = 3 * q + k + 1;  // lane atom j: x = t(f), u = column 0 of f
                 const double x0 = f[9], x1 = f[10], x2 = f[11];
                 const double v = f[0] * (S[0] - (x1 * S[5] - x2 * S[4])) +
@@ -1086,6 +1145,18 @@ __global__ void __launch_bounds__(64) a2b_bwd_chunk(ChunkArgs a) {
         }
     }
     if (nreal > 1) chain_done_finish<kBwdPay>(a, b, nreal, done_old);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64) a2b_bwd_chunk(ChunkArgs a) {
+    constexpr int C = 64 * R;
+    extern __shared__ double s_cs[];  // [6R][64]
+    __shared__ double s_tab[128];
+    Stripe st;
+    const int j = (int)(blockIdx.x >> 3);
+    if (!stripe_init(a, C, st) || j >= st.m * a.nc) return;
+    bool tab = false;
+    bwd_slot<T, R>(a, slot_at(a, C, st, j), s_tab, s_cs, tab);
 }
 
 __global__ void a2b_zero_words(uint32_t* p, size_t n) {
@@ -1475,7 +1546,7 @@ int tpl_a2b_backward(int dtype, void* grad_angles, const void* grad_coords, cons
     a.ckpt = const_cast<double*>(static_cast<const double*>(state));
     bind_sync(a, sync, sync_layout(batch, angles_stride, cfg).bwd);
     const bool f32 = dtype == TPL_A2B_F32;
-    return launch(f32 ? bwd_kernel<float>(R) : bwd_kernel<double>(R), a, 0,
+    return launch(f32 ? bwd_kernel<float>(R) : bwd_kernel<double>(R), a, (size_t)6 * R * 64 * sizeof(double),
                   static_cast<hipStream_t>(stream));
 }
 

@@ -103,7 +103,8 @@ size_t tpl_a2b_state_bytes(int dtype, int batch, int angles_stride, const tpl_a2
 
 /* Bytes of the look-back words of forward and backward (per-chain tickets
  * and done counters, then one record of 64-bit words per chunk, each word
- * marking itself written).  The buffer must be
+ * marking itself written; laid out so that no 256-byte block holds words of
+ * chains that run on different XCDs).  The buffer must be
  * all zero before its first use (tpl_a2b_sync_clear, or any zero fill); every
  * call that ends without TPL_A2B_STATUS_TIMEOUT returns all it wrote to zero,
  * so one buffer serves every later call on the same stream, of any shape and
@@ -147,11 +148,14 @@ int tpl_a2b_backward(int dtype, void *grad_angles, const void *grad_coords, cons
  * backward follows), but its look-back words live in a device array inside
  * this library, tpl_a2b_noalloc_max_bytes() bytes (4 MiB) per GPU, held until
  * the process ends.  A problem whose forward look-back words exceed it
- * (8 bytes per chain plus 96 per chunk of 64 * fwd_res_per_lane residues,
- * each part rounded up to 256 bytes)
- * returns TPL_A2B_ERR_SCRATCH.  Every call on one GPU shares that array, so
- * calls that overlap in time on one GPU (different streams, or different
- * threads without ordering) race and must not be made.  Each call first
+ * returns TPL_A2B_ERR_SCRATCH.  Those words are the tickets and the done
+ * counters, 2 * 8 * ceil(ceil(batch / 8) / 64) * 256 bytes, then the
+ * records, batch * ceil(12 * nc / 32) * 256 bytes with
+ * nc = ceil(angles_stride / (64 * fwd_res_per_lane)); e.g. batch 1024 and
+ * angles_stride 1000 take 0.53 MB with 4 residues per lane.  Every call on
+ * one GPU shares that array, so calls that overlap in time on one GPU
+ * (different streams, or different threads without ordering) race and must
+ * not be made.  Each call first
  * clears the words it uses, with a kernel on `stream`. */
 int tpl_a2b_forward_noalloc(int dtype, const void *angles, void *coords, const int32_t *lengths,
                             uint32_t *status, int batch, int angles_stride,

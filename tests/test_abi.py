@@ -37,10 +37,14 @@ def _state(B, L, rb):
 
 
 def _sync(B, L, rf, rb):
-    # per direction: tickets [B], done [B] (uint32), then one record per
-    # chunk [B][nc] of 12 64-bit words (forward frames) or 6 (backward sums)
+    # per direction: tickets and done counters [8][sbx] (uint32; sbx =
+    # ceil(B / 8) rounded up to 64), then each chain's records, nc chunks of
+    # 12 64-bit words (forward frames) or 6 (backward sums) rounded up to 32
+    # words
+    sbx = -(-(-(-B // 8)) // 64) * 64
+
     def region(nc, pay):
-        return _align(_align(8 * B) + B * nc * pay * 8)
+        return _align(_align(2 * 8 * sbx * 4) + B * (-(-nc * pay // 32) * 32) * 8)
     return region(-(-L // (64 * rf)), 12) + region(-(-L // (64 * rb)), 6)
 
 

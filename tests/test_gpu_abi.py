@@ -82,3 +82,63 @@ def test_plain_via_c_abi_and_tuner():
     assert rc == 0
     assert ms[0] > 0 and ms[1] > 0 and ms[2] == -1.0
     assert (best.fwd_res_per_lane, best.bwd_res_per_lane) in [(1, 1), (4, 1)]
+
+
+_COLD_CALLS = r'''
+import ctypes, json, sys
+import torch
+from tests import abi
+from torchproteinlibrary_amd import synthetic
+lib = abi.load()
+P = lambda t: ctypes.c_void_p(t.data_ptr())
+dev = "cuda"
+stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+a, n, g = synthetic.workload("long16k")
+B, _, L = a.shape
+A, N, G = (torch.tensor(v, device=dev) for v in (a, n, g))
+X = torch.empty(B, 9 * L, device=dev)
+GA = torch.empty(B, 3, L, device=dev)
+out, first = [], None
+for cfg in [(1, 1), (4, 4), (4, 2), (8, 4), (2, 2), (1, 1)]:
+    c = abi.Config(*cfg, 0, 0)
+    st = torch.empty(lib.tpl_a2b_state_bytes(0, B, L, ctypes.byref(c)), dtype=torch.uint8, device=dev)
+    status = torch.zeros(2, dtype=torch.int32, device=dev)
+    sy = torch.zeros(lib.tpl_a2b_sync_bytes(0, B, L, ctypes.byref(c)), dtype=torch.uint8, device=dev)
+    assert lib.tpl_a2b_forward(0, P(A), P(X), P(N), P(st), P(sy), P(status), B, L, ctypes.byref(c), stream) == 0
+    assert lib.tpl_a2b_backward(0, P(GA), P(G), P(A), P(N), P(st), P(sy),
+                                ctypes.c_void_p(status.data_ptr() + 4), B, L, ctypes.byref(c), stream) == 0
+    torch.cuda.synchronize()
+    same = None
+    if cfg == (1, 1):
+        if first is None:
+            first = (X.clone(), GA.clone())
+        else:
+            same = bool(torch.equal(X, first[0]) and torch.equal(GA, first[1]))
+    out.append([list(cfg), status.cpu().tolist(), same, int(sy.count_nonzero())])
+    print(json.dumps(out[-1]), flush=True)
+print("RESULT " + json.dumps(out), flush=True)
+'''
+
+
+def test_first_calls_in_a_fresh_process_do_not_time_out():
+    """Regression: the first calls of a process (code and look-back buffer
+    cold) on long chains with many chunks per chain, where waiting waves used
+    to starve the wave they waited for until the look-back gave up
+    (TPL_A2B_STATUS_TIMEOUT, typically on the first (1, 1) call).  Run in a
+    child process so that nothing is warm; every call must finish with status
+    0 and a clean buffer, and the two (1, 1) calls must agree bit for bit."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    print("child process: long16k, configs (1,1) (4,4) (4,2) (8,4) (2,2) (1,1), first calls", flush=True)
+    r = subprocess.run([sys.executable, "-c", _COLD_CALLS], cwd=root, capture_output=True, text=True,
+                       timeout=110)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stderr[-2000:]
+    res = json.loads([ln for ln in r.stdout.splitlines() if ln.startswith("RESULT ")][-1][7:])
+    for cfg, status, same, dirty in res:
+        assert status == [0, 0], f"config {cfg}: status forward/backward {status}"
+        assert dirty == 0, f"config {cfg}: look-back buffer left {dirty} nonzero bytes"
+    assert res[-1][2] is True

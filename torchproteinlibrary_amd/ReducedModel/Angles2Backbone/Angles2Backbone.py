@@ -42,6 +42,7 @@ from __future__ import annotations
 
 import os
 import threading
+import warnings
 
 import torch
 from torch.autograd import Function
@@ -65,15 +66,42 @@ def _status_word(device):
     return torch.zeros(1, dtype=torch.int32, device=device) if _checks_enabled() else None
 
 
-def _raise_on_status(status, where: str, sync=None) -> None:
+def _run_checked(launch, status, where: str, sync) -> None:
+    """Runs launch(status) and raises on the status bits.  A look-back
+    timeout or unclean look-back words make the result invalid: the words are
+    cleared and the call is run once more, with a RuntimeWarning, and only a
+    second failure raises.  The kernels' waits are bounded (a wave that never
+    sees its record gives up and reports TIMEOUT instead of hanging), so this
+    is the recovery path for a call that ran out of wait time."""
+    launch(status)
     if status is None:
         return
     bits = int(status.item())
     if bits & (_STATUS_TIMEOUT | _STATUS_SYNC):
-        if sync is not None:
-            ext.sync_clear(sync)  # look-back words may be left set
+        ext.sync_clear(sync)  # look-back words may be left set
         what = "look-back wait timed out" if bits & _STATUS_TIMEOUT else "look-back words were not clean"
-        raise RuntimeError("Angles2BackboneFunction: %s %s (result invalid)" % (where, what))
+        warnings.warn("Angles2BackboneFunction: %s %s; cleared and repeated" % (where, what),
+                      RuntimeWarning)
+        status.zero_()
+        launch(status)
+        bits = int(status.item())
+        if bits & (_STATUS_TIMEOUT | _STATUS_SYNC):
+            ext.sync_clear(sync)
+            what = "look-back wait timed out" if bits & _STATUS_TIMEOUT else "look-back words were not clean"
+            raise RuntimeError("Angles2BackboneFunction: %s %s (result invalid)" % (where, what))
+    _raise_on_bits(bits, where)
+
+
+def _raise_on_status(status, where: str) -> None:
+    if status is None:
+        return
+    bits = int(status.item())
+    if bits & (_STATUS_TIMEOUT | _STATUS_SYNC):
+        raise RuntimeError("Angles2BackboneFunction: %s look-back failed (result invalid)" % where)
+    _raise_on_bits(bits, where)
+
+
+def _raise_on_bits(bits: int, where: str) -> None:
     if bits & _STATUS_LENGTH:
         raise ValueError("Angles2BackboneFunction: angles_length outside [0, input.size(2)] "
                          "(clamped on the device)")
@@ -128,9 +156,9 @@ class Angles2BackboneGPUFunction(Function):
         state = _empty_bytes(st_bytes, dev) if ctx.needs_input_grad[0] else None
         sync = _sync_buffer(dev, sy_bytes)
         status = _status_word(dev)
-        ext.Angles2BackboneGPU_forward(input_angles, coords, angles_length, state, sync, status,
-                                       list(cfg))
-        _raise_on_status(status, "forward", sync)
+        _run_checked(lambda st: ext.Angles2BackboneGPU_forward(input_angles, coords, angles_length,
+                                                               state, sync, st, list(cfg)),
+                     status, "forward", sync)
         if state is not None:
             ctx.save_for_backward(input_angles, angles_length, state)
         ctx.cfg = cfg
@@ -149,9 +177,10 @@ class Angles2BackboneGPUFunction(Function):
         _, sy_bytes = ext.workspace_bytes(input_angles.dtype == torch.float64, B, L, list(ctx.cfg))
         sync = _sync_buffer(dev, sy_bytes)
         status = _status_word(dev)
-        ext.Angles2BackboneGPU_backward(gradInput, gradOutput, input_angles, angles_length, state,
-                                        sync, status, list(ctx.cfg))
-        _raise_on_status(status, "backward", sync)
+        _run_checked(lambda st: ext.Angles2BackboneGPU_backward(gradInput, gradOutput, input_angles,
+                                                                angles_length, state, sync, st,
+                                                                list(ctx.cfg)),
+                     status, "backward", sync)
         return gradInput, None
 
 

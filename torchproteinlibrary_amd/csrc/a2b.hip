@@ -481,14 +481,32 @@ struct ChunkArgs {
     uint32_t* status;
     double* ckpt;            // [B][nk][kF]: entry k is the frame of atom 3kK (N of
                              // residue kK), written for 0 < kK < n; null: none
-    uint32_t* tickets;       // [B]
-    uint32_t* done;          // [B]
-    uint64_t* pay;           // [B][nc][kFwdPay] or [B][nc][kBwdPay] record words
+    uint32_t* tickets;       // [8][sbx]: chain b at (b mod 8) * sbx + b / 8
+    uint32_t* done;          // [8][sbx], the same
+    uint64_t* pay;           // [B][prec] record words: chunk c of chain b at b * prec + c * PAY
+    int sbx;                 // words per stripe of tickets and done (a multiple of 64)
+    int prec;                // record words per chain (a multiple of 32)
     int B, L;
     int nc;                  // chunk slots per chain
     int nk, K;               // checkpoints per chain, residues between checkpoints
     int bx;                  // chains per stripe: ceil(B / 8)
 };
+
+// A chain's look-back words.  The chains of one stripe run on one XCD, and no
+// 256-byte block holds words of two stripes: tickets and done counters are
+// stored stripe by stripe in blocks of 64 words, and each chain's records
+// start on a 256-byte boundary.  So every cache line of the sync buffer is
+// written and read by one XCD only, and no line is shared between the
+// non-coherent L2s of two XCDs.
+__device__ __forceinline__ uint32_t* ticket_of(const ChunkArgs& a, int b) {
+    return a.tickets + (size_t)(b & 7) * a.sbx + (b >> 3);
+}
+__device__ __forceinline__ uint32_t* done_of(const ChunkArgs& a, int b) {
+    return a.done + (size_t)(b & 7) * a.sbx + (b >> 3);
+}
+__device__ __forceinline__ uint64_t* records_of(const ChunkArgs& a, int b) {
+    return a.pay + (size_t)b * a.prec;
+}
 
 
 // Positions.  Stripe x = blockIdx.x mod 8 holds the chains b = x + 8i, i < m
@@ -532,12 +550,14 @@ __device__ __forceinline__ Slot make_slot(const ChunkArgs& a, int C, int b, int 
 
 struct Stripe {
     int x, m;     // stripe and its chain count
-    int kk;       // chunk counts per lane in use; 0: slot order
-    int nr[kStripeLanes];  // chunk count of chain x + 8 (lane + 64k)
+    int kk;       // lengths per lane in use; 0: slot order
+    int n[kStripeLanes];       // clamped length of chain x + 8 (lane + 64k)
+    uint64_t bad[kStripeLanes];  // ballot: that length was outside [0, L]
 };
 
-// False if the stripe has no chains.
-__device__ __forceinline__ bool stripe_init(const ChunkArgs& a, int C, Stripe& st) {
+// False if the stripe has no chains.  The stripe's lengths stay in
+// registers, so finding a slot needs no further load.
+__device__ __forceinline__ bool stripe_init(const ChunkArgs& a, Stripe& st) {
     const int lane = threadIdx.x;
     st.x = (int)(blockIdx.x & 7);
     st.m = a.B > st.x ? (a.B - st.x + 7) / 8 : 0;
@@ -546,11 +566,14 @@ __device__ __forceinline__ bool stripe_init(const ChunkArgs& a, int C, Stripe& s
     for (int k = 0; k < kStripeLanes; ++k) {
         const int i = lane + 64 * k;
         int n = 0;
+        bool bad = false;
         if (k < st.kk && i < st.m) {
-            n = a.lengths[st.x + 8 * i];
-            n = n < 0 ? 0 : (n > a.L ? a.L : n);
+            const int raw = a.lengths[st.x + 8 * i];
+            n = raw < 0 ? 0 : (raw > a.L ? a.L : raw);
+            bad = raw != n;
         }
-        st.nr[k] = (n + C - 1) / C;
+        st.n[k] = n;
+        st.bad[k] = __builtin_amdgcn_ballot_w64(bad);
     }
     return st.m > 0;
 }
@@ -564,6 +587,9 @@ __device__ __forceinline__ Slot slot_at(const ChunkArgs& a, int C, const Stripe&
         const int i = (j - r * m + (int)(((int64_t)r * kRowTurn) % m)) % m;
         return make_slot(a, C, st.x + 8 * i, r);
     }
+    int nr[kStripeLanes];  // chunk counts
+#pragma unroll
+    for (int k = 0; k < kStripeLanes; ++k) nr[k] = (st.n[k] + C - 1) / C;
     // cnt_r = #{i : nreal_i > r}; real rows first, then padding rows of
     // m - cnt_r slots each.
     int row_start = 0, r = -1, start = 0;
@@ -573,7 +599,7 @@ __device__ __forceinline__ Slot slot_at(const ChunkArgs& a, int C, const Stripe&
             int cnt = 0;
 #pragma unroll
             for (int k = 0; k < kStripeLanes; ++k)
-                if (k < kk) cnt += __builtin_popcountll(__builtin_amdgcn_ballot_w64(st.nr[k] > rr));
+                if (k < kk) cnt += __builtin_popcountll(__builtin_amdgcn_ballot_w64(nr[k] > rr));
             const int size = pass == 0 ? cnt : m - cnt;
             if (j < row_start + size) {
                 r = rr;
@@ -585,25 +611,28 @@ __device__ __forceinline__ Slot slot_at(const ChunkArgs& a, int C, const Stripe&
         }
     }
     int q = j - start;  // rank within the row
-    int b = st.x;
-    bool found = false;
+    int b = st.x, n = 0;
+    bool found = false, bad = false;
 #pragma unroll
     for (int k = 0; k < kStripeLanes; ++k) {
         if (k >= kk) break;
-        const bool in = (lane + 64 * k < m) && (real ? st.nr[k] > r : st.nr[k] <= r);
+        const bool in = (lane + 64 * k < m) && (real ? nr[k] > r : nr[k] <= r);
         const uint64_t mask = __builtin_amdgcn_ballot_w64(in);
         const int cnt = __builtin_popcountll(mask);
         if (!found && q < cnt) {
             const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-            const uint64_t hit = __builtin_amdgcn_ballot_w64(in && below == q);
-            b = st.x + 8 * (__builtin_ctzll(hit) + 64 * k);
+            const int hl = __builtin_ctzll(__builtin_amdgcn_ballot_w64(in && below == q));
+            b = st.x + 8 * (hl + 64 * k);
+            n = __builtin_amdgcn_readlane(st.n[k], hl);
+            bad = (st.bad[k] >> hl) & 1ull;
             found = true;
         } else if (!found) {
             q -= cnt;
         }
     }
-    return make_slot(a, C, b, r);
+    const int nreal = (n + C - 1) / C;
+    return Slot{b, r, n, nreal, r < nreal, bad};
 }
 
 __device__ __forceinline__ int take_ticket(uint32_t* t) {
@@ -624,17 +653,17 @@ __device__ __forceinline__ uint32_t chain_done_issue(const ChunkArgs& a, int b) 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);  // this wave's record stores have landed
     uint32_t old = 0;
-    if (threadIdx.x == 0) old = atomic_inc(a.done + b);
+    if (threadIdx.x == 0) old = atomic_inc(done_of(a, b));
     return old;
 }
 template <int PAY>
 __device__ __forceinline__ void chain_done_finish(const ChunkArgs& a, int b, int nreal, uint32_t old) {
     if (__builtin_amdgcn_readfirstlane(old) == (uint32_t)(nreal - 1)) {
-        uint64_t* pay = a.pay + (size_t)b * a.nc * PAY;
+        uint64_t* pay = records_of(a, b);
         for (int j = threadIdx.x; j < nreal * PAY; j += 64) record_store(pay + j, 0ull);
         if (threadIdx.x == 0) {
-            word_store(a.tickets + b, 0u);
-            word_store(a.done + b, 0u);
+            word_store(ticket_of(a, b), 0u);
+            word_store(done_of(a, b), 0u);
         }
     }
 }
@@ -643,7 +672,7 @@ __device__ __forceinline__ void chain_done_finish(const ChunkArgs& a, int b, int
 // [0, nreal): the sync buffer was not clean (TPL_A2B_STATUS_SYNC reported;
 // the wave then touches nothing of the chain).
 __device__ __forceinline__ int chain_ticket(const ChunkArgs& a, int b, int nreal) {
-    const int t = take_ticket(a.tickets + b);
+    const int t = take_ticket(ticket_of(a, b));
     if (t < 0 || t >= nreal) {
         if (threadIdx.x == 0) set_status(a.status, TPL_A2B_STATUS_SYNC);
         return -1;
@@ -664,20 +693,38 @@ __device__ __forceinline__ void publish(uint64_t* p, const double (&v)[N]) {
 // Waits until the record at p (read by every valid lane) is written and
 // returns it in v (bounded: false after kSpinLimit polls, with
 // TPL_A2B_STATUS_TIMEOUT reported).  Invalid lanes read nothing.
-constexpr int kSpinLimit = 1 << 21;
+//
+// The wait is gentle, because polling competes with the waves it waits for.
+// Waves of a CU share its vector-memory path, and a wave that is to publish
+// a record still has vector-memory instructions to issue (its angle loads,
+// its record stores).  With every valid lane re-reading its whole record
+// every 64 clocks, the waiting waves of a CU kept a wave there that held an
+// earlier chunk from issuing them for milliseconds to seconds (traced on
+// MI355X: the holder took its ticket, then reached its next vector-memory
+// instruction only when its waiters gave up), and the whole chain waited on
+// it.  So: the first poll reads the whole record (usually it is there); later
+// polls come only from lanes whose record is still missing, read one word
+// (the last, then the whole record once it shows: the words of a record land
+// in any order, so each is checked), and sleep 64 * nap clocks between, nap
+// doubling up to kMaxNap.
+constexpr int kSpinLimit = 1 << 20;
+constexpr int kMaxNap = 32;
 template <int N>
 __device__ __forceinline__ bool wait_record(const uint64_t* p, bool valid, double (&v)[N],
                                             uint32_t* status) {
+    bool ok = !valid;
+    int nap = 1;
     for (int spins = 0; spins < kSpinLimit; ++spins) {
-        bool ok = true;
-        if (valid) {
+        if (!ok && (spins == 0 || record_load(p + (N - 1)) != 0)) {
+            ok = true;
 #pragma unroll
             for (int e = 0; e < N; ++e) v[e] = __longlong_as_double(static_cast<long long>(record_load(p + e)));
 #pragma unroll
             for (int e = 0; e < N; ++e) ok = ok && __double_as_longlong(v[e]) != 0;
         }
-        if (__builtin_amdgcn_ballot_w64(valid && !ok) == 0) return true;
-        __builtin_amdgcn_s_sleep(1);
+        if (__builtin_amdgcn_ballot_w64(!ok) == 0) return true;
+        for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(1);
+        nap = nap < kMaxNap ? 2 * nap : kMaxNap;
     }
     if (threadIdx.x == 0) set_status(status, TPL_A2B_STATUS_TIMEOUT);
     return false;
@@ -706,13 +753,13 @@ __device__ __forceinline__ bool wait_record(const uint64_t* p, bool valid, doubl
 // as I; uniform over the wave).
 __device__ void fwd_lookback(const ChunkArgs& a, int b, int c, double P[kF]) {
     const int lane = threadIdx.x;
-    const uint64_t* pay = a.pay + (size_t)b * a.nc * kFwdPay;
+    const uint64_t* pay = records_of(a, b);
     const int lo = c & ~63;
-    if (lo > 0) {  // the previous group's inclusive prefix
-        double I[kF];
-        if (!wait_record<kF>(pay + (size_t)(lo - 1) * kFwdPay, true, I, a.status)) return;
+    if (lo > 0) {  // the previous group's inclusive prefix, read by lane 0
+        double I[kF] = {};
+        if (!wait_record<kF>(pay + (size_t)(lo - 1) * kFwdPay, lane == 0, I, a.status)) return;
 #pragma unroll
-        for (int e = 0; e < kF; ++e) P[e] = I[e];
+        for (int e = 0; e < kF; ++e) P[e] = read_lane(I[e], 0);
     }
     if (c == lo) return;
     const int j = lo + lane;
@@ -737,7 +784,7 @@ __device__ void fwd_lookback(const ChunkArgs& a, int b, int c, double P[kF]) {
 // chain b (nreal chunks); carry enters as 0.
 __device__ void bwd_lookback(const ChunkArgs& a, int b, int c, int nreal, double carry[6]) {
     const int lane = threadIdx.x;
-    const uint64_t* pay = a.pay + (size_t)b * a.nc * kBwdPay;
+    const uint64_t* pay = records_of(a, b);
     const int lo = c & ~63, hi = min(lo + 64, nreal);  // c's group, clipped to the chain
     const int j = c + 1 + lane;  // lane k reads chunk c + 1 + k
     const bool valid = j < hi;
@@ -747,11 +794,11 @@ __device__ void bwd_lookback(const ChunkArgs& a, int b, int c, int nreal, double
     const int src = hi - c - 1 > 1 ? 63 : 0;   // a window of one chunk: lane 0
 #pragma unroll
     for (int e = 0; e < 6; ++e) carry[e] = read_lane(v[e], src);
-    if (hi < nreal) {  // the next group's inclusive suffix
-        double in[6];
-        if (!wait_record<6>(pay + (size_t)hi * kBwdPay, true, in, a.status)) return;
+    if (hi < nreal) {  // the next group's inclusive suffix, read by lane 0
+        double in[6] = {};
+        if (!wait_record<6>(pay + (size_t)hi * kBwdPay, lane == 0, in, a.status)) return;
 #pragma unroll
-        for (int e = 0; e < 6; ++e) carry[e] += in[e];
+        for (int e = 0; e < 6; ++e) carry[e] += read_lane(in[e], 0);
     }
 }
 
@@ -839,7 +886,7 @@ __device__ __forceinline__ void fwd_slot(const ChunkArgs& a, const Slot& sl, T* 
     // of its group; the last chunk of a group publishes its inclusive prefix
     // instead, for the next group; the chain's last chunk publishes nothing.
     if (nreal > 1) {
-        uint64_t* my = a.pay + ((size_t)b * a.nc + c) * kFwdPay;
+        uint64_t* my = records_of(a, b) + (size_t)c * kFwdPay;
         const bool last = c == nreal - 1;
         const bool boundary = (c & 63) == 63;
         if (!last && !boundary) publish<kF>(my, agg);
@@ -903,7 +950,7 @@ __global__ void __launch_bounds__(64) a2b_fwd_chunk(ChunkArgs a) {
     T* s_x = reinterpret_cast<T*>(s_dyn);
     Stripe st;
     const int j = (int)(blockIdx.x >> 3);
-    if (!stripe_init(a, C, st) || j >= st.m * a.nc) return;
+    if (!stripe_init(a, st) || j >= st.m * a.nc) return;
     bool tab = false;
     fwd_slot<T, R>(a, slot_at(a, C, st, j), s_x, s_tab, tab);
 }
@@ -1070,7 +1117,7 @@ __device__ __forceinline__ void bwd_slot(const ChunkArgs& a, const Slot& sl, dou
     // its inclusive suffix instead, for the previous group; chunk 0 publishes
     // nothing.
     if (nreal > 1) {
-        uint64_t* my = a.pay + ((size_t)b * a.nc + c) * kBwdPay;
+        uint64_t* my = records_of(a, b) + (size_t)c * kBwdPay;
         const bool boundary = (c & 63) == 0;
         if (!boundary) publish<6>(my, tot);
         double carry[6] = {0, 0, 0, 0, 0, 0};
@@ -1154,7 +1201,7 @@ __global__ void __launch_bounds__(64) a2b_bwd_chunk(ChunkArgs a) {
     __shared__ double s_tab[128];
     Stripe st;
     const int j = (int)(blockIdx.x >> 3);
-    if (!stripe_init(a, C, st) || j >= st.m * a.nc) return;
+    if (!stripe_init(a, st) || j >= st.m * a.nc) return;
     bool tab = false;
     bwd_slot<T, R>(a, slot_at(a, C, st, j), s_tab, s_cs, tab);
 }
@@ -1332,17 +1379,23 @@ size_t state_total(int B, int L, const tpl_a2b_config* c) {
     return align256((size_t)B * chunks(L, 64 * c->bwd_res_per_lane) * kF * sizeof(double));
 }
 
-// Sync: the forward's look-back words then the backward's, each as tickets
-// [B], done [B] (uint32), then records [B][nc][pay] (64-bit words).
+// Sync: the forward's look-back words then the backward's (ticket_of,
+// done_of and records_of give a chain's words).
 struct SyncRegion {
     size_t tickets, done, pay, end;
+    int sbx, prec;
 };
+// Tickets and done counters [8][sbx] (sbx = ceil(B / 8) rounded up to 64
+// words), then records [B][prec] (prec = nc * pay rounded up to 32 words).
 SyncRegion sync_region(size_t base, int B, int nc, int pay) {
     SyncRegion r{};
+    const int bx = (B + 7) / 8;
+    r.sbx = (bx + 63) / 64 * 64;
+    r.prec = (int)(((int64_t)nc * pay + 31) / 32 * 32);
     r.tickets = base;
-    r.done = r.tickets + (size_t)B * 4;
-    r.pay = align256(r.done + (size_t)B * 4);
-    r.end = align256(r.pay + (size_t)B * nc * pay * sizeof(uint64_t));
+    r.done = r.tickets + (size_t)8 * r.sbx * 4;
+    r.pay = align256(r.done + (size_t)8 * r.sbx * 4);
+    r.end = align256(r.pay + (size_t)B * r.prec * sizeof(uint64_t));
     return r;
 }
 struct SyncLayout {
@@ -1412,6 +1465,8 @@ void bind_sync(ChunkArgs& a, void* sync, const SyncRegion& r) {
     a.tickets = reinterpret_cast<uint32_t*>(base + r.tickets);
     a.done = reinterpret_cast<uint32_t*>(base + r.done);
     a.pay = reinterpret_cast<uint64_t*>(base + r.pay);
+    a.sbx = r.sbx;
+    a.prec = r.prec;
 }
 
 // The forward's look-back words for callers that cannot allocate

@@ -53,6 +53,14 @@ PARTS = [
 ]
 
 
+# Per-source hipcc flags.  a2b.hip: its atomics are issued by one lane (the
+# status bits, by the lanes that found an error), so LLVM's wave-level atomic
+# combining gains nothing there and adds a readfirstlane, with a wait for the
+# result, right after each atomic.  Without it the backward's ticket
+# increment goes out before its loads and is awaited with them.
+EXTRA_HIP_FLAGS = {"a2b.hip": ["-mllvm", "-amdgpu-atomic-optimizer-strategy=None"]}
+
+
 def lib_path(name: str = LIB_NAME) -> str:
     return os.path.join(PKG, name)
 
@@ -89,6 +97,7 @@ def build_lib(verbose=True, name: str = LIB_NAME, source: str = "a2b.hip") -> st
             "-fPIC", "-shared", "-Wall", "-Werror", "-Wno-unused-command-line-argument",
             # No packed FP32 instructions anywhere in this library.
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        ] + EXTRA_HIP_FLAGS.get(source, []) + [
             "-I", INCLUDE, os.path.join(CSRC, source), "-o", tmp,
         ]
     try:

@@ -635,12 +635,6 @@ __device__ __forceinline__ Slot slot_at(const ChunkArgs& a, int C, const Stripe&
     return Slot{b, r, n, nreal, r < nreal, bad};
 }
 
-__device__ __forceinline__ int take_ticket(uint32_t* t) {
-    uint32_t v = 0;
-    if (threadIdx.x == 0) v = atomic_inc(t);
-    return (int)__builtin_amdgcn_readfirstlane(v);
-}
-
 // Called once by every wave that took a ticket of chain b, after its last
 // access to the chain's records, in two halves so that the counter's round
 // trip overlaps the wave's own stores: chain_done_issue counts the wave done
@@ -670,14 +664,25 @@ __device__ __forceinline__ void chain_done_finish(const ChunkArgs& a, int b, int
 
 // The ticket of a wave of chain b (nreal chunks), or -1 if it is outside
 // [0, nreal): the sync buffer was not clean (TPL_A2B_STATUS_SYNC reported;
-// the wave then touches nothing of the chain).
-__device__ __forceinline__ int chain_ticket(const ChunkArgs& a, int b, int nreal) {
-    const int t = take_ticket(ticket_of(a, b));
+// the wave then touches nothing of the chain).  In two halves:
+// ticket_issue sends the increment (lane 0) and ticket_read waits for its
+// result.  Vector-memory results return in issue order, so loads issued
+// between the two halves do not delay the ticket.
+__device__ __forceinline__ uint32_t ticket_issue(const ChunkArgs& a, int b) {
+    uint32_t v = 0;
+    if (threadIdx.x == 0) v = atomic_inc(ticket_of(a, b));
+    return v;
+}
+__device__ __forceinline__ int ticket_read(const ChunkArgs& a, uint32_t v, int nreal) {
+    const int t = (int)__builtin_amdgcn_readfirstlane(v);
     if (t < 0 || t >= nreal) {
         if (threadIdx.x == 0) set_status(a.status, TPL_A2B_STATUS_SYNC);
         return -1;
     }
     return t;
+}
+__device__ __forceinline__ int chain_ticket(const ChunkArgs& a, int b, int nreal) {
+    return ticket_read(a, ticket_issue(a, b), nreal);
 }
 
 // Lane 0 publishes the uniform values v[0..N) as the record at p.
@@ -1033,10 +1038,16 @@ __device__ __forceinline__ void bwd_slot(const ChunkArgs& a, const Slot& sl, dou
             for (int e = 0; e < kF; ++e) P[e] = ck[e];
         }
     };
+    // The ticket goes out first and the guessed chunk's loads after it, so
+    // the ticket returns first and a wrong guess is reloaded at once.
     const int guess = nreal - 1 - s;  // the chunk of this slot if tickets follow slot order
+    uint32_t tv = nreal > 1 ? ticket_issue(a, b) : 0u;
     load(guess);
     load_ckpt(guess);
-    const int t = nreal > 1 ? chain_ticket(a, b, nreal) : 0;
+    // Keeps the compiler from reading the ticket (and so waiting for it)
+    // before the loads above are issued.
+    __asm__ volatile("" : "+v"(tv) : : "memory");
+    const int t = nreal > 1 ? ticket_read(a, tv, nreal) : 0;
     if (t < 0) return;
     const int c = nreal - 1 - t;
     if (c != guess) {

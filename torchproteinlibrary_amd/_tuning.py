@@ -12,8 +12,10 @@ tensor (its storage pointer and version), so a loop that reuses its lengths
 tensor reads them once; calls made while a stream is being captured never
 read them and never tune: they use the shape's last bucket and cached config,
 or the untuned default.  The first eager call of a key times every candidate
-on the caller's own angles and lengths (forward+backward, median of a few
-runs, in the native tuner ``tpl_a2b_tune``) and caches the fastest.
+on the caller's own angles and lengths and caches the fastest: the native
+tuner ``tpl_a2b_tune`` runs each candidate's forward+backward steps back to
+back, as a training loop does, with the candidates taking turns over a few
+rounds, and keeps each one's median.
 
 Environment: ``TPL_A2B_TUNE=0`` disables tuning (default config always).
 """

@@ -1716,51 +1716,60 @@ int tpl_a2b_tune(int dtype, const void* angles, const int32_t* lengths,
         (void)hipEventDestroy(e0);
         return TPL_A2B_ERR_HIP;
     }
-    float best_ms = -1.f;
-    int best_i = -1;
-    rc = TPL_A2B_OK;
-    std::vector<float> times;
     size_t sync_max = 0;  // the caller sized `sync` for the largest candidate
-    for (int c = 0; c < n_candidates; ++c)
-        if (valid_config(candidates + c))
-            sync_max = std::max(sync_max, tpl_a2b_sync_bytes(dtype, batch, angles_stride, candidates + c));
-    for (int c = 0; c < n_candidates && rc == TPL_A2B_OK; ++c) {
-        const tpl_a2b_config* cf = candidates + c;
-        float ms = -1.f;
-        if (valid_config(cf) && fits_dtype(dtype, cf)) {
-            times.clear();
-            // Every call leaves the look-back words zero; clearing them before
-            // each candidate keeps one failed candidate from affecting the next.
+    std::vector<char> ok(n_candidates, 0);
+    for (int c = 0; c < n_candidates; ++c) {
+        ok[c] = valid_config(candidates + c) && fits_dtype(dtype, candidates + c);
+        if (ok[c]) sync_max = std::max(sync_max, tpl_a2b_sync_bytes(dtype, batch, angles_stride, candidates + c));
+    }
+    // Steps are timed back to back, kStepsPerRun per event pair, as a caller's
+    // loop runs them; the candidates take turns, round by round, so that a
+    // drift of the clock or of the machine's load falls on all of them alike.
+    // Round 0 is the warm-up.  Every call leaves the look-back words zero;
+    // clearing them before each run keeps one failed candidate from
+    // affecting the next.
+    constexpr int kStepsPerRun = 6;
+    std::vector<std::vector<float>> times(n_candidates);
+    rc = TPL_A2B_OK;
+    for (int round = 0; round <= reps && rc == TPL_A2B_OK; ++round) {
+        for (int c = 0; c < n_candidates && rc == TPL_A2B_OK; ++c) {
+            if (!ok[c]) continue;
+            const tpl_a2b_config* cf = candidates + c;
             rc = tpl_a2b_sync_clear(sync, sync_max, stream);
             if (rc) break;
-            for (int it = 0; it <= reps && rc == TPL_A2B_OK; ++it) {
-                if (hipEventRecord(e0, s) != hipSuccess) { rc = TPL_A2B_ERR_HIP; break; }
+            if (hipEventRecord(e0, s) != hipSuccess) { rc = TPL_A2B_ERR_HIP; break; }
+            for (int it = 0; it < kStepsPerRun && rc == TPL_A2B_OK; ++it) {
                 rc = tpl_a2b_forward(dtype, angles, coords, lengths, state, sync, nullptr, batch,
                                      angles_stride, cf, stream);
                 if (rc) break;
                 rc = tpl_a2b_backward(dtype, grad_angles, grad_coords, angles, lengths, state, sync,
                                       nullptr, batch, angles_stride, cf, stream);
-                if (rc) break;
-                if (hipEventRecord(e1, s) != hipSuccess) { rc = TPL_A2B_ERR_HIP; break; }
-                if (hipEventSynchronize(e1) != hipSuccess) { rc = TPL_A2B_ERR_HIP; break; }
-                float t = 0.f;
-                if (hipEventElapsedTime(&t, e0, e1) != hipSuccess) { rc = TPL_A2B_ERR_HIP; break; }
-                if (it > 0) times.push_back(t);  // the first run is the warm-up
             }
-            if (rc == TPL_A2B_OK && !times.empty()) {
-                std::sort(times.begin(), times.end());
-                ms = times[times.size() / 2];
-                if (best_i < 0 || ms < best_ms) {
-                    best_ms = ms;
-                    best_i = c;
-                }
-            }
+            if (rc) break;
+            if (hipEventRecord(e1, s) != hipSuccess) { rc = TPL_A2B_ERR_HIP; break; }
+            if (hipEventSynchronize(e1) != hipSuccess) { rc = TPL_A2B_ERR_HIP; break; }
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, e0, e1) != hipSuccess) { rc = TPL_A2B_ERR_HIP; break; }
+            if (round > 0) times[c].push_back(t / kStepsPerRun);
         }
-        if (ms_per_candidate) ms_per_candidate[c] = ms;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
+    float best_ms = -1.f;
+    int best_i = -1;
+    for (int c = 0; c < n_candidates; ++c) {
+        float ms = -1.f;
+        if (!times[c].empty()) {
+            std::sort(times[c].begin(), times[c].end());
+            ms = times[c][times[c].size() / 2];
+            if (best_i < 0 || ms < best_ms) {
+                best_ms = ms;
+                best_i = c;
+            }
+        }
+        if (ms_per_candidate) ms_per_candidate[c] = ms;
+    }
     if (best_i < 0) return TPL_A2B_ERR_CONFIG;
     *best = candidates[best_i];
     return TPL_A2B_OK;

@@ -356,15 +356,38 @@ __device__ __forceinline__ void lane_load_rows(const T* p, int L, int cnt, int a
     }
 }
 
+// Vector types of 4, 8 and 16 bytes that __builtin_nontemporal_store takes.
+template <int U>
+struct StreamUnit;
+template <>
+struct StreamUnit<2> {
+    using type = uint16_t;
+};
+template <>
+struct StreamUnit<4> {
+    using type = uint32_t;
+};
+template <>
+struct StreamUnit<8> {
+    typedef unsigned int type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct StreamUnit<16> {
+    typedef unsigned int type __attribute__((ext_vector_type(4)));
+};
+
+// Whole aligned runs go out as nontemporal (streaming) stores: the outputs
+// are never read back by the kernel that writes them.
 template <typename T, int N>
 __device__ __forceinline__ void lane_store(T* p, int cnt, const T (&v)[N]) {
     constexpr int U = unit_bytes<T, N>();
-    using V = typename Unit<U>::type;
+    using V = typename StreamUnit<U>::type;
     if (cnt >= N && (reinterpret_cast<uintptr_t>(p) % U) == 0) {
         V w[N * sizeof(T) / U];
         __builtin_memcpy(w, v, sizeof(v));
 #pragma unroll
-        for (int i = 0; i < (int)(N * sizeof(T) / U); ++i) reinterpret_cast<V*>(p)[i] = w[i];
+        for (int i = 0; i < (int)(N * sizeof(T) / U); ++i)
+            __builtin_nontemporal_store(w[i], reinterpret_cast<V*>(p) + i);
     } else {
 #pragma unroll
         for (int i = 0; i < N; ++i)

@@ -10,9 +10,9 @@
 //   1. each thread takes a contiguous run of residues and multiplies their
 //      residue steps S_i = T_CA(phi_i) T_C(psi_i) T_N(omega_i) (FP64 3x4), the
 //      transform from residue i's N frame to residue i+1's;
-//   2. a workgroup scan of those products (Hillis-Steele through LDS) gives
-//      each thread the N frame of its first residue, and a scan of the atom
-//      counts gives its first atom;
+//   2. a workgroup scan of those products (DPP within each wave, the waves'
+//      totals through LDS) gives each thread the N frame of its first
+//      residue, and a scan of the atom counts gives its first atom;
 //   3. each thread walks its residues: N, CA, the side chain, C, writing atoms.
 //
 // The backward uses dL/dalpha = u . (sum_{x in subtree} x cross g
@@ -21,8 +21,9 @@
 // 171-184, reduces to this since F x = u cross (x - p)).  u and p follow from
 // the forward's coordinates, so the backward reads those and needs no frames.
 // The subtree of a backbone node is the rest of its residue plus every later
-// residue: a workgroup suffix scan of per-run (sum g, sum x cross g) supplies
-// the downstream part, and each thread walks its residues once more.
+// residue: a workgroup scan of per-run (sum g, sum x cross g), with runs in
+// reverse thread order so that it is a prefix scan, supplies the downstream
+// part, and each thread walks its residues once more.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -35,9 +36,8 @@ namespace {
 using namespace a2c;
 
 // Workgroup sizes: 256 threads, or 512 for long structures (see use_large_fwd
-// and use_large_bwd), where a thread then walks half as many residues.  Both
-// fit in LDS (forward 38.8 / 64.4 KB, backward 14 / 27 KB) and in two waves
-// per SIMD of VGPRs.
+// and use_large_bwd), where a thread then walks half as many residues.  LDS:
+// forward 13.6 / 14.0 KB (the tables), backward 0.8 / 1.0 KB.
 constexpr int kThreadsSmall = 256, kThreadsLarge = 512;
 
 __constant__ Residue kRes[kTypes] = A2C_RESIDUES;
@@ -56,16 +56,30 @@ struct Tables {
     double tab[128];    // the sincos table of sincos_tab.h
 };
 
+// The table sincos (sincos_tab.h) over the workgroup's LDS copy of its table.
+struct TabSinCos {
+    const double* tab;
+    __device__ __forceinline__ void operator()(double x, double* s, double* c) const {
+        sincos_tab(x, tab, s, c);
+    }
+};
+
+// The sines and cosines of the constant angles come from the table sincos
+// (within 2.2e-16 of the math library's): the library's FP64 sincos took
+// 4-6 us of every workgroup's start.
 template <int NT>
 __device__ void fill_tables(Tables& tb) {
+    for (int k = threadIdx.x; k < 128; k += NT) tb.tab[k] = kTabCS[k];
+    __syncthreads();  // tb.tab, for the rest
+    const TabSinCos sc{tb.tab};
     for (int k = threadIdx.x; k < kTypes * kMaxAtoms; k += NT) {
         const Atom at = kAtom[k / kMaxAtoms][k % kMaxAtoms];
-        local_position(at, tb.loc[k]);
+        local_position_with(at, tb.loc[k], sc);
     }
     for (int k = threadIdx.x; k < kTypes * kMaxSide; k += NT) {
         const Side s = kSide[k / kMaxSide][k % kMaxSide];
-        sincos(s.beta, &tb.side[k][1], &tb.side[k][0]);
-        sincos(s.alpha, &tb.side[k][3], &tb.side[k][2]);
+        sc(s.beta, &tb.side[k][1], &tb.side[k][0]);
+        sc(s.alpha, &tb.side[k][3], &tb.side[k][2]);
         tb.side[k][4] = s.d;
         tb.side_i[k] = (s.row & 0xff) | (s.first << 8) | (s.count << 16);
     }
@@ -73,11 +87,9 @@ __device__ void fill_tables(Tables& tb) {
         const Residue r = kRes[threadIdx.x];
         tb.res_i[threadIdx.x] = r.natoms | (r.nca << 8) | (r.nside << 16);
     }
-    for (int k = threadIdx.x; k < 128; k += NT) tb.tab[k] = kTabCS[k];
-    if (threadIdx.x == 0) {
-        sincos(geo::C_N_CA, &tb.bb[0][1], &tb.bb[0][0]);
-        sincos(geo::N_CA_C, &tb.bb[1][1], &tb.bb[1][0]);
-        sincos(geo::CA_C_N, &tb.bb[2][1], &tb.bb[2][0]);
+    if (threadIdx.x < 3) {
+        const double beta = threadIdx.x == 0 ? geo::C_N_CA : (threadIdx.x == 1 ? geo::N_CA_C : geo::CA_C_N);
+        sc(beta, &tb.bb[threadIdx.x][1], &tb.bb[threadIdx.x][0]);
     }
 }
 
@@ -117,6 +129,89 @@ __device__ __forceinline__ void identity(double m[12]) {
 __device__ __forceinline__ void apply(const double m[12], const double v[3], double x[3]) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) x[i] = m[4 * i] * v[0] + m[4 * i + 1] * v[1] + m[4 * i + 2] * v[2] + m[4 * i + 3];
+}
+
+// ---- wave-level scans (DPP; all 64 lanes of the wave active) ----
+// GFX9 DPP controls: row shifts, row broadcasts, wave shift right by one.
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143, kWaveShr1 = 0x138;
+
+// DPP move of a double; lanes that receive nothing (or rows masked off)
+// keep `old`.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_double(double v, double old) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(v), CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(v), CTRL, ROW_MASK, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_int(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, false);
+}
+
+__device__ __forceinline__ double ident12(int k) {
+    return (k == 0 || k == 5 || k == 10) ? 1.0 : 0.0;
+}
+
+// One Hillis-Steele step of the ordered product: m <- (source lane's m) * m,
+// with the identity where a lane receives nothing.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void frame_scan_step(double m[12]) {
+    double g[12], r[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) g[k] = dpp_double<CTRL, ROW_MASK>(m[k], ident12(k));
+    compose(g, m, r);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) m[k] = r[k];
+}
+
+// Inclusive ordered product over the wave's lanes (lane l ends with
+// m_0 * ... * m_l).
+__device__ __forceinline__ void wave_scan_frames(double m[12]) {
+    frame_scan_step<kRowShr1, 0xf>(m);
+    frame_scan_step<kRowShr2, 0xf>(m);
+    frame_scan_step<kRowShr4, 0xf>(m);
+    frame_scan_step<kRowShr8, 0xf>(m);
+    frame_scan_step<kRowBcast15, 0xa>(m);
+    frame_scan_step<kRowBcast31, 0xc>(m);
+}
+
+// Inclusive prefix sums over the wave's lanes.
+__device__ __forceinline__ double wave_scan_double(double v) {
+    v += dpp_double<kRowShr1, 0xf>(v, 0.0);
+    v += dpp_double<kRowShr2, 0xf>(v, 0.0);
+    v += dpp_double<kRowShr4, 0xf>(v, 0.0);
+    v += dpp_double<kRowShr8, 0xf>(v, 0.0);
+    v += dpp_double<kRowBcast15, 0xa>(v, 0.0);
+    v += dpp_double<kRowBcast31, 0xc>(v, 0.0);
+    return v;
+}
+__device__ __forceinline__ int wave_scan_int(int v) {
+    v += dpp_int<kRowShr1, 0xf>(v);
+    v += dpp_int<kRowShr2, 0xf>(v);
+    v += dpp_int<kRowShr4, 0xf>(v);
+    v += dpp_int<kRowShr8, 0xf>(v);
+    v += dpp_int<kRowBcast15, 0xa>(v);
+    v += dpp_int<kRowBcast31, 0xc>(v);
+    return v;
+}
+
+// The structure's length: the first 0 byte of its sequence row, or
+// seq_stride (s_len enters as seq_stride).  Each thread reads 4 consecutive
+// bytes per round, the loads issued together.
+template <int NT>
+__device__ __forceinline__ void find_length(const uint8_t* seq, int seq_stride, int* s_len) {
+    constexpr int kPer = 4;
+    for (int i0 = (int)threadIdx.x * kPer; i0 < seq_stride; i0 += NT * kPer) {
+        uint8_t c[kPer];
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) c[e] = i0 + e < seq_stride ? seq[i0 + e] : (uint8_t)1;
+        int first = -1;
+#pragma unroll
+        for (int e = kPer - 1; e >= 0; --e)
+            if (c[e] == 0) first = i0 + e;
+        if (first >= 0) atomicMin(s_len, first);
+    }
 }
 
 template <typename T>
@@ -180,16 +275,16 @@ struct Run {
 
 template <int NT, typename T>
 __device__ bool prepare(const uint8_t* seq, int seq_stride, const T* ang, int ang_stride,
-                        int atoms_stride, uint32_t* status, const Tables& tb, double (*s_m)[NT],
-                        int* s_c, int* s_flag, Run& run, double frame[12]) {
+                        int atoms_stride, uint32_t* status, const Tables& tb, double (*s_m)[NT / 64],
+                        int* s_c, int* s_flag, Run& run, double frame[12], ResAngles<T>& av0,
+                        uint8_t& code0) {
     const int tid = threadIdx.x;
     if (tid == 0) {
         s_flag[0] = seq_stride;  // length
         s_flag[1] = 0;           // status bits
     }
     __syncthreads();
-    for (int i = tid; i < seq_stride; i += NT)
-        if (seq[i] == 0) atomicMin(&s_flag[0], i);
+    find_length<NT>(seq, seq_stride, &s_flag[0]);
     __syncthreads();
     int L = s_flag[0];
     if (L > ang_stride) {
@@ -198,6 +293,12 @@ __device__ bool prepare(const uint8_t* seq, int seq_stride, const T* ang, int an
     }
     const int chunk = (L + NT - 1) / NT;
     const int r0 = min(L, tid * chunk), r1 = min(L, r0 + chunk);
+    // The walk's first residue: its angles and letter are loaded now, so
+    // that they arrive while the workgroup scans.
+    if (r0 < r1) {
+        av0.load(ang, ang_stride, r0);
+        code0 = seq[r0];
+    }
     double P[12];
     identity(P);
     int cnt = 0;
@@ -212,43 +313,46 @@ __device__ bool prepare(const uint8_t* seq, int seq_stride, const T* ang, int an
         residue_step(P, tb, ang, ang_stride, i);
     }
     if (bad) atomicOr(&s_flag[1], (int)TPL_A2C_STATUS_SEQ);
-    // inclusive scan: P_t <- P_0 ... P_t (ordered product), cnt likewise
-#pragma unroll
-    for (int k = 0; k < 12; ++k) s_m[k][tid] = P[k];
-    s_c[tid] = cnt;
-    for (int off = 1; off < NT; off <<= 1) {
-        __syncthreads();
-        double Q[12];
-        int c2 = 0;
-        if (tid >= off) {
-#pragma unroll
-            for (int k = 0; k < 12; ++k) Q[k] = s_m[k][tid - off];
-            c2 = s_c[tid - off];
-        }
-        __syncthreads();
-        if (tid >= off) {
-            double R[12];
-            compose(Q, P, R);
-#pragma unroll
-            for (int k = 0; k < 12; ++k) {
-                P[k] = R[k];
-                s_m[k][tid] = R[k];
-            }
-            cnt += c2;
-            s_c[tid] = cnt;
-        }
+    // Ordered product of the runs' steps and sum of their atom counts over
+    // the workgroup: inclusive within each wave by DPP, then across waves
+    // through LDS (the waves' totals; each wave composes those before it).
+    // frame = product of the earlier runs' steps, atom0 = their atoms.
+    constexpr int NW = NT / 64;
+    const int wv = tid >> 6;
+    int cinc = cnt;
+    if ((tid & ~63) * chunk < L) {  // a wave whose runs are all past the structure holds identities and zeros
+        cinc = wave_scan_int(cnt);
+        wave_scan_frames(P);
     }
+    if ((tid & 63) == 63) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) s_m[k][wv] = P[k];
+        s_c[wv] = cinc;
+    }
+    double E[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) E[k] = dpp_double<kWaveShr1, 0xf>(P[k], ident12(k));
     __syncthreads();
-    run.total = s_c[NT - 1];
-    if (run.total > atoms_stride && tid == 0) atomicOr(&s_flag[1], (int)TPL_A2C_STATUS_ATOMS);
-    if (tid == 0) {
-        identity(frame);
-        run.atom0 = 0;
-    } else {
+    double W[12];
+    identity(W);
+    int before = 0, total = 0;
 #pragma unroll
-        for (int k = 0; k < 12; ++k) frame[k] = s_m[k][tid - 1];
-        run.atom0 = s_c[tid - 1];
+    for (int w = 0; w < NW; ++w) {
+        if (w < wv) {
+            double Tw[12], R2[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) Tw[k] = s_m[k][w];
+            compose(W, Tw, R2);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) W[k] = R2[k];
+            before += s_c[w];
+        }
+        total += s_c[w];
     }
+    compose(W, E, frame);
+    run.atom0 = before + cinc - cnt;
+    run.total = total;
+    if (run.total > atoms_stride && tid == 0) atomicOr(&s_flag[1], (int)TPL_A2C_STATUS_ATOMS);
     __syncthreads();
     const int flags = s_flag[1];
     if (flags && tid == 0 && status) __hip_atomic_fetch_or(status, (uint32_t)flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -272,8 +376,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 2
                                                     uint32_t* __restrict__ atom_names, int atoms_stride,
                                                     uint32_t* status) {
     __shared__ Tables tb;
-    __shared__ double s_m[12][NT];
-    __shared__ int s_c[NT];
+    __shared__ double s_m[12][NT / 64];  // the waves' products
+    __shared__ int s_c[NT / 64];          // the waves' atom counts
     __shared__ int s_flag[2];
     const int b = blockIdx.x;
     const uint8_t* seq = seqs + (size_t)b * seq_stride;
@@ -283,12 +387,21 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 2
     __syncthreads();
     Run run;
     double G[12];
-    prepare<NT>(seq, seq_stride, ang, ang_stride, atoms_stride, status, tb, s_m, s_c, s_flag, run, G);
+    ResAngles<T> av;
+    uint8_t code = 0;
+    prepare<NT>(seq, seq_stride, ang, ang_stride, atoms_stride, status, tb, s_m, s_c, s_flag, run, G, av, code);
     int a = run.atom0;
     uint32_t* rn = res_names ? res_names + (size_t)b * atoms_stride : nullptr;
     uint32_t* an = atom_names ? atom_names + (size_t)b * atoms_stride : nullptr;
     for (int i = run.r0; i < run.r1; ++i) {
-        const int t = residue_type(seq[i]);
+        // the next residue's angles and letter go out before this one's work
+        ResAngles<T> nav;
+        uint8_t ncode = 0;
+        if (i + 1 < run.r1) {
+            nav.load(ang, ang_stride, i + 1);
+            ncode = seq[i + 1];
+        }
+        const int t = residue_type(code);
         const int ri = tb.res_i[t];
         const int natoms = ri & 0xff, nca = (ri >> 8) & 0xff, nside = ri >> 16;
         const double* loc = tb.loc[t * kMaxAtoms];
@@ -300,8 +413,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 2
             o[1] = (T)x[1];
             o[2] = (T)x[2];
         };
-        ResAngles<T> av;
-        av.load(ang, ang_stride, i);
         put(0, G);
         double s, c, F[12], CA[12];
 #pragma unroll
@@ -337,6 +448,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 2
 #pragma unroll
         for (int k = 0; k < 12; ++k) G[k] = CA[k];
         a += natoms;
+        av = nav;
+        code = ncode;
     }
     for (int k = 3 * run.total + threadIdx.x; k < 3 * atoms_stride; k += NT) out[k] = (T)0;
     if (rn)
@@ -379,16 +492,18 @@ __device__ void count_run(const uint8_t* seq, int seq_stride, int ang_stride, in
         s_flag[1] = 0;
     }
     __syncthreads();
-    for (int i = tid; i < seq_stride; i += NT)
-        if (seq[i] == 0) atomicMin(&s_flag[0], i);
+    find_length<NT>(seq, seq_stride, &s_flag[0]);
     __syncthreads();
     int L = s_flag[0];
     if (L > ang_stride) {
         if (tid == 0) atomicOr(&s_flag[1], (int)TPL_A2C_STATUS_LENGTH);
         L = 0;
     }
+    // Runs go to threads in reverse order (run q = NT - 1 - tid), so that
+    // sums over a run and every later run are prefix sums over threads.
     const int chunk = (L + NT - 1) / NT;
-    const int r0 = min(L, tid * chunk), r1 = min(L, r0 + chunk);
+    const int q = NT - 1 - tid;
+    const int r0 = min(L, q * chunk), r1 = min(L, r0 + chunk);
     int cnt = 0;
     bool bad = false;
     for (int i = r0; i < r1; ++i) {
@@ -399,17 +514,21 @@ __device__ void count_run(const uint8_t* seq, int seq_stride, int ang_stride, in
             cnt += res_i[t] & 0xff;
     }
     if (bad) atomicOr(&s_flag[1], (int)TPL_A2C_STATUS_SEQ);
-    s_c[tid] = cnt;
-    for (int off = 1; off < NT; off <<= 1) {
-        __syncthreads();
-        const int c2 = tid >= off ? s_c[tid - off] : 0;
-        __syncthreads();
-        cnt += c2;
-        s_c[tid] = cnt;
-    }
+    // atoms of this run and the later ones: prefix over threads, by DPP
+    // within the wave and the earlier waves' totals through LDS
+    constexpr int NW = NT / 64;
+    const int wv = tid >> 6;
+    const int cinc = wave_scan_int(cnt);
+    if ((tid & 63) == 63) s_c[wv] = cinc;
     __syncthreads();
-    run.total = s_c[NT - 1];
-    run.atom0 = tid ? s_c[tid - 1] : 0;
+    int before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        if (w < wv) before += s_c[w];
+        total += s_c[w];
+    }
+    run.total = total;
+    run.atom0 = total - (before + cinc);  // atoms of the runs before this one
     if (run.total > atoms_stride && tid == 0) atomicOr(&s_flag[1], (int)TPL_A2C_STATUS_ATOMS);
     __syncthreads();
     const int flags = s_flag[1];
@@ -458,8 +577,8 @@ __global__ void __launch_bounds__(NT) a2c_bwd(const uint8_t* __restrict__ seqs, 
                                               uint32_t* status) {
     __shared__ int s_res[kTypes];
     __shared__ int s_side[kTypes * kMaxSide];
-    __shared__ double s_v[6][NT];
-    __shared__ int s_c[NT];
+    __shared__ double s_v[6][NT / 64];  // the waves' sums
+    __shared__ int s_c[NT / 64];        // the waves' atom counts
     __shared__ int s_flag[2];
     const int b = blockIdx.x, tid = threadIdx.x;
     const uint8_t* seq = seqs + (size_t)b * seq_stride;
@@ -485,7 +604,9 @@ __global__ void __launch_bounds__(NT) a2c_bwd(const uint8_t* __restrict__ seqs, 
         for (int i = run.r0; i < run.r1; ++i) a += s_res[residue_type(seq[i])] & 0xff;
         atom_sums(xs, gs, run.atom0, a, run_sum);
     }
-    // suffix scan over runs: incl = sums over every atom from this run on
+    // incl = sums over every atom from this run on: runs are in reverse
+    // thread order, so an inclusive prefix sum over threads (DPP within the
+    // wave, then the earlier waves' totals through LDS)
     double v[6];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -493,18 +614,15 @@ __global__ void __launch_bounds__(NT) a2c_bwd(const uint8_t* __restrict__ seqs, 
         v[3 + c] = run_sum.xg[c];
     }
 #pragma unroll
-    for (int k = 0; k < 6; ++k) s_v[k][tid] = v[k];
-    for (int off = 1; off < NT; off <<= 1) {
-        __syncthreads();
-        double w[6];
+    for (int k = 0; k < 6; ++k) v[k] = wave_scan_double(v[k]);
+    if ((tid & 63) == 63) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) w[k] = tid + off < NT ? s_v[k][tid + off] : 0.0;
-        __syncthreads();
+        for (int k = 0; k < 6; ++k) s_v[k][tid >> 6] = v[k];
+    }
+    __syncthreads();
+    for (int w = 0; w < (tid >> 6); ++w) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            v[k] += w[k];
-            s_v[k][tid] = v[k];
-        }
+        for (int k = 0; k < 6; ++k) v[k] += s_v[k][w];
     }
     Sums incl;
 #pragma unroll
@@ -588,11 +706,11 @@ __global__ void __launch_bounds__(NT) a2c_bwd(const uint8_t* __restrict__ seqs, 
 
 // 512-thread workgroups when some structure may be longer than 256 residues:
 // always for the backward (mixed lengths 30-1000 at B=1024: 155 -> 129 us),
-// and for the forward only when the structures are fewer than the CUs (at
-// B=1024 512 threads measured slower: 85 -> 89 us, with fewer workgroups per
-// CU).
+// and for the forward only when the structures are at most as many as the
+// CUs (B=256 of 1000 residues: 33.7 -> 28.6 us; at B=1024 with mixed lengths
+// 512 threads measured slower, 57 -> 72 us, with fewer workgroups per CU).
 bool use_large_fwd(int batch, int ang_stride) {
-    return batch < 256 && ang_stride > kThreadsSmall;
+    return batch <= 256 && ang_stride > kThreadsSmall;
 }
 
 bool use_large_bwd(int ang_stride) {

@@ -212,29 +212,47 @@ A2C_HD inline int residue_type(uint8_t c) {
     }
 }
 
-// An atom's position in its group's frame.
-A2C_HD inline void local_position(const Atom& at, double v[3]) {
+// An atom's position in its group's frame, with sc(x, &sin, &cos) giving the
+// sine and cosine.
+template <typename SinCos>
+A2C_HD inline void local_position_with(const Atom& at, double v[3], SinCos sc) {
     const double h = at.a - kPi / 2.0;
+    double sa, ca, sh, ch, sb, cb;
     switch (at.kind) {
         case kLiteral: v[0] = at.r; v[1] = at.a; v[2] = at.b; return;
-        case kXZ: v[0] = at.r * cos(at.a); v[1] = 0.0; v[2] = at.r * sin(at.a); return;
+        case kXZ: sc(at.a, &sa, &ca); v[0] = at.r * ca; v[1] = 0.0; v[2] = at.r * sa; return;
         case kForkPlus:
         case kForkMinus: {
             const double s = at.kind == kForkPlus ? 1.0 : -1.0;
-            v[0] = at.r * sin(h);
-            v[1] = s * at.r * cos(h) * sin(at.b);
-            v[2] = at.r * cos(h) * cos(at.b);
+            sc(h, &sh, &ch);
+            sc(at.b, &sb, &cb);
+            v[0] = at.r * sh;
+            v[1] = s * at.r * ch * sb;
+            v[2] = at.r * ch * cb;
             return;
         }
         case kPlanePlus:
         case kPlaneMinus:
-            v[0] = at.r * sin(h);
+            sc(h, &sh, &ch);
+            v[0] = at.r * sh;
             v[1] = 0.0;
-            v[2] = (at.kind == kPlanePlus ? 1.0 : -1.0) * at.r * cos(h);
+            v[2] = (at.kind == kPlanePlus ? 1.0 : -1.0) * at.r * ch;
             return;
-        case kRing: v[0] = at.r * cos(at.a); v[1] = at.r * sin(at.a); v[2] = 0.0; return;
+        case kRing: sc(at.a, &sa, &ca); v[0] = at.r * ca; v[1] = at.r * sa; v[2] = 0.0; return;
         default: v[0] = v[1] = v[2] = 0.0; return;
     }
+}
+
+struct LibSinCos {
+    A2C_HD void operator()(double x, double* s, double* c) const {
+        *s = sin(x);
+        *c = cos(x);
+    }
+};
+
+// An atom's position in its group's frame (the math library's sin and cos).
+A2C_HD inline void local_position(const Atom& at, double v[3]) {
+    local_position_with(at, v, LibSinCos{});
 }
 
 }  // namespace a2c

@@ -189,8 +189,8 @@ int tpl_a2b_backward_plain(int dtype, void *grad_angles, const void *grad_coords
 
 /* Autotuner: times forward+backward for each candidate on the given buffers
  * (hipEvents on `stream` around 6 back-to-back steps; the candidates take
- * turns for `reps` rounds after one warm-up round, and each gets the median
- * of its rounds, in ms per step).  It
+ * turns for `reps` passes after one warm-up pass, and each gets the median
+ * of its passes, in ms per step).  It
  * writes the fastest candidate to *best and each candidate's ms to
  * ms_per_candidate (may be NULL; invalid candidates get -1).  state and sync
  * must be sized for the largest candidate (sync zeroed).  It synchronises the stream, so

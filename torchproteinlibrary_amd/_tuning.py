@@ -15,7 +15,7 @@ or the untuned default.  The first eager call of a key times every candidate
 on the caller's own angles and lengths and caches the fastest: the native
 tuner ``tpl_a2b_tune`` runs each candidate's forward+backward steps back to
 back, as a training loop does, with the candidates taking turns over a few
-rounds, and keeps each one's median.
+passes, and keeps each one's median.
 
 Environment: ``TPL_A2B_TUNE=0`` disables tuning (default config always).
 """

@@ -1746,15 +1746,15 @@ int tpl_a2b_tune(int dtype, const void* angles, const int32_t* lengths,
         if (ok[c]) sync_max = std::max(sync_max, tpl_a2b_sync_bytes(dtype, batch, angles_stride, candidates + c));
     }
     // Steps are timed back to back, kStepsPerRun per event pair, as a caller's
-    // loop runs them; the candidates take turns, round by round, so that a
+    // loop runs them; the candidates take turns, pass by pass, so that a
     // drift of the clock or of the machine's load falls on all of them alike.
-    // Round 0 is the warm-up.  Every call leaves the look-back words zero;
-    // clearing them before each run keeps one failed candidate from
+    // The first pass is the warm-up.  Every call leaves the look-back words
+    // zero; clearing them before each run keeps one failed candidate from
     // affecting the next.
     constexpr int kStepsPerRun = 6;
     std::vector<std::vector<float>> times(n_candidates);
     rc = TPL_A2B_OK;
-    for (int round = 0; round <= reps && rc == TPL_A2B_OK; ++round) {
+    for (int pass = 0; pass <= reps && rc == TPL_A2B_OK; ++pass) {
         for (int c = 0; c < n_candidates && rc == TPL_A2B_OK; ++c) {
             if (!ok[c]) continue;
             const tpl_a2b_config* cf = candidates + c;
@@ -1773,7 +1773,7 @@ int tpl_a2b_tune(int dtype, const void* angles, const int32_t* lengths,
             if (hipEventSynchronize(e1) != hipSuccess) { rc = TPL_A2B_ERR_HIP; break; }
             float t = 0.f;
             if (hipEventElapsedTime(&t, e0, e1) != hipSuccess) { rc = TPL_A2B_ERR_HIP; break; }
-            if (round > 0) times[c].push_back(t / kStepsPerRun);
+            if (pass > 0) times[c].push_back(t / kStepsPerRun);
         }
     }
     (void)hipEventDestroy(e0);

@@ -273,3 +273,50 @@ class Angles2Backbone(Module):
         if self.form == "plain":
             return Angles2BackbonePlainGPUFunction.apply(input, angles_length)
         return Angles2BackboneGPUFunction.apply(input, angles_length)
+
+
+class Angles2BackboneStep:
+    """Angles2Backbone forward and backward for fixed shapes, captured once
+    into one HIP graph and replayed.
+
+    For loops that run the same shapes many times (fitting, serving): one
+    ``replay()`` runs the forward (``angles`` -> ``coords``) and the backward
+    (``grad_coords`` -> ``grad_angles``) on the current stream, with nothing
+    allocated, no host synchronisation and one launch per step.  The graph
+    reads ``angles``, ``angles_length`` and ``grad_coords`` from their memory
+    at every replay, so a loop updates them in place between replays; it
+    writes ``coords`` and ``grad_angles``.  The chunk sizes are the tuned ones
+    for these inputs, or ``config`` (forward, backward residues per lane).
+
+    Replays do not check for NaN or out-of-range lengths (the checks of the
+    layer read a status word on the host); lengths must stay within
+    ``[0, angles.size(2)]``.  There is no counterpart in the reference, whose
+    layer synchronises the host on every call (Angles2Backbone.py:27,49).
+    """
+
+    def __init__(self, angles, angles_length, config=None):
+        _check_rank(angles)
+        if not angles.is_cuda or not angles_length.is_cuda:
+            raise RuntimeError("Angles2BackboneStep: angles and angles_length must be GPU tensors")
+        if not angles.is_contiguous() or not angles_length.is_contiguous():
+            raise RuntimeError("Angles2BackboneStep: angles and angles_length must be contiguous")
+        if angles.dtype not in (torch.float32, torch.float64) or angles_length.dtype != torch.int32:
+            raise RuntimeError("Angles2BackboneStep: angles must be float32 or float64, angles_length int32")
+        B, _, L = angles.shape
+        cfg = tuple(config) if config is not None else _tuning.config_for(angles, angles_length)
+        dev, dt = angles.device, angles.dtype
+        self.angles = angles
+        self.angles_length = angles_length
+        self.config = cfg
+        self.coords = torch.zeros(B, 9 * L, dtype=dt, device=dev)
+        self.grad_coords = torch.zeros(B, 9 * L, dtype=dt, device=dev)
+        self.grad_angles = torch.zeros(B, 3, L, dtype=dt, device=dev)
+        st_bytes, sy_bytes = ext.workspace_bytes(dt == torch.float64, B, L, list(cfg))
+        self._state = _empty_bytes(max(int(st_bytes), 256), dev)
+        self._sync = torch.zeros(max(int(sy_bytes), 256), dtype=torch.uint8, device=dev)
+        self._graph = ext.StepGraph(angles, angles_length, self.grad_coords, self.coords,
+                                    self.grad_angles, self._state, self._sync, list(cfg))
+
+    def replay(self):
+        """Runs one forward+backward step on the current stream."""
+        self._graph.replay()

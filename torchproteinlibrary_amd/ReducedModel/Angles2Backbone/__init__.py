@@ -4,4 +4,5 @@ from .Angles2Backbone import (  # noqa: F401
     Angles2BackboneCPUFunction,
     Angles2BackboneGPUFunction,
     Angles2BackbonePlainGPUFunction,
+    Angles2BackboneStep,
 )

@@ -438,9 +438,76 @@ std::tuple<double, double, int64_t> bench_graph(std::vector<std::vector<torch::T
     return {(double)ms, std::chrono::duration<double, std::milli>(t1 - t0).count(), reps * per};
 }
 
+// One forward+backward step over fixed buffers, captured once into a HIP
+// graph (on a private stream, thread-local capture mode) and replayed on the
+// current stream.  Each replay reads angles, lengths and grad_output from
+// their memory and writes output_coords and grad_input; the look-back words
+// (sync) must be zero before the first replay, and every replay leaves them
+// zero.  The tensors are held for the graph's lifetime.
+class StepGraph {
+  public:
+    StepGraph(torch::Tensor angles, torch::Tensor lengths, torch::Tensor grad_output,
+              torch::Tensor output_coords, torch::Tensor grad_input, torch::Tensor state,
+              torch::Tensor sync, std::vector<int64_t> config)
+        : held_{angles, lengths, grad_output, output_coords, grad_input, state, sync} {
+        const tpl_a2b_config cfg = to_config(config);
+        auto [B, L] = check_inputs(angles, lengths);
+        const int dt = dtype_code(angles);
+        c10::DeviceGuard guard(angles.device());
+        device_ = angles.device().index();
+        check_out(grad_output, angles, {B, 9 * (int64_t)L}, "gradOutput");
+        check_out(output_coords, angles, {B, 9 * (int64_t)L}, "output_coords");
+        check_out(grad_input, angles, {B, 3, L}, "gradInput");
+        void* st = opt_ptr(state, tpl_a2b_state_bytes(dt, B, L, &cfg), "state", angles);
+        void* sy = opt_ptr(sync, tpl_a2b_sync_bytes(dt, B, L, &cfg), "sync", angles);
+        hipStream_t cs;
+        TORCH_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) == hipSuccess,
+                    "StepGraph: hipStreamCreateWithFlags failed");
+        std::string err;
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            err = "hipStreamBeginCapture failed";
+        } else {
+            int rc = tpl_a2b_forward(dt, angles.data_ptr(), output_coords.data_ptr(),
+                                     lengths.data_ptr<int32_t>(), st, sy, nullptr, B, L, &cfg, cs);
+            if (rc == TPL_A2B_OK)
+                rc = tpl_a2b_backward(dt, grad_input.data_ptr(), grad_output.data_ptr(), angles.data_ptr(),
+                                      lengths.data_ptr<int32_t>(), st, sy, nullptr, B, L, &cfg, cs);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(cs, &g);
+            if (rc != TPL_A2B_OK || ec != hipSuccess || !g)
+                err = rc != TPL_A2B_OK ? tpl_a2b_strerror(rc) : "hipStreamEndCapture failed";
+            else if (hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0) != hipSuccess)
+                err = "hipGraphInstantiate failed";
+            if (g) (void)hipGraphDestroy(g);
+        }
+        (void)hipStreamDestroy(cs);
+        TORCH_CHECK(err.empty(), "StepGraph: ", err);
+    }
+    StepGraph(const StepGraph&) = delete;
+    StepGraph& operator=(const StepGraph&) = delete;
+    ~StepGraph() {
+        if (exec_) (void)hipGraphExecDestroy(exec_);
+    }
+    void replay() {
+        c10::DeviceGuard guard(held_[0].device());
+        hipStream_t s = c10::hip::getCurrentHIPStream(device_).stream();
+        TORCH_CHECK(hipGraphLaunch(exec_, s) == hipSuccess, "StepGraph: hipGraphLaunch failed");
+    }
+
+  private:
+    std::vector<torch::Tensor> held_;
+    hipGraphExec_t exec_ = nullptr;
+    int device_ = 0;
+};
+
 }  // namespace
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
+    pybind11::class_<StepGraph>(m, "StepGraph")
+        .def(pybind11::init<torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor,
+                            torch::Tensor, torch::Tensor, std::vector<int64_t>>(),
+             "capture one fwd+bwd step into a HIP graph")
+        .def("replay", &StepGraph::replay, "launch the step on the current stream");
     m.doc() = "Angles2Backbone for MI355X: tensor glue over the tpl_a2b C ABI";
     m.def("Angles2BackboneGPU_forward", &forward, "Angles2Backbone gpu forward (chunk scan)");
     m.def("Angles2BackboneGPU_backward", &backward, "Angles2Backbone gpu backward (O(L) adjoint)");

@@ -20,6 +20,14 @@ def test_cpu_tensors_go_to_the_host_path_float64_only():
         Angles2Backbone()(torch.zeros(2, 3, 4), torch.tensor([4, 2], dtype=torch.int32))
 
 
+def test_step_graph_needs_gpu_tensors():
+    from torchproteinlibrary_amd.ReducedModel.Angles2Backbone import Angles2BackboneStep
+    with pytest.raises(RuntimeError, match="GPU tensors"):
+        Angles2BackboneStep(torch.zeros(2, 3, 4), torch.tensor([4, 2], dtype=torch.int32))
+    with pytest.raises(Exception, match="Incorrect input size"):
+        Angles2BackboneStep(torch.zeros(2, 12), torch.tensor([4, 2], dtype=torch.int32))
+
+
 def test_bad_form():
     with pytest.raises(ValueError):
         Angles2Backbone("fast")

@@ -199,6 +199,37 @@ def test_large_workgroups_match_small_and_oracle():
     assert np.abs(ga[2].cpu().numpy() - want_g).max() <= 1e-10 * max(1.0, np.abs(want_g).max())
 
 
+def test_wave_boundary_lengths_both_workgroup_sizes():
+    """Lengths at the 64-thread wave boundaries of the workgroup scans (one
+    residue per thread in 512-thread workgroups, one or two in 256-thread
+    ones), so that runs end in the last lane of a wave, in the first lane of
+    the next, and waves past the structure skip the frame scan.  The batch of
+    9 runs the forward on 512 threads; the same structures padded to 256 with
+    empty sequences run it on 256 threads.  FP64 against the oracle, both
+    directions."""
+    from torchproteinlibrary_amd.FullAtomModel import Angles2Coords
+    rng = np.random.default_rng(41)
+    lengths = [63, 64, 65, 127, 128, 129, 255, 256, 257]
+    print("a2c wave-boundary lengths", lengths, flush=True)
+    seqs, ang = _batch(rng, lengths, 257, torch.float64)
+    nb = len(lengths)
+    pad = seqs + [""] * (256 - nb)
+    ang_pad = np.concatenate([ang, np.zeros((256 - nb, 8, 257))])
+    for ss, aa in ((seqs, ang), (pad, ang_pad)):
+        X = torch.tensor(aa, device=DEV, requires_grad=True)
+        coords = Angles2Coords()(X, ss)[0]
+        g = rng.normal(size=tuple(coords.shape))
+        coords.backward(torch.tensor(g, device=DEV))
+        c = coords.detach().cpu().numpy()
+        for b in range(nb):
+            want_c, _, _ = O.forward_one(seqs[b], ang[b])
+            assert np.abs(c[b, :want_c.size] - want_c).max() < 1e-9, (len(ss), lengths[b])
+            assert not c[b, want_c.size:].any()
+            want = O.backward_fast(seqs[b], ang[b], np.asarray(g[b], np.float64))
+            rel = np.abs(X.grad[b].cpu().numpy() - want).max() / np.abs(want).max()
+            assert rel <= 1e-9, (len(ss), lengths[b], rel)
+
+
 def test_reference_signature_backward():
     """Angles2Coords_backward with the reference's arguments (angles, not
     coordinates) equals the layer's backward."""

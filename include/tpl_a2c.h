@@ -81,6 +81,25 @@ int tpl_a2c_backward(int dtype, const uint8_t *sequences, int seq_stride, const 
                      const void *grad_coords, int atoms_stride, void *grad_angles, int angles_stride,
                      uint32_t *status, int batch, tpl_a2c_stream_t stream);
 
+/* Backward kernels.  TPL_A2C_BWD_RUNS: each thread walks a run of residues
+ * (fastest when the batch is small against the GPU).  TPL_A2C_BWD_TILES:
+ * the structure is walked in tiles of 1024 consecutive atoms, one per
+ * thread pair, each atom read once with coalesced loads (fastest when the
+ * batch fills the GPU; angles_stride at most 2048, else TPL_A2C_ERR_SIZE).
+ * TPL_A2C_BWD_AUTO picks TILES for batch * angles_stride >= 100000 within
+ * that limit.  Both give the same gradients up to rounding. */
+#define TPL_A2C_BWD_AUTO 0
+#define TPL_A2C_BWD_RUNS 1
+#define TPL_A2C_BWD_TILES 2
+
+/* tpl_a2c_backward with the kernel chosen by `form`; tpl_a2c_backward is
+ * TPL_A2C_BWD_AUTO.  No counterpart in the reference: for callers that pin
+ * a kernel (tests comparing the two, timing). */
+int tpl_a2c_backward_form(int form, int dtype, const uint8_t *sequences, int seq_stride,
+                          const void *coords, const void *grad_coords, int atoms_stride,
+                          void *grad_angles, int angles_stride, uint32_t *status, int batch,
+                          tpl_a2c_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -291,3 +291,78 @@ def test_matches_reference_outputs():
         assert np.abs(X.grad[0].cpu().numpy() - wg).max() <= 1e-10 * max(1.0, np.abs(wg).max())
         k += 1
     assert k == 3
+
+
+def _backward_forms(seqs, ang, g, dtype):
+    """Gradients by each backward kernel (tpl_a2c_backward_form: 1 = runs of
+    residues per thread, 2 = atom tiles) from the layer's coordinates, into
+    NaN-filled outputs, with the status word checked."""
+    from torchproteinlibrary_amd.FullAtomModel import Angles2Coords
+    from torchproteinlibrary_amd.FullAtomModel.Angles2Coords import convertStringList
+    lib = abi_a2c.load()
+    X = torch.tensor(ang, dtype=dtype, device=DEV)
+    coords = Angles2Coords()(X, seqs)[0].contiguous()
+    S = convertStringList(seqs).to(DEV)
+    A = coords.shape[1] // 3
+    G = torch.tensor(g[:, :3 * A], dtype=dtype, device=DEV).contiguous()
+    dt = 0 if dtype == torch.float32 else 1
+    out = {}
+    for form in (1, 2):
+        ga = torch.full_like(X, float("nan"))
+        st = torch.zeros(1, dtype=torch.int32, device=DEV)
+        torch.cuda.synchronize()
+        print(f"a2c backward form {form}: B={len(seqs)} stride={ang.shape[2]} dtype={dtype}", flush=True)
+        assert lib.tpl_a2c_backward_form(form, dt, S.data_ptr(), S.shape[1], coords.data_ptr(), G.data_ptr(), A,
+                                         ga.data_ptr(), ang.shape[2], st.data_ptr(), len(seqs), None) == 0
+        torch.cuda.synchronize()
+        assert int(st.item()) == 0
+        out[form] = ga.cpu().numpy().astype(np.float64)
+        assert np.isfinite(out[form]).all()
+    return out[1], out[2]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_backward_kernels_match_oracle_small(dtype):
+    """Both backward kernels on the small mixed batch (lengths 1-80 and the
+    20-residue sequence) against the oracle, with exact zeros past each
+    structure."""
+    rng = np.random.default_rng(41)
+    lengths = [1, 2, 20, 57, 80, 33, 5, 64, 3, 41, 77, 12]
+    seqs, ang = _batch(rng, lengths, 83, dtype)
+    seqs[2] = O.AA20
+    g = rng.normal(size=(len(seqs), 3 * max(O.num_atoms(s) for s in seqs)))
+    for b, s in enumerate(seqs):
+        g[b, 3 * O.num_atoms(s):] = 0.0
+    g = g.astype(np.float32 if dtype == torch.float32 else np.float64).astype(np.float64)
+    runs, tiles = _backward_forms(seqs, ang, g, dtype)
+    want = O.backward(seqs, ang, g)
+    scale = np.abs(want).max(axis=(1, 2), keepdims=True)
+    tol = 1e-10 if dtype == torch.float64 else 1e-5
+    for got in (runs, tiles):
+        assert (np.abs(got - want) / scale).max() <= tol
+        for b, n in enumerate(lengths):
+            assert not got[b, :, n:].any()
+            assert got[b, 2, n - 1] == 0.0
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_backward_kernels_agree_at_tile_limits(dtype):
+    """Both backward kernels at the tile kernel's 2048-residue limit: all
+    tryptophan (14 atoms a residue, 28 672 atoms, 29 tiles), all glycine (4
+    atoms) and a random 1500-residue sequence padded to 2048.  The kernels
+    agree to 1e-9 of the largest gradient (both sum in FP64 from the same
+    coordinates, in different orders), and the random one matches the
+    oracle's O(atoms) backward (FP64 1e-9, FP32 1e-4 relative)."""
+    rng = np.random.default_rng(43)
+    seqs, ang = _batch(rng, [2048, 2048, 1500], 2048, dtype)
+    seqs[0], seqs[1] = "W" * 2048, "G" * 2048
+    g = rng.normal(size=(3, 3 * 2048 * 14))
+    for b, s in enumerate(seqs):
+        g[b, 3 * O.num_atoms(s):] = 0.0
+    g = g.astype(np.float32 if dtype == torch.float32 else np.float64).astype(np.float64)
+    runs, tiles = _backward_forms(seqs, ang, g, dtype)
+    for b in range(3):
+        assert np.abs(runs[b] - tiles[b]).max() <= 1e-9 * np.abs(runs[b]).max(), b
+    want = O.backward_fast(seqs[2], ang[2], g[2])
+    rel = np.abs(tiles[2] - want).max() / np.abs(want).max()
+    assert rel <= (1e-9 if dtype == torch.float64 else 1e-4), rel

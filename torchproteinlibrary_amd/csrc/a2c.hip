@@ -23,7 +23,9 @@
 // The subtree of a backbone node is the rest of its residue plus every later
 // residue: a workgroup scan of per-run (sum g, sum x cross g), with runs in
 // reverse thread order so that it is a prefix scan, supplies the downstream
-// part, and each thread walks its residues once more.
+// part, and each thread walks its residues once more.  For batches that fill the
+// GPU a second backward walks each structure in tiles of consecutive atoms
+// instead (a2c_bwd_tiled), with coalesced loads and each atom read once.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -185,6 +187,20 @@ __device__ __forceinline__ double wave_scan_double(double v) {
     v += dpp_double<kRowBcast15, 0xa>(v, 0.0);
     v += dpp_double<kRowBcast31, 0xc>(v, 0.0);
     return v;
+}
+// Inclusive prefix sums within each row of 16 lanes.
+__device__ __forceinline__ double row_scan_double(double v) {
+    v += dpp_double<kRowShr1, 0xf>(v, 0.0);
+    v += dpp_double<kRowShr2, 0xf>(v, 0.0);
+    v += dpp_double<kRowShr4, 0xf>(v, 0.0);
+    v += dpp_double<kRowShr8, 0xf>(v, 0.0);
+    return v;
+}
+// The value of lane `lane` (wave-uniform) of a double.
+__device__ __forceinline__ double readlane_double(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ int wave_scan_int(int v) {
     v += dpp_int<kRowShr1, 0xf>(v);
@@ -483,9 +499,12 @@ __device__ __forceinline__ void sums_add(Sums& s, const double x[3], const doubl
 
 // Per structure, without frames: length, validity, this thread's run and its
 // first atom (workgroup scan of the atom counts).  Flags as in prepare().
+// With s_ra, also each residue's first atom | type << 24 (visible to the
+// workgroup after its next barrier).
 template <int NT>
 __device__ void count_run(const uint8_t* seq, int seq_stride, int ang_stride, int atoms_stride,
-                          uint32_t* status, const int* res_i, int* s_c, int* s_flag, Run& run) {
+                          uint32_t* status, const int* res_i, int* s_c, int* s_flag, Run& run,
+                          int* s_ra = nullptr) {
     const int tid = threadIdx.x;
     if (tid == 0) {
         s_flag[0] = seq_stride;
@@ -508,10 +527,12 @@ __device__ void count_run(const uint8_t* seq, int seq_stride, int ang_stride, in
     bool bad = false;
     for (int i = r0; i < r1; ++i) {
         const int t = residue_type(seq[i]);
-        if (t < 0)
+        if (t < 0) {
             bad = true;
-        else
+        } else {
+            if (s_ra) s_ra[i] = cnt | (t << 24);
             cnt += res_i[t] & 0xff;
+        }
     }
     if (bad) atomicOr(&s_flag[1], (int)TPL_A2C_STATUS_SEQ);
     // atoms of this run and the later ones: prefix over threads, by DPP
@@ -529,6 +550,8 @@ __device__ void count_run(const uint8_t* seq, int seq_stride, int ang_stride, in
     }
     run.total = total;
     run.atom0 = total - (before + cinc);  // atoms of the runs before this one
+    if (s_ra)
+        for (int i = r0; i < r1; ++i) s_ra[i] += run.atom0;
     if (run.total > atoms_stride && tid == 0) atomicOr(&s_flag[1], (int)TPL_A2C_STATUS_ATOMS);
     __syncthreads();
     const int flags = s_flag[1];
@@ -704,6 +727,310 @@ __global__ void __launch_bounds__(NT) a2c_bwd(const uint8_t* __restrict__ seqs, 
         if (k % ang_stride >= run.L) ga[k] = (T)0;
 }
 
+// ---- backward over atom tiles ----
+// The run-per-thread backward above makes every load of a wave touch 64
+// cache lines (the lanes' runs lie far apart), and it reads each atom twice.
+// This one walks a structure from its end in tiles of up to kTileAtoms
+// consecutive atoms, cut at residue starts: thread t takes the tile's atoms
+// hi-1-Kt ... hi-K-Kt, so a wave's loads cover consecutive atoms, and each
+// atom is read once.  Per tile: a workgroup prefix scan over threads (in
+// descending atom order, so a suffix over atoms) of (sum g, sum x cross g)
+// plus the carry from the later tiles gives the suffix sum SUF[k] at every
+// atom; every node's subtree is a suffix (backbone nodes: from the node's
+// origin on) or a suffix minus the suffix at the residue's C (side-chain
+// nodes: side groups are contiguous and end right before C), so the thread
+// holding a node's origin atom writes that node's gradient.  The next tile's
+// atoms are loaded while the current one is processed.
+constexpr int kTileThreads = 512;
+constexpr int kTileK = 2;                                   // atoms per thread and tile
+constexpr int kTileAtoms = kTileThreads * kTileK;           // 1024
+constexpr int kTileRes = kTileAtoms / 4;                    // residues per tile at most (4 atoms or more each)
+constexpr int kTileStride = kTileAtoms - (kMaxAtoms - 1);   // tile p starts at the first residue starting at or after p * this
+constexpr int kTileMaxRes = 2048;                           // residues per structure (the residue starts live in LDS)
+constexpr int kTileMaxTiles = kTileMaxRes * kMaxAtoms / kTileStride + 1;
+
+// 1 / sqrt(x) for x > 0: the hardware estimate and two Newton steps
+// (each squares the relative error).
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = y * (1.5 - h * y * y);
+    y = y * (1.5 - h * y * y);
+    return y;
+}
+
+// u . (xg - p cross g) with u the unit vector from q to p (bond_torque with
+// the reciprocal square root above).
+__device__ __forceinline__ double bond_torque_rsq(const double p[3], const double q[3], const double s[6]) {
+    const double d0 = p[0] - q[0], d1 = p[1] - q[1], d2 = p[2] - q[2];
+    const double v0 = s[3] - (p[1] * s[2] - p[2] * s[1]);
+    const double v1 = s[4] - (p[2] * s[0] - p[0] * s[2]);
+    const double v2 = s[5] - (p[0] * s[1] - p[1] * s[0]);
+    return (d0 * v0 + d1 * v1 + d2 * v2) * rsqrt_nr(d0 * d0 + d1 * d1 + d2 * d2);
+}
+
+// The tile's atoms of this thread; atoms below lo read atom lo (in bounds)
+// and are zeroed.
+template <typename T>
+__device__ __forceinline__ void load_tile_atoms(const T* xs, const T* gs, int lo, int hi, T (&x)[kTileK][3],
+                                                T (&g)[kTileK][3]) {
+#pragma unroll
+    for (int m = 0; m < kTileK; ++m) {
+        const int k = hi - 1 - (kTileK * (int)threadIdx.x + m);
+        const size_t kc = 3 * (size_t)(k >= lo ? k : lo);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            x[m][c] = xs[kc + c];
+            g[m][c] = gs[kc + c];
+        }
+        if (k < lo) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x[m][c] = g[m][c] = (T)0;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) a2c_bwd_tiled(const uint8_t* __restrict__ seqs, int seq_stride,
+                                                              const T* __restrict__ coords,
+                                                              const T* __restrict__ grad_coords, int atoms_stride,
+                                                              T* __restrict__ grad_angles, int ang_stride,
+                                                              uint32_t* status) {
+    constexpr int NT = kTileThreads, NW = NT / 64, K = kTileK;
+    __shared__ int s_res[kTypes];                   // natoms | nca << 8 | nside << 16 | bound rows << 24
+    __shared__ int s_side[kTypes * kMaxSide];       // row & 0xff | first << 8 | count << 16
+    __shared__ uint32_t s_slot[kTypes * kMaxAtoms]; // per (type, slot): the role bits (tile_roles)
+    __shared__ int s_ra[kTileMaxRes];               // residue's first atom | type << 24
+    __shared__ int s_tile[kTileMaxTiles];           // each tile's first residue
+    __shared__ uint32_t s_role[2][kTileAtoms];      // tile atoms' roles (tile_roles)
+    __shared__ T s_x[2][3 * kTileAtoms];            // tile atoms' positions
+    __shared__ double s_cs[kTileRes][6];            // suffix sums at each tile residue's C
+    __shared__ double s_v[6][NW];                   // the waves' sums
+    __shared__ int s_c[NW];
+    __shared__ int s_flag[2];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const uint8_t* seq = seqs + (size_t)b * seq_stride;
+    const T* xs = coords + (size_t)b * 3 * atoms_stride;
+    const T* gs = grad_coords + (size_t)b * 3 * atoms_stride;
+    T* ga = grad_angles + (size_t)b * 8 * ang_stride;
+    for (int k = tid; k < kTypes * kMaxSide; k += NT) {
+        const Side sd = kSide[k / kMaxSide][k % kMaxSide];
+        s_side[k] = (sd.row & 0xff) | (sd.first << 8) | (sd.count << 16);
+    }
+    if (tid < kTypes) {
+        const Residue r = kRes[tid];
+        s_res[tid] = r.natoms | (r.nca << 8) | (r.nside << 16);
+    }
+    __syncthreads();
+    // Per (type t, atom slot s), the role bits of tile_roles but the
+    // residue's index: whether the atom is the origin of a node with an
+    // angle, the angle's row, whether that angle belongs to the previous
+    // residue (omega, on N, from the previous C two atoms back), how many
+    // atoms back the parent node's origin lies, whether the node is a side
+    // one, whether the atom is C.  A side node hangs on the node before it,
+    // or on CA when that one is the fixed first node without atoms.  And per
+    // type, the side rows (3..7) it writes: nodes with atoms and a bound
+    // angle.
+    for (int k = tid; k < kTypes * kMaxAtoms; k += NT) {
+        const int t = k / kMaxAtoms, s = k % kMaxAtoms;
+        const int natoms = s_res[t] & 0xff, nside = (s_res[t] >> 16) & 0xff;
+        uint32_t role = s == natoms - 2 ? 1u << 18 : 0u;
+        if (s == 0)
+            role |= (1u << 8) | (2u << 9) | (1u << 12) | (2u << 13);  // omega: N on the previous C
+        else if (s == 1)
+            role |= (1u << 8) | (0u << 9) | (1u << 13);  // phi: CA on N
+        else if (s == natoms - 2)
+            role |= (1u << 8) | (1u << 9) | ((uint32_t)(s - 1) << 13);  // psi: C on CA
+        else if (s < natoms)
+            for (int n = 0; n < nside; ++n) {
+                const int si = s_side[t * kMaxSide + n];
+                const int row = (int)(int8_t)(si & 0xff), first = (si >> 8) & 0xff, count = si >> 16;
+                if (count > 0 && row >= 0 && first == s) {
+                    const int sp = n > 0 ? s_side[t * kMaxSide + n - 1] : 0;
+                    const int parent = (sp >> 16) > 0 ? (sp >> 8) & 0xff : 1;
+                    role |= (1u << 8) | ((uint32_t)row << 9) | ((uint32_t)(s - parent) << 13) | (1u << 17);
+                }
+            }
+        s_slot[k] = role;
+    }
+    int bound = 0;
+    if (tid < kTypes) {
+        for (int n = 0; n < ((s_res[tid] >> 16) & 0xff); ++n) {
+            const int si = s_side[tid * kMaxSide + n];
+            if ((si >> 16) > 0 && (int)(int8_t)(si & 0xff) >= 0) bound |= 1 << (si & 0xff);
+        }
+    }
+    __syncthreads();
+    if (tid < kTypes) s_res[tid] |= bound << 24;
+    Run run;
+    count_run<NT>(seq, seq_stride, ang_stride, atoms_stride, status, s_res, s_c, s_flag, run, s_ra);
+    __syncthreads();
+    const int L = run.L;
+    for (int i = tid; i < L; i += NT) {
+        const int p = (s_ra[i] & 0xffffff) / kTileStride;
+        if (i == 0 || p != (s_ra[i - 1] & 0xffffff) / kTileStride) s_tile[p] = i;
+    }
+    __syncthreads();
+    const int ntiles = L > 0 ? (s_ra[L - 1] & 0xffffff) / kTileStride + 1 : 0;
+    const size_t stride = (size_t)ang_stride;
+    // Roles of tile p's atoms into s_role[p & 1], one residue per thread:
+    // bits 0-7 the residue's index in the tile, bit 8 set if the atom is the
+    // origin of a node with an angle, bits 9-11 that angle's row, bit 12 if
+    // it belongs to the previous residue (omega, on N), bits 13-16 how many
+    // atoms back the parent node's origin lies, bit 17 for side-chain nodes
+    // (the suffix at C is subtracted), bit 18 if the atom is the residue's
+    // C.  The thread also writes the residue's angles that no node carries:
+    // the side rows its type does not bind, and omega of the last residue.
+    auto tile_roles = [&](int p) {
+        const int ilo = s_tile[p], ihi = p + 1 < ntiles ? s_tile[p + 1] : L;
+        const int lo = s_ra[ilo] & 0xffffff;
+        if (tid < ihi - ilo) {
+            const int i = ilo + tid, ra = s_ra[i];
+            const int a = ra & 0xffffff, t = ra >> 24, rt = s_res[t], n = rt & 0xff;
+#pragma unroll
+            for (int s = 0; s < kMaxAtoms; ++s)
+                if (s < n) {
+                    uint32_t role = s_slot[t * kMaxAtoms + s] | (uint32_t)tid;
+                    if (s == 0 && i == 0) role &= ~(1u << 8);  // the first residue has no omega before it
+                    s_role[p & 1][a - lo + s] = role;
+                }
+            const int bnd = (int)((uint32_t)rt >> 24);
+            for (int r = 3; r < 8; ++r)
+                if (!((bnd >> r) & 1)) ga[r * stride + i] = (T)0;
+            if (i == L - 1) ga[2 * stride + i] = (T)0;
+        }
+    };
+    double carry[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) carry[c] = 0.0;
+    T cx[K][3], cg[K][3];
+    int lo = 0, hi = 0;
+    if (ntiles > 0) {
+        lo = s_ra[s_tile[ntiles - 1]] & 0xffffff;
+        hi = run.total;
+        load_tile_atoms(xs, gs, lo, hi, cx, cg);
+        tile_roles(ntiles - 1);
+        __syncthreads();
+    }
+    for (int p = ntiles - 1; p >= 0; --p) {
+        const int buf = p & 1;
+        const int ilo = s_tile[p];
+        // the previous residue's C (omega of residue ilo - 1 turns about the
+        // bond from it), then the next (lower) tile's atoms
+        T qprev[3] = {(T)0, (T)0, (T)0};
+        if (ilo > 0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) qprev[c] = xs[3 * (size_t)(lo - 2) + c];
+        }
+        T nx[K][3], ng[K][3];
+        const int nlo = p > 0 ? s_ra[s_tile[p - 1]] & 0xffffff : 0;
+        load_tile_atoms(xs, gs, nlo, p > 0 ? lo : nlo, nx, ng);
+        // this tile's positions, for the parent origins; the next tile's roles
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+            const int k = hi - 1 - (K * tid + m);
+            if (k >= lo) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) s_x[buf][3 * (k - lo) + c] = cx[m][c];
+            }
+        }
+        if (p > 0) tile_roles(p - 1);
+        // per atom: the node whose origin it is, its angle's slot in ga, its
+        // parent's origin in s_x (-1: qprev), whether the suffix at C is
+        // subtracted, and whether the atom is a C
+        int dst[K], qi[K], il[K];
+        bool sub[K], isc[K];
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+            const int k = hi - 1 - (K * tid + m);
+            const uint32_t role = k >= lo ? s_role[buf][k - lo] : 0u;
+            il[m] = role & 0xff;
+            dst[m] = (role >> 8) & 1 ? (int)((role >> 9) & 7) * ang_stride + ilo + il[m] - (int)((role >> 12) & 1) : -1;
+            qi[m] = k - (int)((role >> 13) & 15) - lo;
+            sub[m] = (role >> 17) & 1;
+            isc[m] = k >= lo && ((role >> 18) & 1);
+        }
+        // (g, x cross g) per atom; the thread's sum; prefix over threads
+        double w[K][6], tot[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) tot[c] = 0.0;
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+            const double x0 = (double)cx[m][0], x1 = (double)cx[m][1], x2 = (double)cx[m][2];
+            const double g0 = (double)cg[m][0], g1 = (double)cg[m][1], g2 = (double)cg[m][2];
+            w[m][0] = g0;
+            w[m][1] = g1;
+            w[m][2] = g2;
+            w[m][3] = x1 * g2 - x2 * g1;
+            w[m][4] = x2 * g0 - x0 * g2;
+            w[m][5] = x0 * g1 - x1 * g0;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) tot[c] += w[m][c];
+        }
+        double incl[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) incl[c] = wave_scan_double(tot[c]);
+        if ((tid & 63) == 63) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) s_v[c][tid >> 6] = incl[c];
+        }
+        __syncthreads();
+        // SUF at the thread's atoms: carry (later tiles) + later waves +
+        // later lanes + the thread's atoms from the highest down.  Lane v of
+        // every wave takes wave v's total and a row scan makes them
+        // inclusive over waves (NW <= 16), read back with readlane.
+        const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+        double wt[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) wt[c] = row_scan_double(lane < NW ? s_v[c][lane] : 0.0);
+        double suf[K][6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const double all = readlane_double(wt[c], NW - 1);
+            const double before = wv > 0 ? readlane_double(wt[c], wv - 1) : 0.0;
+            double acc = carry[c] + before + (incl[c] - tot[c]);
+#pragma unroll
+            for (int m = 0; m < K; ++m) {
+                acc += w[m][c];
+                suf[m][c] = acc;
+            }
+            carry[c] += all;
+        }
+#pragma unroll
+        for (int m = 0; m < K; ++m)
+            if (isc[m]) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) s_cs[il[m]][c] = suf[m][c];
+            }
+        __syncthreads();
+        // gradients of the nodes whose origins this thread holds
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+            if (dst[m] < 0) continue;
+            const double p3[3] = {(double)cx[m][0], (double)cx[m][1], (double)cx[m][2]};
+            double q[3], sm[6];
+            const int qk = qi[m] >= 0 ? qi[m] : 0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) q[c] = (double)(qi[m] >= 0 ? s_x[buf][3 * qk + c] : qprev[c]);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) sm[c] = sub[m] ? suf[m][c] - s_cs[il[m]][c] : suf[m][c];
+            ga[dst[m]] = (T)bond_torque_rsq(p3, q, sm);
+        }
+        hi = lo;
+        lo = nlo;
+#pragma unroll
+        for (int m = 0; m < K; ++m)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                cx[m][c] = nx[m][c];
+                cg[m][c] = ng[m][c];
+            }
+    }
+    // angles past the sequence (and every angle of a flagged structure)
+    for (int k = tid; k < 8 * ang_stride; k += NT)
+        if (k % ang_stride >= L) ga[k] = (T)0;
+}
+
 // 512-thread workgroups when some structure may be longer than 256 residues:
 // always for the backward (mixed lengths 30-1000 at B=1024: 155 -> 129 us),
 // and for the forward only when the structures are at most as many as the
@@ -715,6 +1042,18 @@ bool use_large_fwd(int batch, int ang_stride) {
 
 bool use_large_bwd(int ang_stride) {
     return ang_stride > kThreadsSmall;
+}
+
+// The atom-tile backward where the batch fills the GPU: it moves the atoms
+// with coalesced loads and reads each once, but walks a structure's tiles
+// one after another, so a lone long structure takes longer than with
+// per-thread runs.  Measured (FP32, backward us, runs -> tiles): mixed
+// lengths 30-1000 at B=1024 108 -> 87, 256 x 1000 88 -> 49, 128 x 1000
+// 52 -> 48, 512 x 300 34 -> 25, 1024 x 270 56 -> 47; 64 x 1000 44 -> 47,
+// 128 x 300 17.5 -> 18.9, 32 x 350 17.8 -> 18.9, 1024 x 30 13.7 -> 21.4,
+// 1 x 1000 36 -> 45.
+bool use_tiles(int batch, int ang_stride) {
+    return ang_stride <= kTileMaxRes && (int64_t)batch * ang_stride >= 100000;
 }
 
 int check(int dtype, int B, int seq_stride, int ang_stride, int atoms_stride) {
@@ -786,15 +1125,31 @@ int tpl_a2c_forward(int dtype, const uint8_t* sequences, int seq_stride, const v
     return hipGetLastError() == hipSuccess ? TPL_A2C_OK : TPL_A2C_ERR_LAUNCH;
 }
 
-int tpl_a2c_backward(int dtype, const uint8_t* sequences, int seq_stride, const void* coords,
-                     const void* grad_coords, int atoms_stride, void* grad_angles, int angles_stride,
-                     uint32_t* status, int batch, tpl_a2c_stream_t stream) {
+int tpl_a2c_backward_form(int form, int dtype, const uint8_t* sequences, int seq_stride, const void* coords,
+                          const void* grad_coords, int atoms_stride, void* grad_angles, int angles_stride,
+                          uint32_t* status, int batch, tpl_a2c_stream_t stream) {
+    if (form != TPL_A2C_BWD_AUTO && form != TPL_A2C_BWD_RUNS && form != TPL_A2C_BWD_TILES) return TPL_A2C_ERR_ARG;
     const int rc = check(dtype, batch, seq_stride, angles_stride, atoms_stride);
     if (rc != TPL_A2C_OK) return rc;
+    if (form == TPL_A2C_BWD_TILES && angles_stride > kTileMaxRes) return TPL_A2C_ERR_SIZE;
     if (batch == 0 || angles_stride == 0) return TPL_A2C_OK;
     if (!sequences || !grad_angles || (atoms_stride > 0 && (!coords || !grad_coords)))
         return TPL_A2C_ERR_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (form == TPL_A2C_BWD_AUTO) form = use_tiles(batch, angles_stride) ? TPL_A2C_BWD_TILES : TPL_A2C_BWD_RUNS;
+    if (form == TPL_A2C_BWD_TILES) {
+        if (dtype == TPL_A2C_F32)
+            hipLaunchKernelGGL((a2c_bwd_tiled<float>), dim3(batch), dim3(kTileThreads), 0, s, sequences,
+                               seq_stride, static_cast<const float*>(coords),
+                               static_cast<const float*>(grad_coords), atoms_stride,
+                               static_cast<float*>(grad_angles), angles_stride, status);
+        else
+            hipLaunchKernelGGL((a2c_bwd_tiled<double>), dim3(batch), dim3(kTileThreads), 0, s, sequences,
+                               seq_stride, static_cast<const double*>(coords),
+                               static_cast<const double*>(grad_coords), atoms_stride,
+                               static_cast<double*>(grad_angles), angles_stride, status);
+        return hipGetLastError() == hipSuccess ? TPL_A2C_OK : TPL_A2C_ERR_LAUNCH;
+    }
     const bool large = use_large_bwd(angles_stride);
     if (dtype == TPL_A2C_F32) {
         const float *x = static_cast<const float*>(coords), *g = static_cast<const float*>(grad_coords);
@@ -816,6 +1171,13 @@ int tpl_a2c_backward(int dtype, const uint8_t* sequences, int seq_stride, const 
                                sequences, seq_stride, x, g, atoms_stride, ga, angles_stride, status);
     }
     return hipGetLastError() == hipSuccess ? TPL_A2C_OK : TPL_A2C_ERR_LAUNCH;
+}
+
+int tpl_a2c_backward(int dtype, const uint8_t* sequences, int seq_stride, const void* coords,
+                     const void* grad_coords, int atoms_stride, void* grad_angles, int angles_stride,
+                     uint32_t* status, int batch, tpl_a2c_stream_t stream) {
+    return tpl_a2c_backward_form(TPL_A2C_BWD_AUTO, dtype, sequences, seq_stride, coords, grad_coords,
+                                 atoms_stride, grad_angles, angles_stride, status, batch, stream);
 }
 
 }  // extern "C"

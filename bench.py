@@ -545,11 +545,13 @@ def a2c_bench(lengths_mixed, steps, warmup, cpu_seconds, with_cpu):
     forward and backward launches over rotating buffer sets (names not
     written).  Random sequences (uniform over the 20 amino acids) and angles
     phi ~ N(-1.1, 0.3), psi ~ N(2.3, 0.4), omega = pi + N(0, 0.05), side-chain
-    angles ~ U(-pi, pi)."""
+    angles ~ U(-pi, pi).  The structures go to the GPU in the order the layer
+    uses (``schedule``: the most atoms first when sizes differ)."""
     import numpy as np
     import torch
     from torchproteinlibrary_amd._native_xform import ext as xext
     from torchproteinlibrary_amd.FullAtomModel.Angles2Coords import convertStringList
+    from torchproteinlibrary_amd.FullAtomModel.Angles2Coords.Angles2Coords import schedule
 
     rng = np.random.default_rng(7)
     dev = torch.device("cuda:0")
@@ -576,11 +578,14 @@ def a2c_bench(lengths_mixed, steps, warmup, cpu_seconds, with_cpu):
         natoms = [xext.getSeqNumAtoms(q) for q in seqs]
         A = max(natoms)
         S = convertStringList(seqs).to(dev)
+        order = schedule(natoms)
         sets = []
         for _ in range(nsets):
             X = torch.tensor(ang, dtype=dtype, device=dev)
             sets.append([S, X, torch.empty(len(seqs), 3 * A, dtype=dtype, device=dev),
                          torch.randn(len(seqs), 3 * A, dtype=dtype, device=dev), torch.empty_like(X)])
+            if order is not None:
+                sets[-1].append(order.to(dev))
         torch.cuda.synchronize()
         st, f, b = xext.a2c_bench(sets, steps, warmup)
         residues, atoms = int(sum(lengths)), int(sum(natoms))

@@ -55,6 +55,7 @@ enum tpl_a2c_error {
 #define TPL_A2C_STATUS_SEQ 1u    /* a sequence holds a letter outside the 20 amino acids */
 #define TPL_A2C_STATUS_LENGTH 2u /* a sequence is longer than angles_stride */
 #define TPL_A2C_STATUS_ATOMS 4u  /* a structure has more atoms than atoms_stride */
+#define TPL_A2C_STATUS_ORDER 8u  /* an order entry outside 0..batch-1 (that workgroup did nothing) */
 
 const char *tpl_a2c_strerror(int code);
 
@@ -92,11 +93,27 @@ int tpl_a2c_backward(int dtype, const uint8_t *sequences, int seq_stride, const 
 #define TPL_A2C_BWD_RUNS 1
 #define TPL_A2C_BWD_TILES 2
 
-/* tpl_a2c_backward with the kernel chosen by `form`; tpl_a2c_backward is
- * TPL_A2C_BWD_AUTO.  No counterpart in the reference: for callers that pin
- * a kernel (tests comparing the two, timing). */
-int tpl_a2c_backward_form(int form, int dtype, const uint8_t *sequences, int seq_stride,
-                          const void *coords, const void *grad_coords, int atoms_stride,
+/* Scheduling order.  One workgroup builds one structure, and workgroups
+ * start in index order; with mixed lengths the long structures that happen
+ * to come last keep the GPU busy after the rest is done.  With `order` (a
+ * device int32 array holding a permutation of 0..batch-1, or NULL for
+ * 0..batch-1) workgroup w takes structure order[w]; the outputs are the same
+ * bit for bit.  Longest first (the atom counts sorted in descending order,
+ * which the caller knows from the sequences) is the intended use.  An entry
+ * outside 0..batch-1 sets TPL_A2C_STATUS_ORDER and its workgroup does
+ * nothing; a repeated entry leaves the structures it displaces unwritten.
+ * No counterpart in the reference (its CPU loop has no such order). */
+int tpl_a2c_forward_ordered(const int32_t *order, int dtype, const uint8_t *sequences, int seq_stride,
+                            const void *angles, int angles_stride, void *coords, uint8_t *res_names,
+                            uint8_t *atom_names, int atoms_stride, uint32_t *status, int batch,
+                            tpl_a2c_stream_t stream);
+
+/* tpl_a2c_backward with the kernel chosen by `form` and the scheduling
+ * order above; tpl_a2c_backward is TPL_A2C_BWD_AUTO without an order.  No
+ * counterpart in the reference: for callers that pin a kernel or pass an
+ * order (the Python layer, tests comparing the kernels, timing). */
+int tpl_a2c_backward_form(int form, const int32_t *order, int dtype, const uint8_t *sequences,
+                          int seq_stride, const void *coords, const void *grad_coords, int atoms_stride,
                           void *grad_angles, int angles_stride, uint32_t *status, int batch,
                           tpl_a2c_stream_t stream);
 

@@ -20,5 +20,6 @@ def load():
     L.tpl_a2c_num_atoms.argtypes = [ctypes.c_char_p]
     L.tpl_a2c_forward.argtypes = [i32, vp, i32, vp, i32, vp, vp, vp, i32, vp, i32, vp]
     L.tpl_a2c_backward.argtypes = [i32, vp, i32, vp, vp, i32, vp, i32, vp, i32, vp]
-    L.tpl_a2c_backward_form.argtypes = [i32, i32, vp, i32, vp, vp, i32, vp, i32, vp, i32, vp]
+    L.tpl_a2c_backward_form.argtypes = [i32, vp, i32, vp, i32, vp, vp, i32, vp, i32, vp, i32, vp]
+    L.tpl_a2c_forward_ordered.argtypes = [vp, i32, vp, i32, vp, i32, vp, vp, vp, i32, vp, i32, vp]
     return L

@@ -312,8 +312,8 @@ def _backward_forms(seqs, ang, g, dtype):
         st = torch.zeros(1, dtype=torch.int32, device=DEV)
         torch.cuda.synchronize()
         print(f"a2c backward form {form}: B={len(seqs)} stride={ang.shape[2]} dtype={dtype}", flush=True)
-        assert lib.tpl_a2c_backward_form(form, dt, S.data_ptr(), S.shape[1], coords.data_ptr(), G.data_ptr(), A,
-                                         ga.data_ptr(), ang.shape[2], st.data_ptr(), len(seqs), None) == 0
+        assert lib.tpl_a2c_backward_form(form, None, dt, S.data_ptr(), S.shape[1], coords.data_ptr(), G.data_ptr(),
+                                         A, ga.data_ptr(), ang.shape[2], st.data_ptr(), len(seqs), None) == 0
         torch.cuda.synchronize()
         assert int(st.item()) == 0
         out[form] = ga.cpu().numpy().astype(np.float64)
@@ -366,3 +366,56 @@ def test_backward_kernels_agree_at_tile_limits(dtype):
     want = O.backward_fast(seqs[2], ang[2], g[2])
     rel = np.abs(tiles[2] - want).max() / np.abs(want).max()
     assert rel <= (1e-9 if dtype == torch.float64 else 1e-4), rel
+
+
+def test_scheduling_order_changes_nothing():
+    """Workgroups taking the structures in the layer's order (most atoms
+    first), in a reversed order or in batch order give the same outputs bit
+    for bit, through both backward kernels; an order entry outside the batch
+    sets the status bit and raises in the layer."""
+    from torchproteinlibrary_amd._native_xform import ext
+    from torchproteinlibrary_amd.FullAtomModel.Angles2Coords import convertStringList
+    from torchproteinlibrary_amd.FullAtomModel.Angles2Coords.Angles2Coords import schedule
+    lib = abi_a2c.load()
+    rng = np.random.default_rng(47)
+    lengths = [int(v) for v in rng.integers(1, 300, size=40)]
+    seqs, ang = _batch(rng, lengths, 300, torch.float32)
+    natoms = [O.num_atoms(s) for s in seqs]
+    A = max(natoms)
+    S = convertStringList(seqs).to(DEV)
+    X = torch.tensor(ang, dtype=torch.float32, device=DEV)
+    G = torch.randn(len(seqs), 3 * A, device=DEV)
+    first = schedule(natoms)
+    assert first is not None and sorted(first.tolist()) == list(range(len(seqs)))
+    assert all(natoms[first[k]] >= natoms[first[k + 1]] for k in range(len(seqs) - 1))
+    orders = [None, first.to(DEV), torch.flip(first, [0]).to(DEV)]
+    outs = []
+    for order in orders:
+        c = torch.full((len(seqs), 3 * A), float("nan"), device=DEV)
+        rn = torch.full((len(seqs), A, 4), 255, dtype=torch.uint8, device=DEV)
+        an = torch.full_like(rn, 255)
+        st = torch.zeros(1, dtype=torch.int32, device=DEV)
+        ext.Angles2Coords_forward(S, X, c, rn, an, st, order)
+        grads = []
+        for form in (1, 2):
+            ga = torch.full_like(X, float("nan"))
+            optr = None if order is None else order.data_ptr()
+            torch.cuda.synchronize()
+            assert lib.tpl_a2c_backward_form(form, optr, 0, S.data_ptr(), S.shape[1], c.data_ptr(), G.data_ptr(), A,
+                                             ga.data_ptr(), 300, st.data_ptr(), len(seqs), None) == 0
+            grads.append(ga)
+        torch.cuda.synchronize()
+        assert int(st.item()) == 0
+        outs.append((c, rn, an, *grads))
+    for other in outs[1:]:
+        for u, v in zip(outs[0], other):
+            assert torch.equal(u, v)
+    assert torch.isfinite(outs[0][0]).all() and torch.isfinite(outs[0][3]).all()
+    bad = first.clone()
+    bad[3] = len(seqs)
+    st = torch.zeros(1, dtype=torch.int32, device=DEV)
+    c = torch.empty(len(seqs), 3 * A, device=DEV)
+    ext.Angles2Coords_forward(S, X, c, torch.empty(len(seqs), A, 4, dtype=torch.uint8, device=DEV),
+                              torch.empty(len(seqs), A, 4, dtype=torch.uint8, device=DEV), st, bad.to(DEV))
+    torch.cuda.synchronize()
+    assert int(st.item()) & 8

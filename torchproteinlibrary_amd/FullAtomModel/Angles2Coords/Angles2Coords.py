@@ -41,7 +41,7 @@ from ..._native_xform import ext
 
 __all__ = ["Angles2Coords", "Angles2CoordsFunction", "convertStringList", "convertString"]
 
-_STATUS_SEQ, _STATUS_LENGTH, _STATUS_ATOMS = 1, 2, 4
+_STATUS_SEQ, _STATUS_LENGTH, _STATUS_ATOMS, _STATUS_ORDER = 1, 2, 4, 8
 _LETTERS = set("ACDEFGHIKLMNPQRSTVWY")
 
 
@@ -61,6 +61,20 @@ def convertString(string):
     return torch.from_numpy(np.frombuffer(bytes(string + "\0", "ascii"), dtype=np.uint8).copy())
 
 
+def schedule(num_atoms):
+    """The order in which the GPU takes the structures: the most atoms first
+    (ties in batch order), as an int32 CPU tensor, or None when every
+    structure has the same size (batch order is then as good).  One
+    workgroup builds one structure and workgroups start in index order, so
+    with mixed sizes this keeps the long structures from starting last:
+    mixed lengths 30-1000 at B=1024 measured 59 -> 50 us (forward) and
+    112 -> 77 us (backward)."""
+    n = np.asarray(num_atoms, dtype=np.int64)
+    if n.size < 2 or (n == n[0]).all():
+        return None
+    return torch.from_numpy(np.argsort(-n, kind="stable").astype(np.int32))
+
+
 def _checks_enabled() -> bool:
     return os.environ.get("TPL_A2B_CHECKS", "1") != "0" and not torch.cuda.is_current_stream_capturing()
 
@@ -74,6 +88,8 @@ def _check(out, status, where):
             raise ValueError("Angles2CoordsFunction: a sequence is longer than input_angles.size(2)")
         if s & _STATUS_ATOMS:
             raise ValueError("Angles2CoordsFunction: a structure has more atoms than the output holds")
+        if s & _STATUS_ORDER:
+            raise RuntimeError("Angles2CoordsFunction: the scheduling order holds an index outside the batch")
         if math.isnan(out.sum()):
             raise Exception(f"Angles2CoordsFunction: {where} Nan")
 
@@ -81,10 +97,13 @@ def _check(out, status, where):
 class Angles2CoordsFunction(Function):
     """Protein angles -> coordinates (reference Angles2Coords.py:34-71).
     ``sequenceTensor`` is the uint8 [B, S] tensor of ``convertStringList`` on
-    the angles' device."""
+    the angles' device.  ``order`` (optional, int32 [B]) is the order in
+    which the GPU takes the structures (a permutation of 0..B-1; the outputs
+    do not depend on it): ``Angles2Coords`` passes the structures from the
+    most atoms to the fewest when their sizes differ."""
 
     @staticmethod
-    def forward(ctx, input_angles, sequenceTensor, num_atoms):
+    def forward(ctx, input_angles, sequenceTensor, num_atoms, order=None):
         if input_angles.dim() != 3 or input_angles.size(1) != 8:
             raise ValueError("Angles2CoordsFunction: input_angles must be [batch, 8, length], got "
                              f"{tuple(input_angles.size())}")
@@ -101,9 +120,12 @@ class Angles2CoordsFunction(Function):
         res_names = torch.empty(B, max_atoms, 4, dtype=torch.uint8, device=dev)
         atom_names = torch.empty(B, max_atoms, 4, dtype=torch.uint8, device=dev)
         status = torch.zeros(1, dtype=torch.int32, device=dev) if _checks_enabled() else None
-        ext.Angles2Coords_forward(seqs, input_angles, coords, res_names, atom_names, status)
+        if order is not None:
+            order = order.to(dev).contiguous()
+        ext.Angles2Coords_forward(seqs, input_angles, coords, res_names, atom_names, status, order)
         _check(coords, status, "forward")
         ctx.save_for_backward(seqs, coords)
+        ctx.order = order
         ctx.mark_non_differentiable(res_names, atom_names, num_atoms)
         return coords, res_names, atom_names, num_atoms
 
@@ -114,9 +136,9 @@ class Angles2CoordsFunction(Function):
         grad_atoms = grad_atoms.contiguous()
         grad_angles = torch.empty(ctx.angles_shape, dtype=coords.dtype, device=coords.device)
         status = torch.zeros(1, dtype=torch.int32, device=coords.device) if _checks_enabled() else None
-        ext.Angles2Coords_backward_coords(grad_atoms, grad_angles, seqs, coords, status)
+        ext.Angles2Coords_backward_coords(grad_atoms, grad_angles, seqs, coords, status, ctx.order)
         _check(grad_angles, status, "backward")
-        return grad_angles, None, None
+        return grad_angles, None, None, None
 
 
 class Angles2Coords(Module):
@@ -135,4 +157,4 @@ class Angles2Coords(Module):
         self.num_atoms = [ext.getSeqNumAtoms(s) if s else 0 for s in sequences]
         num_atoms = torch.tensor(self.num_atoms, dtype=torch.int32, device=input_angles.device)
         stringListTensor = convertStringList(sequences)
-        return Angles2CoordsFunction.apply(input_angles, stringListTensor, num_atoms)
+        return Angles2CoordsFunction.apply(input_angles, stringListTensor, num_atoms, schedule(self.num_atoms))

@@ -281,6 +281,18 @@ __device__ __forceinline__ void residue_step(double m[12], const Tables& tb, con
     step(m, tb.bb[2][0], tb.bb[2][1], c, s, geo::R_C_N);
 }
 
+// The structure this workgroup builds: order[blockIdx.x], or blockIdx.x
+// without an order.  An index outside the batch is reported
+// (TPL_A2C_STATUS_ORDER) and the workgroup does nothing (-1).
+__device__ __forceinline__ int structure_of(const int* order, uint32_t* status) {
+    if (!order) return (int)blockIdx.x;
+    const int b = __builtin_amdgcn_readfirstlane(order[blockIdx.x]);
+    if (b >= 0 && b < (int)gridDim.x) return b;
+    if (threadIdx.x == 0 && status)
+        __hip_atomic_fetch_or(status, TPL_A2C_STATUS_ORDER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return -1;
+}
+
 // Per structure: the sequence length (first 0 byte), the residues' validity,
 // this thread's run [r0, r1), and the workgroup scans giving the N frame of
 // residue r0 (frame) and its first atom (atom0).  Returns false (for every
@@ -390,12 +402,13 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 2
                                                     const T* __restrict__ angles, int ang_stride,
                                                     T* __restrict__ coords, uint32_t* __restrict__ res_names,
                                                     uint32_t* __restrict__ atom_names, int atoms_stride,
-                                                    uint32_t* status) {
+                                                    uint32_t* status, const int* __restrict__ order) {
     __shared__ Tables tb;
     __shared__ double s_m[12][NT / 64];  // the waves' products
     __shared__ int s_c[NT / 64];          // the waves' atom counts
     __shared__ int s_flag[2];
-    const int b = blockIdx.x;
+    const int b = structure_of(order, status);
+    if (b < 0) return;
     const uint8_t* seq = seqs + (size_t)b * seq_stride;
     const T* ang = angles + (size_t)b * 8 * ang_stride;
     T* out = coords + (size_t)b * 3 * atoms_stride;
@@ -597,13 +610,14 @@ template <int NT, typename T>
 __global__ void __launch_bounds__(NT) a2c_bwd(const uint8_t* __restrict__ seqs, int seq_stride,
                                               const T* __restrict__ coords, const T* __restrict__ grad_coords,
                                               int atoms_stride, T* __restrict__ grad_angles, int ang_stride,
-                                              uint32_t* status) {
+                                              uint32_t* status, const int* __restrict__ order) {
     __shared__ int s_res[kTypes];
     __shared__ int s_side[kTypes * kMaxSide];
     __shared__ double s_v[6][NT / 64];  // the waves' sums
     __shared__ int s_c[NT / 64];        // the waves' atom counts
     __shared__ int s_flag[2];
-    const int b = blockIdx.x, tid = threadIdx.x;
+    const int b = structure_of(order, status), tid = threadIdx.x;
+    if (b < 0) return;
     const uint8_t* seq = seqs + (size_t)b * seq_stride;
     const T* xs = coords + (size_t)b * 3 * atoms_stride;
     const T* gs = grad_coords + (size_t)b * 3 * atoms_stride;
@@ -795,7 +809,7 @@ __global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_
                                                               const T* __restrict__ coords,
                                                               const T* __restrict__ grad_coords, int atoms_stride,
                                                               T* __restrict__ grad_angles, int ang_stride,
-                                                              uint32_t* status) {
+                                                              uint32_t* status, const int* __restrict__ order) {
     constexpr int NT = kTileThreads, NW = NT / 64, K = kTileK;
     __shared__ int s_res[kTypes];                   // natoms | nca << 8 | nside << 16 | bound rows << 24
     __shared__ int s_side[kTypes * kMaxSide];       // row & 0xff | first << 8 | count << 16
@@ -808,7 +822,8 @@ __global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_
     __shared__ double s_v[6][NW];                   // the waves' sums
     __shared__ int s_c[NW];
     __shared__ int s_flag[2];
-    const int b = blockIdx.x, tid = threadIdx.x;
+    const int b = structure_of(order, status), tid = threadIdx.x;
+    if (b < 0) return;
     const uint8_t* seq = seqs + (size_t)b * seq_stride;
     const T* xs = coords + (size_t)b * 3 * atoms_stride;
     const T* gs = grad_coords + (size_t)b * 3 * atoms_stride;
@@ -1090,9 +1105,10 @@ int tpl_a2c_num_atoms(const char* sequence) {
     return n;
 }
 
-int tpl_a2c_forward(int dtype, const uint8_t* sequences, int seq_stride, const void* angles,
-                    int angles_stride, void* coords, uint8_t* res_names, uint8_t* atom_names,
-                    int atoms_stride, uint32_t* status, int batch, tpl_a2c_stream_t stream) {
+int tpl_a2c_forward_ordered(const int32_t* order, int dtype, const uint8_t* sequences, int seq_stride,
+                            const void* angles, int angles_stride, void* coords, uint8_t* res_names,
+                            uint8_t* atom_names, int atoms_stride, uint32_t* status, int batch,
+                            tpl_a2c_stream_t stream) {
     const int rc = check(dtype, batch, seq_stride, angles_stride, atoms_stride);
     if (rc != TPL_A2C_OK) return rc;
     if (batch == 0) return TPL_A2C_OK;
@@ -1108,26 +1124,33 @@ int tpl_a2c_forward(int dtype, const uint8_t* sequences, int seq_stride, const v
         float* c = static_cast<float*>(coords);
         if (large)
             hipLaunchKernelGGL((a2c_fwd<kThreadsLarge, float>), dim3(batch), dim3(kThreadsLarge), 0, s,
-                               sequences, seq_stride, a, angles_stride, c, rn, an, atoms_stride, status);
+                               sequences, seq_stride, a, angles_stride, c, rn, an, atoms_stride, status, order);
         else
             hipLaunchKernelGGL((a2c_fwd<kThreadsSmall, float>), dim3(batch), dim3(kThreadsSmall), 0, s,
-                               sequences, seq_stride, a, angles_stride, c, rn, an, atoms_stride, status);
+                               sequences, seq_stride, a, angles_stride, c, rn, an, atoms_stride, status, order);
     } else {
         const double* a = static_cast<const double*>(angles);
         double* c = static_cast<double*>(coords);
         if (large)
             hipLaunchKernelGGL((a2c_fwd<kThreadsLarge, double>), dim3(batch), dim3(kThreadsLarge), 0, s,
-                               sequences, seq_stride, a, angles_stride, c, rn, an, atoms_stride, status);
+                               sequences, seq_stride, a, angles_stride, c, rn, an, atoms_stride, status, order);
         else
             hipLaunchKernelGGL((a2c_fwd<kThreadsSmall, double>), dim3(batch), dim3(kThreadsSmall), 0, s,
-                               sequences, seq_stride, a, angles_stride, c, rn, an, atoms_stride, status);
+                               sequences, seq_stride, a, angles_stride, c, rn, an, atoms_stride, status, order);
     }
     return hipGetLastError() == hipSuccess ? TPL_A2C_OK : TPL_A2C_ERR_LAUNCH;
 }
 
-int tpl_a2c_backward_form(int form, int dtype, const uint8_t* sequences, int seq_stride, const void* coords,
-                          const void* grad_coords, int atoms_stride, void* grad_angles, int angles_stride,
-                          uint32_t* status, int batch, tpl_a2c_stream_t stream) {
+int tpl_a2c_forward(int dtype, const uint8_t* sequences, int seq_stride, const void* angles,
+                    int angles_stride, void* coords, uint8_t* res_names, uint8_t* atom_names,
+                    int atoms_stride, uint32_t* status, int batch, tpl_a2c_stream_t stream) {
+    return tpl_a2c_forward_ordered(nullptr, dtype, sequences, seq_stride, angles, angles_stride, coords,
+                                   res_names, atom_names, atoms_stride, status, batch, stream);
+}
+
+int tpl_a2c_backward_form(int form, const int32_t* order, int dtype, const uint8_t* sequences, int seq_stride,
+                          const void* coords, const void* grad_coords, int atoms_stride, void* grad_angles,
+                          int angles_stride, uint32_t* status, int batch, tpl_a2c_stream_t stream) {
     if (form != TPL_A2C_BWD_AUTO && form != TPL_A2C_BWD_RUNS && form != TPL_A2C_BWD_TILES) return TPL_A2C_ERR_ARG;
     const int rc = check(dtype, batch, seq_stride, angles_stride, atoms_stride);
     if (rc != TPL_A2C_OK) return rc;
@@ -1142,12 +1165,12 @@ int tpl_a2c_backward_form(int form, int dtype, const uint8_t* sequences, int seq
             hipLaunchKernelGGL((a2c_bwd_tiled<float>), dim3(batch), dim3(kTileThreads), 0, s, sequences,
                                seq_stride, static_cast<const float*>(coords),
                                static_cast<const float*>(grad_coords), atoms_stride,
-                               static_cast<float*>(grad_angles), angles_stride, status);
+                               static_cast<float*>(grad_angles), angles_stride, status, order);
         else
             hipLaunchKernelGGL((a2c_bwd_tiled<double>), dim3(batch), dim3(kTileThreads), 0, s, sequences,
                                seq_stride, static_cast<const double*>(coords),
                                static_cast<const double*>(grad_coords), atoms_stride,
-                               static_cast<double*>(grad_angles), angles_stride, status);
+                               static_cast<double*>(grad_angles), angles_stride, status, order);
         return hipGetLastError() == hipSuccess ? TPL_A2C_OK : TPL_A2C_ERR_LAUNCH;
     }
     const bool large = use_large_bwd(angles_stride);
@@ -1156,19 +1179,19 @@ int tpl_a2c_backward_form(int form, int dtype, const uint8_t* sequences, int seq
         float* ga = static_cast<float*>(grad_angles);
         if (large)
             hipLaunchKernelGGL((a2c_bwd<kThreadsLarge, float>), dim3(batch), dim3(kThreadsLarge), 0, s,
-                               sequences, seq_stride, x, g, atoms_stride, ga, angles_stride, status);
+                               sequences, seq_stride, x, g, atoms_stride, ga, angles_stride, status, order);
         else
             hipLaunchKernelGGL((a2c_bwd<kThreadsSmall, float>), dim3(batch), dim3(kThreadsSmall), 0, s,
-                               sequences, seq_stride, x, g, atoms_stride, ga, angles_stride, status);
+                               sequences, seq_stride, x, g, atoms_stride, ga, angles_stride, status, order);
     } else {
         const double *x = static_cast<const double*>(coords), *g = static_cast<const double*>(grad_coords);
         double* ga = static_cast<double*>(grad_angles);
         if (large)
             hipLaunchKernelGGL((a2c_bwd<kThreadsLarge, double>), dim3(batch), dim3(kThreadsLarge), 0, s,
-                               sequences, seq_stride, x, g, atoms_stride, ga, angles_stride, status);
+                               sequences, seq_stride, x, g, atoms_stride, ga, angles_stride, status, order);
         else
             hipLaunchKernelGGL((a2c_bwd<kThreadsSmall, double>), dim3(batch), dim3(kThreadsSmall), 0, s,
-                               sequences, seq_stride, x, g, atoms_stride, ga, angles_stride, status);
+                               sequences, seq_stride, x, g, atoms_stride, ga, angles_stride, status, order);
     }
     return hipGetLastError() == hipSuccess ? TPL_A2C_OK : TPL_A2C_ERR_LAUNCH;
 }
@@ -1176,7 +1199,7 @@ int tpl_a2c_backward_form(int form, int dtype, const uint8_t* sequences, int seq
 int tpl_a2c_backward(int dtype, const uint8_t* sequences, int seq_stride, const void* coords,
                      const void* grad_coords, int atoms_stride, void* grad_angles, int angles_stride,
                      uint32_t* status, int batch, tpl_a2c_stream_t stream) {
-    return tpl_a2c_backward_form(TPL_A2C_BWD_AUTO, dtype, sequences, seq_stride, coords, grad_coords,
+    return tpl_a2c_backward_form(TPL_A2C_BWD_AUTO, nullptr, dtype, sequences, seq_stride, coords, grad_coords,
                                  atoms_stride, grad_angles, angles_stride, status, batch, stream);
 }
 

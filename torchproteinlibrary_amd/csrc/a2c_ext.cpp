@@ -50,12 +50,24 @@ uint32_t* status_ptr(const c10::optional<torch::Tensor>& status, const torch::Te
     return reinterpret_cast<uint32_t*>(status->data_ptr());
 }
 
+// The optional scheduling order (tpl_a2c_forward_ordered): int32 [batch] on
+// the angles' device.
+const int32_t* order_ptr(const c10::optional<torch::Tensor>& order, const torch::Tensor& like) {
+    if (!order.has_value() || !order->defined()) return nullptr;
+    A2C_CHECK_GPU(*order);
+    TORCH_CHECK(order->scalar_type() == torch::kInt && order->dim() == 1 && order->size(0) == like.size(0),
+                "order must be int32 [batch]");
+    TORCH_CHECK(order->device() == like.device(), "order must be on the device of the angles");
+    return order->data_ptr<int32_t>();
+}
+
 hipStream_t stream_of(const torch::Tensor& t) {
     return c10::hip::getCurrentHIPStream(t.device().index()).stream();
 }
 
 void forward(torch::Tensor sequences, torch::Tensor input_angles, torch::Tensor output_coords,
-             torch::Tensor res_names, torch::Tensor atom_names, c10::optional<torch::Tensor> status) {
+             torch::Tensor res_names, torch::Tensor atom_names, c10::optional<torch::Tensor> status,
+             c10::optional<torch::Tensor> order) {
     check_common(sequences, input_angles);
     A2C_CHECK_GPU(output_coords);
     A2C_CHECK_GPU(res_names);
@@ -74,16 +86,18 @@ void forward(torch::Tensor sequences, torch::Tensor input_angles, torch::Tensor 
     }
     TORCH_CHECK(output_coords.device() == input_angles.device(), "coordinates must be on the device of the angles");
     c10::DeviceGuard guard(input_angles.device());
-    check_rc(tpl_a2c_forward(dtype_code(input_angles), sequences.data_ptr<uint8_t>(), (int)sequences.size(1),
-                             input_angles.data_ptr(), (int)input_angles.size(2), output_coords.data_ptr(),
-                             res_names.data_ptr<uint8_t>(), atom_names.data_ptr<uint8_t>(), (int)A,
-                             status_ptr(status, input_angles), (int)B, stream_of(input_angles)),
+    check_rc(tpl_a2c_forward_ordered(order_ptr(order, input_angles), dtype_code(input_angles),
+                                     sequences.data_ptr<uint8_t>(), (int)sequences.size(1), input_angles.data_ptr(),
+                                     (int)input_angles.size(2), output_coords.data_ptr(),
+                                     res_names.data_ptr<uint8_t>(), atom_names.data_ptr<uint8_t>(), (int)A,
+                                     status_ptr(status, input_angles), (int)B, stream_of(input_angles)),
              "Angles2Coords_forward");
 }
 
 // From the forward's coordinates (what the Python layer saves).
 void backward_coords(torch::Tensor grad_atoms, torch::Tensor grad_angles, torch::Tensor sequences,
-                     torch::Tensor coords, c10::optional<torch::Tensor> status) {
+                     torch::Tensor coords, c10::optional<torch::Tensor> status,
+                     c10::optional<torch::Tensor> order) {
     check_common(sequences, grad_angles);
     A2C_CHECK_GPU(grad_atoms);
     A2C_CHECK_GPU(coords);
@@ -99,10 +113,10 @@ void backward_coords(torch::Tensor grad_atoms, torch::Tensor grad_angles, torch:
     const int64_t A = grad_atoms.size(1) / 3;
     TORCH_CHECK(A <= INT32_MAX / 3, "grad_atoms too large");
     c10::DeviceGuard guard(grad_angles.device());
-    check_rc(tpl_a2c_backward(dtype_code(grad_angles), sequences.data_ptr<uint8_t>(), (int)sequences.size(1),
-                              coords.data_ptr(), grad_atoms.data_ptr(), (int)A, grad_angles.data_ptr(),
-                              (int)grad_angles.size(2), status_ptr(status, grad_angles), (int)B,
-                              stream_of(grad_angles)),
+    check_rc(tpl_a2c_backward_form(TPL_A2C_BWD_AUTO, order_ptr(order, grad_angles), dtype_code(grad_angles),
+                                   sequences.data_ptr<uint8_t>(), (int)sequences.size(1), coords.data_ptr(),
+                                   grad_atoms.data_ptr(), (int)A, grad_angles.data_ptr(), (int)grad_angles.size(2),
+                                   status_ptr(status, grad_angles), (int)B, stream_of(grad_angles)),
              "Angles2Coords_backward");
 }
 
@@ -124,7 +138,7 @@ void backward(torch::Tensor grad_atoms, torch::Tensor grad_angles, torch::Tensor
                              nullptr, (int)A, status_ptr(status, input_angles), (int)input_angles.size(0),
                              stream_of(input_angles)),
              "Angles2Coords_backward");
-    backward_coords(grad_atoms, grad_angles, sequences, coords, status);
+    backward_coords(grad_atoms, grad_angles, sequences, coords, status, c10::nullopt);
 }
 
 int num_atoms(const std::string& sequence) {
@@ -134,14 +148,16 @@ int num_atoms(const std::string& sequence) {
 }
 
 // Timing loop for bench.py: sets of [sequences, angles, coords, grad_coords,
-// grad_angles] rotated per launch (no names, no status).  Returns the average
+// grad_angles] (and optionally the scheduling order, int32 [batch]) rotated
+// per launch (no names, no status).  Returns the average
 // HIP-event ms of (fwd+bwd step, fwd alone, bwd alone) over `steps` launches
 // each, after `warmup`.
 std::tuple<double, double, double> bench(std::vector<std::vector<torch::Tensor>> sets, int64_t steps,
                                          int64_t warmup) {
     TORCH_CHECK(!sets.empty() && steps > 0 && warmup >= 0, "bench: bad arguments");
     for (auto& t : sets) {
-        TORCH_CHECK(t.size() == 5, "bench: a set has 5 tensors");
+        TORCH_CHECK(t.size() == 5 || t.size() == 6, "bench: a set has 5 tensors (6 with an order)");
+        if (t.size() == 6) (void)order_ptr(t[5], t[1]);
         check_common(t[0], t[1]);
         for (int k = 2; k < 5; ++k) {
             A2C_CHECK_GPU(t[k]);
@@ -153,15 +169,17 @@ std::tuple<double, double, double> bench(std::vector<std::vector<torch::Tensor>>
     hipStream_t stream = stream_of(sets[0][1]);
     const int dt = dtype_code(sets[0][1]);
     auto fwd = [&](std::vector<torch::Tensor>& t) {
-        check_rc(tpl_a2c_forward(dt, t[0].data_ptr<uint8_t>(), (int)t[0].size(1), t[1].data_ptr(),
-                                 (int)t[1].size(2), t[2].data_ptr(), nullptr, nullptr, (int)(t[2].size(1) / 3),
-                                 nullptr, (int)t[1].size(0), stream),
+        const int32_t* order = t.size() == 6 ? t[5].data_ptr<int32_t>() : nullptr;
+        check_rc(tpl_a2c_forward_ordered(order, dt, t[0].data_ptr<uint8_t>(), (int)t[0].size(1), t[1].data_ptr(),
+                                         (int)t[1].size(2), t[2].data_ptr(), nullptr, nullptr,
+                                         (int)(t[2].size(1) / 3), nullptr, (int)t[1].size(0), stream),
                  "forward");
     };
     auto bwd = [&](std::vector<torch::Tensor>& t) {
-        check_rc(tpl_a2c_backward(dt, t[0].data_ptr<uint8_t>(), (int)t[0].size(1), t[2].data_ptr(),
-                                  t[3].data_ptr(), (int)(t[3].size(1) / 3), t[4].data_ptr(), (int)t[4].size(2),
-                                  nullptr, (int)t[1].size(0), stream),
+        const int32_t* order = t.size() == 6 ? t[5].data_ptr<int32_t>() : nullptr;
+        check_rc(tpl_a2c_backward_form(TPL_A2C_BWD_AUTO, order, dt, t[0].data_ptr<uint8_t>(), (int)t[0].size(1),
+                                       t[2].data_ptr(), t[3].data_ptr(), (int)(t[3].size(1) / 3), t[4].data_ptr(),
+                                       (int)t[4].size(2), nullptr, (int)t[1].size(0), stream),
                  "backward");
     };
     hipEvent_t e0, e1;
@@ -191,14 +209,15 @@ std::tuple<double, double, double> bench(std::vector<std::vector<torch::Tensor>>
 void register_a2c(pybind11::module& m) {
     namespace py = pybind11;
     auto st = py::arg("status") = py::none();
+    auto od = py::arg("order") = py::none();
     m.def("Angles2Coords_forward", &forward, "full-atom coordinates and names from angles (GPU)",
           py::arg("sequences"), py::arg("input_angles"), py::arg("output_coords"), py::arg("res_names"),
-          py::arg("atom_names"), st);
+          py::arg("atom_names"), st, od);
     m.def("Angles2Coords_backward", &backward, "gradient of the coordinates with respect to the angles (GPU)",
           py::arg("grad_atoms"), py::arg("grad_angles"), py::arg("sequences"), py::arg("input_angles"), st);
     m.def("Angles2Coords_backward_coords", &backward_coords,
           "the same from the forward's coordinates (GPU)", py::arg("grad_atoms"), py::arg("grad_angles"),
-          py::arg("sequences"), py::arg("coords"), st);
+          py::arg("sequences"), py::arg("coords"), st, od);
     m.def("getSeqNumAtoms", &num_atoms, "atoms of a sequence", py::arg("sequence"));
     m.def("a2c_bench", &bench, "average event ms of Angles2Coords (step, fwd, bwd)");
 }

@@ -583,12 +583,29 @@ __device__ __forceinline__ void load3(const T* p, double v[3]) {
     v[2] = (double)p[2];
 }
 
-// Sums over atoms [k0, k1) of a structure.  (Unrolling by four atoms with
-// the loads issued first measured slower: 153 -> 204 us for the mixed1000
-// backward with it in pass A only, 281 us with it everywhere.)
-template <typename T>
+// Sums over atoms [k0, k1) of a structure.  With UNROLL, four atoms at a time
+// with their loads issued first.  This kernel now serves small batches,
+// where it is latency-bound: at 512 threads, 32 x 350 residues 18.0 -> 16.9
+// us, one structure of 1000 36.0 -> 33.9, 64 x 1000 43.9 -> 41.0.  At 256
+// threads the extra registers cost a wave per SIMD (1024 x 30: 13.9 -> 18.7
+// us), so it stays off there.  (When this kernel also served B=1024 of
+// mixed lengths 30-1000, unrolling measured slower there: 153 -> 204 us.)
+template <bool UNROLL, typename T>
 __device__ __forceinline__ void atom_sums(const T* xs, const T* gs, int k0, int k1, Sums& acc) {
-    for (int k = k0; k < k1; ++k) {
+    int k = k0;
+    if (UNROLL) {
+        for (; k + 4 <= k1; k += 4) {
+            double x[4][3], g[4][3];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                load3(xs + 3 * (size_t)(k + j), x[j]);
+                load3(gs + 3 * (size_t)(k + j), g[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) sums_add(acc, x[j], g[j]);
+        }
+    }
+    for (; k < k1; ++k) {
         double x[3], g[3];
         load3(xs + 3 * (size_t)k, x);
         load3(gs + 3 * (size_t)k, g);
@@ -639,7 +656,7 @@ __global__ void __launch_bounds__(NT) a2c_bwd(const uint8_t* __restrict__ seqs, 
     {
         int a = run.atom0;
         for (int i = run.r0; i < run.r1; ++i) a += s_res[residue_type(seq[i])] & 0xff;
-        atom_sums(xs, gs, run.atom0, a, run_sum);
+        atom_sums<NT == kThreadsLarge>(xs, gs, run.atom0, a, run_sum);
     }
     // incl = sums over every atom from this run on: runs are in reverse
     // thread order, so an inclusive prefix sum over threads (DPP within the
@@ -678,9 +695,9 @@ __global__ void __launch_bounds__(NT) a2c_bwd(const uint8_t* __restrict__ seqs, 
         sums_zero(sCA);
         sums_zero(sSide);
         sums_zero(sC);
-        atom_sums(xs, gs, a, a + 1, sN);
-        atom_sums(xs, gs, a + 1, a + 1 + nca, sCA);
-        atom_sums(xs, gs, a + natoms - 2, a + natoms, sC);
+        atom_sums<NT == kThreadsLarge>(xs, gs, a, a + 1, sN);
+        atom_sums<NT == kThreadsLarge>(xs, gs, a + 1, a + 1 + nca, sCA);
+        atom_sums<NT == kThreadsLarge>(xs, gs, a + natoms - 2, a + natoms, sC);
         double pN[3], pCA[3], pC[3];
         load3(xs + 3 * (size_t)a, pN);
         load3(xs + 3 * (size_t)(a + 1), pCA);
@@ -694,7 +711,7 @@ __global__ void __launch_bounds__(NT) a2c_bwd(const uint8_t* __restrict__ seqs, 
             const int si = s_side[t * kMaxSide + n];
             const int row = (int)(int8_t)(si & 0xff), first = (si >> 8) & 0xff, count = si >> 16;
             if (count == 0) continue;
-            atom_sums(xs, gs, a + first, a + first + count, sSide);
+            atom_sums<NT == kThreadsLarge>(xs, gs, a + first, a + first + count, sSide);
             if (row >= 0) {
                 const int sp = n > 0 ? s_side[t * kMaxSide + n - 1] : 0;
                 const int pfirst = (sp >> 8) & 0xff, pcount = sp >> 16;

@@ -29,6 +29,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+#include <vector>
+
 #include "a2c_table.h"
 #include "sincos_tab.h"
 #include "tpl_a2c.h"
@@ -39,7 +42,7 @@ using namespace a2c;
 
 // Workgroup sizes: 256 threads, or 512 for long structures (see use_large_fwd
 // and use_large_bwd), where a thread then walks half as many residues.  LDS:
-// forward 13.6 / 14.0 KB (the tables), backward 0.8 / 1.0 KB.
+// forward 13.6 / 14.0 KB (the tables), run backward 0.8 / 1.0 KB.
 constexpr int kThreadsSmall = 256, kThreadsLarge = 512;
 
 __constant__ Residue kRes[kTypes] = A2C_RESIDUES;
@@ -93,6 +96,28 @@ __device__ void fill_tables(Tables& tb) {
         const double beta = threadIdx.x == 0 ? geo::C_N_CA : (threadIdx.x == 1 ? geo::N_CA_C : geo::CA_C_N);
         sc(beta, &tb.bb[threadIdx.x][1], &tb.bb[threadIdx.x][0]);
     }
+}
+
+// The tables are the same for every workgroup and call: a one-workgroup
+// kernel computes them once per device into g_tab, and each forward
+// workgroup copies that (1.3 us less per workgroup than computing them:
+// 32 x 350 residues 17.5 -> 16.2 us, 1024 x 30 15.2 -> 13.4 us).
+__device__ Tables g_tab;
+
+__global__ void __launch_bounds__(256) a2c_init_tables() {
+    __shared__ Tables tb;
+    fill_tables<256>(tb);
+    __syncthreads();
+    const double* src = reinterpret_cast<const double*>(&tb);
+    double* dst = reinterpret_cast<double*>(&g_tab);
+    for (int k = threadIdx.x; k < (int)(sizeof(Tables) / 8); k += 256) dst[k] = src[k];
+}
+
+template <int NT>
+__device__ __forceinline__ void copy_tables(Tables& tb) {
+    const double* src = reinterpret_cast<const double*>(&g_tab);
+    double* dst = reinterpret_cast<double*>(&tb);
+    for (int k = threadIdx.x; k < (int)(sizeof(Tables) / 8); k += NT) dst[k] = src[k];
 }
 
 // m (3x4, row-major) <- m * Ry(beta) Tx(d) Rx(alpha), with cb/sb of beta and
@@ -412,7 +437,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 2
     const uint8_t* seq = seqs + (size_t)b * seq_stride;
     const T* ang = angles + (size_t)b * 8 * ang_stride;
     T* out = coords + (size_t)b * 3 * atoms_stride;
-    fill_tables<NT>(tb);
+    copy_tables<NT>(tb);
     __syncthreads();
     Run run;
     double G[12];
@@ -1088,6 +1113,23 @@ bool use_tiles(int batch, int ang_stride) {
     return ang_stride <= kTileMaxRes && (int64_t)batch * ang_stride >= 100000;
 }
 
+// g_tab of the current device, filled on the first forward there: the
+// kernel runs on the caller's stream, which is then synchronised once, so
+// that forwards on any stream of the device find the tables.
+bool tables_ready(hipStream_t s) {
+    static std::mutex mu;
+    static std::vector<char> ready;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return false;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)ready.size() <= dev) ready.resize(dev + 1, 0);
+    if (ready[dev]) return true;
+    hipLaunchKernelGGL(a2c_init_tables, dim3(1), dim3(256), 0, s);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) return false;
+    ready[dev] = 1;
+    return true;
+}
+
 int check(int dtype, int B, int seq_stride, int ang_stride, int atoms_stride) {
     if (dtype != TPL_A2C_F32 && dtype != TPL_A2C_F64) return TPL_A2C_ERR_ARG;
     if (B < 0 || seq_stride < 0 || ang_stride < 0 || atoms_stride < 0) return TPL_A2C_ERR_ARG;
@@ -1133,6 +1175,7 @@ int tpl_a2c_forward_ordered(const int32_t* order, int dtype, const uint8_t* sequ
         (!res_names) != (!atom_names))
         return TPL_A2C_ERR_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!tables_ready(s)) return TPL_A2C_ERR_LAUNCH;
     uint32_t* rn = reinterpret_cast<uint32_t*>(res_names);
     uint32_t* an = reinterpret_cast<uint32_t*>(atom_names);
     const bool large = use_large_fwd(batch, angles_stride);

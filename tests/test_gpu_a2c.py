@@ -171,18 +171,18 @@ def test_batch_mixed_lengths_full_size():
 
 
 def test_large_workgroups_match_small_and_oracle():
-    """A few long structures (batch < 256, stride > 256) run the forward on
-    512-thread workgroups; the same structures padded to a batch of 256 with
-    empty sequences run it on 256-thread workgroups (the backward takes 512
-    threads in both, for the stride).  Both agree with each other and with
-    the oracle (FP64)."""
+    """A few long structures (batch <= 512, stride > 256) run the forward on
+    512-thread workgroups; the same structures padded to a batch of 513 with
+    empty sequences run it on 256-thread workgroups (and the backward on the
+    atom-tile kernel instead of per-thread runs, for batch x stride).  Both
+    agree with each other and with the oracle (FP64)."""
     from torchproteinlibrary_amd.FullAtomModel import Angles2Coords
     rng = np.random.default_rng(12)
     lengths = [300, 520, 7, 260]
     print("a2c 512-thread path: lengths", lengths, flush=True)
     seqs, ang = _batch(rng, lengths, 520, torch.float64)
-    pad = seqs + [""] * 252
-    ang_pad = np.concatenate([ang, np.zeros((252, 8, 520))])
+    pad = seqs + [""] * 509
+    ang_pad = np.concatenate([ang, np.zeros((509, 8, 520))])
     Xa = torch.tensor(ang, device=DEV, requires_grad=True)
     Xb = torch.tensor(ang_pad, device=DEV, requires_grad=True)
     ca = Angles2Coords()(Xa, seqs)[0]
@@ -192,7 +192,7 @@ def test_large_workgroups_match_small_and_oracle():
     assert np.abs(ca.detach().cpu().numpy() - want).max() < 1e-9
     g = torch.randn_like(ca)
     ca.backward(g)
-    cb.backward(torch.cat([g, torch.zeros(252, g.shape[1], dtype=g.dtype, device=DEV)]))
+    cb.backward(torch.cat([g, torch.zeros(509, g.shape[1], dtype=g.dtype, device=DEV)]))
     ga, gb = Xa.grad, Xb.grad[:4]
     assert (ga - gb).abs().max().item() <= 1e-10 * gb.abs().max().item()
     want_g = O.backward_one(seqs[2], ang[2], g[2].cpu().numpy())
@@ -204,7 +204,7 @@ def test_wave_boundary_lengths_both_workgroup_sizes():
     residue per thread in 512-thread workgroups, one or two in 256-thread
     ones), so that runs end in the last lane of a wave, in the first lane of
     the next, and waves past the structure skip the frame scan.  The batch of
-    9 runs the forward on 512 threads; the same structures padded to 256 with
+    9 runs the forward on 512 threads; the same structures padded to 513 with
     empty sequences run it on 256 threads.  FP64 against the oracle, both
     directions."""
     from torchproteinlibrary_amd.FullAtomModel import Angles2Coords
@@ -213,8 +213,8 @@ def test_wave_boundary_lengths_both_workgroup_sizes():
     print("a2c wave-boundary lengths", lengths, flush=True)
     seqs, ang = _batch(rng, lengths, 257, torch.float64)
     nb = len(lengths)
-    pad = seqs + [""] * (256 - nb)
-    ang_pad = np.concatenate([ang, np.zeros((256 - nb, 8, 257))])
+    pad = seqs + [""] * (513 - nb)
+    ang_pad = np.concatenate([ang, np.zeros((513 - nb, 8, 257))])
     for ss, aa in ((seqs, ang), (pad, ang_pad)):
         X = torch.tensor(aa, device=DEV, requires_grad=True)
         coords = Angles2Coords()(X, ss)[0]

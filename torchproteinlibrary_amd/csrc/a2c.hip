@@ -1089,12 +1089,13 @@ __global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_
 }
 
 // 512-thread workgroups when some structure may be longer than 256 residues:
-// always for the backward (mixed lengths 30-1000 at B=1024: 155 -> 129 us),
-// and for the forward only when the structures are at most as many as the
-// CUs (B=256 of 1000 residues: 33.7 -> 28.6 us; at B=1024 with mixed lengths
-// 512 threads measured slower, 57 -> 72 us, with fewer workgroups per CU).
+// always for the run backward (mixed lengths 30-1000 at B=1024: 155 -> 129
+// us), and for the forward only up to twice as many structures as CUs (B=256
+// of 1000 residues: 33.7 -> 28.6 us; B=512 of 300: 23.8 -> 21.8 us; at
+// B=1024 512 threads measured slower, with fewer workgroups per CU: mixed
+// lengths 30-1000 48.4 -> 55.7 us, 1024 x 270 32.0 -> 39.6 us).
 bool use_large_fwd(int batch, int ang_stride) {
-    return batch <= 256 && ang_stride > kThreadsSmall;
+    return batch <= 512 && ang_stride > kThreadsSmall;
 }
 
 bool use_large_bwd(int ang_stride) {

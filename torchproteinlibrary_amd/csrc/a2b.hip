@@ -37,7 +37,9 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "sincos_tab.h"
@@ -983,6 +985,131 @@ __global__ void __launch_bounds__(64) a2b_fwd_chunk(ChunkArgs a) {
     fwd_slot<T, R>(a, slot_at(a, C, st, j), s_x, s_tab, tab);
 }
 
+// ------------------------------------------------------------ forward chain
+// Chains of at most kChainMax chunks (L <= kChainMax * C) can skip the
+// look-back: one workgroup per chain, one wave per chunk slot, and the
+// chunks' aggregates meet in LDS across one barrier.  No tickets, records or
+// done counters, and each wave finds its chunk from the workgroup index.
+// Wave w takes slot (w + b) mod W, so that a chain's first chunk, the only
+// one every chain has, does not always fall on the same wave (and SIMD).
+// Otherwise as fwd_slot: pass 1, DPP scan, E = (earlier chunks' aggregates
+// in order) * (lane prefix), checkpoints, pass 2, stores; the slot right
+// past the chain writes its zeros.
+constexpr int kChainMax = 4;
+
+template <typename T, int R, int W>
+__global__ void __launch_bounds__(64 * W) a2b_fwd_chain(ChunkArgs a) {
+    constexpr int C = 64 * R;
+    constexpr int E9 = 9 * R;
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    __shared__ double s_tab[128];
+    __shared__ double s_agg[W][kF];
+    const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int b = (int)blockIdx.x, L = a.L;
+    const int raw = a.lengths[b];
+    const int n = raw < 0 ? 0 : (raw > L ? L : raw);
+    const int nreal = (n + C - 1) / C;
+    const int c = (wave + b) % W;  // this wave's chunk slot
+    const bool real = c < nreal;
+    if (raw != n && threadIdx.x == 0) set_status(a.status, TPL_A2B_STATUS_LENGTH);
+    for (int i = (int)threadIdx.x; i < 128; i += 64 * W) s_tab[i] = kTabCS[i];
+    T* s_x = reinterpret_cast<T*>(s_dyn) + (size_t)wave * 9 * C;
+    T* s_l = s_x + lane * E9;
+    T* xrow = static_cast<T*>(a.coords) + (size_t)b * 9 * L;
+    const T* arow = static_cast<const T*>(a.angles) + (size_t)b * 3 * L;
+    const int r0 = c * C;
+    const int rl = r0 + lane * R;
+    const int cnt = min(max(n - rl, 0), R);
+    T ang[3][R];
+    if (real) lane_load_rows<T, R>(arow + rl, L, cnt, L - rl, ang);
+    __syncthreads();  // s_tab
+
+    // Pass 1 and the wave's scan.
+    double E[kF];
+    frame_identity(E);
+    if (real) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double sn, cs;
+                sincos_tab((double)ang[k][q], s_tab, &sn, &cs);
+                if (k == 0) frame_step<0>(E, cs, sn);
+                if (k == 1) frame_step<1>(E, cs, sn);
+                if (k == 2) frame_step<2>(E, cs, sn);
+                const int at = 3 * q + k + 1;
+                if (at < 3 * R) {
+                    s_l[3 * at] = (T)E[9];
+                    s_l[3 * at + 1] = (T)E[10];
+                    s_l[3 * at + 2] = (T)E[11];
+                }
+            }
+        }
+        wave_scan_frames(E);
+        double agg[kF];
+#pragma unroll
+        for (int e = 0; e < kF; ++e) agg[e] = read_lane(E[e], 63);
+        if (lane == 0) {
+#pragma unroll
+            for (int e = 0; e < kF; ++e) s_agg[c][e] = agg[e];
+        }
+        wave_shift_frames(E);
+    }
+    __syncthreads();  // the aggregates
+    if (real && c > 0) {
+        double P[kF], Q[kF];
+#pragma unroll
+        for (int e = 0; e < kF; ++e) P[e] = s_agg[0][e];
+        for (int j = 1; j < c; ++j) {
+            frame_rmul_to(P, s_agg[j], Q);
+#pragma unroll
+            for (int e = 0; e < kF; ++e) P[e] = Q[e];
+        }
+        frame_lmul(P, E);
+    }
+    if (!real) {
+        if (c == nreal) {  // the first slot past the chain writes all its zeros
+            const int f0 = 9 * c * C;
+            wave_store_run<T>(xrow + f0, 9 * L - f0, 0, s_x, lane);
+        }
+        return;
+    }
+
+    // Checkpoint at a backward-chunk boundary inside the chain.
+    if (a.ckpt && rl > 0 && rl < n && (rl & (a.K - 1)) == 0) {
+        double* d = a.ckpt + ((size_t)b * a.nk + rl / a.K) * kF;
+#pragma unroll
+        for (int e = 0; e < kF; ++e) d[e] = E[e];
+    }
+
+    // Pass 2, in place in the lane's slot.
+    if (cnt > 0) {
+        float chk = 0.f;
+#pragma unroll
+        for (int at = 0; at < 3 * R; ++at) {
+            double x[3];
+            if (at == 0) {
+                x[0] = E[9];
+                x[1] = E[10];
+                x[2] = E[11];
+            } else {
+                affine(E, (double)s_l[3 * at], (double)s_l[3 * at + 1], (double)s_l[3 * at + 2], x);
+            }
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                const T o = (T)x[cc];
+                s_l[3 * at + cc] = o;
+                chk += (float)o;
+            }
+        }
+        if (!__builtin_isfinite(chk)) set_status(a.status, TPL_A2B_STATUS_NAN);
+    }
+    wave_lds_sync();
+    const int g0 = 9 * r0;
+    const int wcount = min(9 * C, 9 * L - g0);
+    wave_store_run<T>(xrow + g0, wcount, min(max(9 * n - g0, 0), wcount), s_x, lane);
+}
+
 // ----------------------------------------------------------- backward chunk
 // Chain atom j >= 1 (atoms N_0, CA_0, C_0, N_1, ...) is placed by the
 // transform whose angle theta_j is phi_r, psi_r, omega_r for j = 3r+1, 3r+2,
@@ -1240,6 +1367,191 @@ __global__ void __launch_bounds__(64) a2b_bwd_chunk(ChunkArgs a) {
     bwd_slot<T, R>(a, slot_at(a, C, st, j), s_tab, s_cs, tab);
 }
 
+// ----------------------------------------------------------- backward chain
+// The backward of chains of at most kChainMax chunks: one workgroup per
+// chain, one wave per chunk slot (slot (w + b) mod W, as in the forward), and
+// each chunk's sum over its atoms meets the others in LDS across one barrier:
+// chunk c adds the sums of chunks c+1 .. nreal-1, in that order.  Otherwise
+// as bwd_slot.
+template <typename T, int R, int W>
+__global__ void __launch_bounds__(64 * W) a2b_bwd_chain(ChunkArgs a) {
+    constexpr int C = 64 * R;
+    extern __shared__ double s_dynd[];  // [W][6R][64]: each wave's (cos, sin)
+    __shared__ double s_tab[128];
+    __shared__ double s_tot[W][6];
+    const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int b = (int)blockIdx.x, L = a.L;
+    const int raw = a.lengths[b];
+    const int n = raw < 0 ? 0 : (raw > L ? L : raw);
+    const int nreal = (n + C - 1) / C;
+    const int c = (wave + b) % W;  // this wave's chunk slot
+    const bool real = c < nreal;
+    if (raw != n && threadIdx.x == 0) set_status(a.status, TPL_A2B_STATUS_LENGTH);
+    for (int i = (int)threadIdx.x; i < 128; i += 64 * W) s_tab[i] = kTabCS[i];
+    double* s_cs = s_dynd + (size_t)wave * 6 * R * 64;
+    T* grow = static_cast<T*>(a.grad_angles) + (size_t)b * 3 * L;
+    const T* arow = static_cast<const T*>(a.angles) + (size_t)b * 3 * L;
+    const T* gin = static_cast<const T*>(a.grad) + (size_t)b * 9 * L;
+    const int r0 = c * C;
+    const int rl = r0 + lane * R;
+    const int cnt = min(max(n - rl, 0), R);
+    T ang[3][R];
+    T gr[9 * R];
+    double P[kF];  // checkpoint: frame of the chunk's first atom (uniform)
+    if (real) {
+        if (cnt > 0) {  // a lane wholly past the chain reads nothing
+            lane_load_rows<T, R>(arow + rl, L, cnt, L - rl, ang);
+            lane_load<T, 9 * R>(gin + (size_t)9 * rl, 9 * cnt, gr, 9 * (L - rl));
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+                for (int i = 0; i < R; ++i) ang[kk][i] = (T)0;
+#pragma unroll
+            for (int i = 0; i < 9 * R; ++i) gr[i] = (T)0;
+        }
+        if (c == 0) {
+            frame_identity(P);
+        } else {
+            const double* ck = a.ckpt + ((size_t)b * a.nk + c) * kF;
+#pragma unroll
+            for (int e = 0; e < kF; ++e) P[e] = ck[e];
+        }
+    }
+    __syncthreads();  // s_tab
+
+    double f[kF], W6[6], S[6];
+    if (real) {
+        // Pass 1 in the lane's own frame.
+        frame_identity(f);
+        double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double F[3] = {(double)gr[0], (double)gr[1], (double)gr[2]};
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double sn, cs;
+                sincos_tab((double)ang[k][q], s_tab, &sn, &cs);
+                s_cs[(6 * q + 2 * k) * 64 + lane] = cs;
+                s_cs[(6 * q + 2 * k + 1) * 64 + lane] = sn;
+                if (k == 0) frame_step<0>(f, cs, sn);
+                if (k == 1) frame_step<1>(f, cs, sn);
+                if (k == 2) frame_step<2>(f, cs, sn);
+                const int at = 3 * q + k + 1;
+                if (at < 3 * R) {
+                    const double g0 = (double)gr[3 * at], g1 = (double)gr[3 * at + 1],
+                                 g2 = (double)gr[3 * at + 2];
+#pragma unroll
+                    for (int m = 0; m < 3; ++m) {
+                        M[3 * m] = fma(f[9 + m], g0, M[3 * m]);
+                        M[3 * m + 1] = fma(f[9 + m], g1, M[3 * m + 1]);
+                        M[3 * m + 2] = fma(f[9 + m], g2, M[3 * m + 2]);
+                    }
+                    F[0] += g0;
+                    F[1] += g1;
+                    F[2] += g2;
+                }
+            }
+        }
+        wave_scan_frames(f);
+        wave_shift_frames(f);
+        frame_lmul(P, f);  // f = E, the frame of the lane's first atom
+        auto rm = [&](int j, int k) {
+            return f[j] * M[k] + f[3 + j] * M[3 + k] + f[6 + j] * M[6 + k];
+        };
+        W6[0] = rm(1, 2) - rm(2, 1) + (f[10] * F[2] - f[11] * F[1]);
+        W6[1] = rm(2, 0) - rm(0, 2) + (f[11] * F[0] - f[9] * F[2]);
+        W6[2] = rm(0, 1) - rm(1, 0) + (f[9] * F[1] - f[10] * F[0]);
+        W6[3] = F[0];
+        W6[4] = F[1];
+        W6[5] = F[2];
+#pragma unroll
+        for (int e = 0; e < 6; ++e) S[e] = W6[e];
+        wave_scan_sum<6>(S);  // inclusive prefix over lanes
+        double tot[6];
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+            tot[e] = read_lane(S[e], 63);
+            S[e] = tot[e] - S[e];
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int e = 0; e < 6; ++e) s_tot[c][e] = tot[e];
+        }
+    }
+    __syncthreads();  // the chunks' sums
+    if (!real) {
+        if (c == nreal) {  // the first slot past the chain writes all its zeros
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                wave_store_run<T>(grow + (size_t)k * L + c * C, L - c * C, 0, static_cast<const T*>(nullptr), lane);
+        }
+        return;
+    }
+    for (int j = c + 1; j < nreal; ++j) {
+#pragma unroll
+        for (int e = 0; e < 6; ++e) S[e] += s_tot[j][e];
+    }
+    if (cnt > 0) {
+        // S becomes the suffix from the lane's atom 1: add the lane's own
+        // atoms, drop atom 0.
+        {
+            const double g0 = (double)gr[0], g1 = (double)gr[1], g2 = (double)gr[2];
+            S[0] += W6[0] - (f[10] * g2 - f[11] * g1);
+            S[1] += W6[1] - (f[11] * g0 - f[9] * g2);
+            S[2] += W6[2] - (f[9] * g1 - f[10] * g0);
+            S[3] += W6[3] - g0;
+            S[4] += W6[4] - g1;
+            S[5] += W6[5] - g2;
+        }
+        T out[3][R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double cs = s_cs[(6 * q + 2 * k) * 64 + lane];
+                const double sn = s_cs[(6 * q + 2 * k + 1) * 64 + lane];
+                if (k == 0) frame_step<0>(f, cs, sn);
+                if (k == 1) frame_step<1>(f, cs, sn);
+                if (k == 2) frame_step<2>(f, cs, sn);
+                const int j = 3 * q + k + 1;
+                const double x0 = f[9], x1 = f[10], x2 = f[11];
+                const double v = f[0] * (S[0] - (x1 * S[5] - x2 * S[4])) +
+                                 f[1] * (S[1] - (x2 * S[3] - x0 * S[5])) +
+                                 f[2] * (S[2] - (x0 * S[4] - x1 * S[3]));
+                const int r = rl + q;
+                const bool zero = (r >= n) || (r == n - 1 && k > 0);
+                out[k][q] = zero ? (T)0 : (T)v;
+                if (j < 3 * R) {
+                    const double g0 = (double)gr[3 * j], g1 = (double)gr[3 * j + 1],
+                                 g2 = (double)gr[3 * j + 2];
+                    S[0] -= x1 * g2 - x2 * g1;
+                    S[1] -= x2 * g0 - x0 * g2;
+                    S[2] -= x0 * g1 - x1 * g0;
+                    S[3] -= g0;
+                    S[4] -= g1;
+                    S[5] -= g2;
+                }
+            }
+        }
+        bool bad = false;
+        const int crow = min(max(L - rl, 0), R);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) bad |= (out[k][q] != out[k][q]);
+            lane_store<T, R>(grow + (size_t)k * L + rl, crow, out[k]);
+        }
+        if (bad) set_status(a.status, TPL_A2B_STATUS_NAN);
+    } else {
+        const int crow = min(max(L - rl, 0), R);
+        if (crow > 0) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) lane_store_zero<T, R>(grow + (size_t)k * L + rl, crow);
+        }
+    }
+}
+
 __global__ void a2b_zero_words(uint32_t* p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0u;
@@ -1461,6 +1773,56 @@ KernelFn fwd_kernel<double>(int r) {
         default: return a2b_fwd_chunk<double, 4>;
     }
 }
+// Chain kernels: W = slots per chain (1..kChainMax), dynamic LDS at most
+// kChainMaxLds (W waves' staging).
+constexpr size_t kChainMaxLds = 64u << 10;
+template <typename T, int R>
+KernelFn fwd_chain_w(int w) {
+    switch (w) {
+        case 1: return a2b_fwd_chain<T, R, 1>;
+        case 2: return a2b_fwd_chain<T, R, 2>;
+        case 3: return a2b_fwd_chain<T, R, 3>;
+        default: return a2b_fwd_chain<T, R, 4>;
+    }
+}
+template <typename T>
+KernelFn fwd_chain_kernel(int r, int w) {
+    switch (r) {
+        case 1: return fwd_chain_w<T, 1>(w);
+        case 2: return fwd_chain_w<T, 2>(w);
+        case 4: return fwd_chain_w<T, 4>(w);
+        default: return fwd_chain_w<T, 8>(w);
+    }
+}
+template <>
+KernelFn fwd_chain_kernel<double>(int r, int w) {
+    switch (r) {
+        case 1: return fwd_chain_w<double, 1>(w);
+        case 2: return fwd_chain_w<double, 2>(w);
+        default: return fwd_chain_w<double, 4>(w);
+    }
+}
+template <typename T, int R>
+KernelFn bwd_chain_w(int w) {
+    switch (w) {
+        case 1: return a2b_bwd_chain<T, R, 1>;
+        case 2: return a2b_bwd_chain<T, R, 2>;
+        case 3: return a2b_bwd_chain<T, R, 3>;
+        default: return a2b_bwd_chain<T, R, 4>;
+    }
+}
+template <typename T>
+KernelFn bwd_chain_kernel(int r, int w) {
+    switch (r) {
+        case 1: return bwd_chain_w<T, 1>(w);
+        case 2: return bwd_chain_w<T, 2>(w);
+        default: return bwd_chain_w<T, 4>(w);
+    }
+}
+template <>
+KernelFn bwd_chain_kernel<double>(int r, int w) {
+    return r == 1 ? bwd_chain_w<double, 1>(w) : bwd_chain_w<double, 2>(w);
+}
 template <typename T>
 KernelFn bwd_kernel(int r) {
     switch (r) {
@@ -1532,6 +1894,44 @@ void* scratch_address() {
 int launch(KernelFn k, const ChunkArgs& a, size_t lds, hipStream_t stream) {
     const dim3 grid((unsigned)(8 * a.bx * a.nc)), block(64);
     hipLaunchKernelGGL(k, grid, block, lds, stream, a);
+    return hipGetLastError() == hipSuccess ? TPL_A2B_OK : TPL_A2B_ERR_LAUNCH;
+}
+
+// Workgroups of kernel k (threads each, dynamic LDS lds) that the device
+// holds at once: the per-CU occupancy times the CUs, cached per device and
+// kernel; 0 when the runtime cannot say.
+int resident_workgroups(KernelFn k, int threads, size_t lds) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, const void*, int, size_t>, int> cache;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    const auto key = std::make_tuple(dev, reinterpret_cast<const void*>(k), threads, lds);
+    std::lock_guard<std::mutex> lock(mu);
+    const auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k), threads, lds) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    return cache[key] = per_cu * cus;
+}
+
+// The chain kernels take chains of at most kChainMax chunks whose waves'
+// staging (lds each) fits, one workgroup of a.nc waves per chain, and only
+// when every chain's workgroup is resident from the start.  Otherwise the
+// chunk kernels, whose one-wave workgroups put the work first: at B=1024 of
+// lengths 30-1000 with 4 residues per lane the backward's chain workgroups
+// (218 VGPRs, two per CU) ran in two rounds behind waves with nothing to do,
+// 16.3 -> 21.2 us; where they all fit at once they win (steps of fwd+bwd:
+// 256 x 700 residues 24.6 -> 17.0 us, 512 x 50-500 22.7 -> 17.3 us, 32 x 200
+// 14.9 -> 10.9 us, each at the best config of either form).
+bool chain_fits(KernelFn k, const ChunkArgs& a, size_t lds) {
+    if (a.nc > kChainMax || (size_t)a.nc * lds > kChainMaxLds) return false;
+    return a.B <= resident_workgroups(k, 64 * a.nc, a.nc * lds);
+}
+int launch_chain(KernelFn k, const ChunkArgs& a, size_t lds, hipStream_t stream) {
+    hipLaunchKernelGGL(k, dim3((unsigned)a.B), dim3(64 * a.nc), a.nc * lds, stream, a);
     return hipGetLastError() == hipSuccess ? TPL_A2B_OK : TPL_A2B_ERR_LAUNCH;
 }
 
@@ -1610,6 +2010,10 @@ int tpl_a2b_forward(int dtype, const void* angles, void* coords, const int32_t* 
     bind_sync(a, sync, sync_layout(batch, angles_stride, cfg).fwd);
     const bool f32 = dtype == TPL_A2B_F32;
     const size_t lds = (size_t)9 * 64 * R * (f32 ? 4 : 8);
+    if (a.nc <= kChainMax) {
+        const KernelFn kc = f32 ? fwd_chain_kernel<float>(R, a.nc) : fwd_chain_kernel<double>(R, a.nc);
+        if (chain_fits(kc, a, lds)) return launch_chain(kc, a, lds, static_cast<hipStream_t>(stream));
+    }
     return launch(f32 ? fwd_kernel<float>(R) : fwd_kernel<double>(R), a, lds,
                   static_cast<hipStream_t>(stream));
 }
@@ -1635,7 +2039,12 @@ int tpl_a2b_backward(int dtype, void* grad_angles, const void* grad_coords, cons
     a.ckpt = const_cast<double*>(static_cast<const double*>(state));
     bind_sync(a, sync, sync_layout(batch, angles_stride, cfg).bwd);
     const bool f32 = dtype == TPL_A2B_F32;
-    return launch(f32 ? bwd_kernel<float>(R) : bwd_kernel<double>(R), a, (size_t)6 * R * 64 * sizeof(double),
+    const size_t lds = (size_t)6 * R * 64 * sizeof(double);
+    if (a.nc <= kChainMax) {
+        const KernelFn kc = f32 ? bwd_chain_kernel<float>(R, a.nc) : bwd_chain_kernel<double>(R, a.nc);
+        if (chain_fits(kc, a, lds)) return launch_chain(kc, a, lds, static_cast<hipStream_t>(stream));
+    }
+    return launch(f32 ? bwd_kernel<float>(R) : bwd_kernel<double>(R), a, lds,
                   static_cast<hipStream_t>(stream));
 }
 
@@ -1651,6 +2060,21 @@ int tpl_a2b_forward_noalloc(int dtype, const void* angles, void* coords, const i
     if (!angles || !coords || !lengths) return TPL_A2B_ERR_ARG;
     const SyncRegion r = sync_layout(batch, angles_stride, cfg).fwd;
     if (r.end > kScratchBytes) return TPL_A2B_ERR_SCRATCH;
+    const int R = cfg->fwd_res_per_lane;
+    ChunkArgs a = base_args(batch, angles_stride, 64 * R, cfg);
+    a.angles = angles;
+    a.coords = coords;
+    a.lengths = lengths;
+    a.status = status;
+    a.ckpt = nullptr;
+    const bool f32 = dtype == TPL_A2B_F32;
+    const size_t lds = (size_t)9 * 64 * R * (f32 ? 4 : 8);
+    // Chains of at most kChainMax chunks need no look-back words (the same
+    // kernel as tpl_a2b_forward's).
+    if (a.nc <= kChainMax) {
+        const KernelFn kc = f32 ? fwd_chain_kernel<float>(R, a.nc) : fwd_chain_kernel<double>(R, a.nc);
+        if (chain_fits(kc, a, lds)) return launch_chain(kc, a, lds, static_cast<hipStream_t>(stream));
+    }
     void* base = scratch_address();
     if (!base) return TPL_A2B_ERR_HIP;
     // Tickets, done counters and records must start at zero.  Every call
@@ -1659,16 +2083,7 @@ int tpl_a2b_forward_noalloc(int dtype, const void* angles, void* coords, const i
     // kernel (capture-safe).
     rc = tpl_a2b_sync_clear(base, r.end, stream);
     if (rc) return rc;
-    const int R = cfg->fwd_res_per_lane;
-    ChunkArgs a = base_args(batch, angles_stride, 64 * R, cfg);
-    a.angles = angles;
-    a.coords = coords;
-    a.lengths = lengths;
-    a.status = status;
-    a.ckpt = nullptr;
     bind_sync(a, base, r);
-    const bool f32 = dtype == TPL_A2B_F32;
-    const size_t lds = (size_t)9 * 64 * R * (f32 ? 4 : 8);
     return launch(f32 ? fwd_kernel<float>(R) : fwd_kernel<double>(R), a, lds,
                   static_cast<hipStream_t>(stream));
 }

@@ -261,6 +261,16 @@ def probe(args):
     print("probe done", flush=True)
 
 
+def _kernel_side(name):
+    """"forward" or "backward" for the Angles2Backbone kernels of either form
+    (chunk kernels, or one workgroup per chain), else None."""
+    if "a2b_fwd_chunk" in name or "a2b_fwd_chain" in name:
+        return "forward"
+    if "a2b_bwd_chunk" in name or "a2b_bwd_chain" in name:
+        return "backward"
+    return None
+
+
 def _pmc_medians(outdir, counter, last=5):
     """({kernel name: median counter value over its dispatches}, median over
     the last `last` dispatches) from a rocprofv3 --output-format csv run."""
@@ -284,8 +294,8 @@ def _pmc_medians(outdir, counter, last=5):
 
 
 def _trace_avg_ms(outdir):
-    """{"forward": ms, "backward": ms}: mean end - start of the chunk kernels'
-    dispatches in a rocprofv3 --kernel-trace csv run."""
+    """{"forward": ms, "backward": ms}: mean end - start of the Angles2Backbone
+    kernels' dispatches in a rocprofv3 --kernel-trace csv run."""
     import csv
     import glob
     durs = {"forward": [], "backward": []}
@@ -293,7 +303,7 @@ def _trace_avg_ms(outdir):
         with open(f, newline="") as fh:
             for row in csv.DictReader(fh):
                 name = row.get("Kernel_Name", "")
-                k = "forward" if "a2b_fwd_chunk" in name else ("backward" if "a2b_bwd_chunk" in name else None)
+                k = _kernel_side(name)
                 if k:
                     durs[k].append((int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) * 1e-6)
     if not (durs["forward"] and durs["backward"]):
@@ -329,8 +339,8 @@ def traffic(args, cfg):
             if "probe done" not in r.stdout:
                 return None
             med, calib = _pmc_medians(od, counter)
-            fwd = [v for k, v in med.items() if "a2b_fwd_chunk" in k]
-            bwd = [v for k, v in med.items() if "a2b_bwd_chunk" in k]
+            fwd = [v for k, v in med.items() if _kernel_side(k) == "forward"]
+            bwd = [v for k, v in med.items() if _kernel_side(k) == "backward"]
             if not (fwd and bwd and calib):
                 return None
             scale = 262144.0 / calib  # the calibration kernel reads and writes 256 MiB

@@ -73,9 +73,12 @@ enum tpl_a2b_error {
 /* Launch shape of the chunk kernels.  A forward chunk is 64*fwd_res_per_lane
  * consecutive residues of one chain and a backward chunk 64*bwd_res_per_lane;
  * one 64-lane wave takes one chunk, and the chunks of one chain are chained
- * by a decoupled look-back.  The backward chunk is also the checkpoint
- * stride: the forward saves the FP64 frame at every backward-chunk boundary,
- * so forward and backward of one step must use the same config. */
+ * by a decoupled look-back.  When a chain has at most 4 chunks and every
+ * chain's workgroup fits on the device at once, one workgroup takes a whole
+ * chain instead (a wave per chunk, the chunks meeting in LDS; the sync words
+ * go unused).  The backward chunk is also the checkpoint stride: the forward
+ * saves the FP64 frame at every backward-chunk boundary, so forward and
+ * backward of one step must use the same config. */
 typedef struct tpl_a2b_config {
     int32_t fwd_res_per_lane; /* 1, 2, 4 or 8 (TPL_A2B_F64: up to 4) */
     int32_t bwd_res_per_lane; /* 1, 2 or 4 (TPL_A2B_F64: up to 2) */

@@ -30,6 +30,10 @@
 //     per-atom state are read back), sums (x x g, g) over its atoms, takes
 //     the suffix sums over lanes (DPP) and later chunks (look-back), and
 //     walks its atoms once more for the gradients.
+// Chain kernels (a2b_fwd_chain / a2b_bwd_chain): the same chunk steps with
+// one workgroup per chain of at most 4 chunks, the chunks meeting in LDS
+// instead of through the look-back; used when every chain's workgroup is
+// resident at once.
 // Plain kernels (a2b_fwd_plain / a2b_bwd_plain) restate the reference GPU
 // algorithm without the O(L^2) derivative tensor: they are the baseline.
 #include <hip/hip_runtime.h>

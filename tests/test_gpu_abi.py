@@ -142,3 +142,58 @@ def test_first_calls_in_a_fresh_process_do_not_time_out():
         assert status == [0, 0], f"config {cfg}: status forward/backward {status}"
         assert dirty == 0, f"config {cfg}: look-back buffer left {dirty} nonzero bytes"
     assert res[-1][2] is True
+
+
+def _abi_step(lib, a, n, g, cfg, dtype):
+    """Forward and backward through the C ABI (FP64 I/O when dtype is
+    torch.float64): coordinates, angle gradients and the status word."""
+    dev = "cuda"
+    dt = 1 if dtype == torch.float64 else 0
+    A = torch.tensor(a, dtype=dtype, device=dev)
+    N = torch.tensor(n, device=dev)
+    G = torch.tensor(g, dtype=dtype, device=dev)
+    B, _, L = a.shape
+    c = abi.Config(*cfg, 0, 0)
+    st = torch.empty(max(1, lib.tpl_a2b_state_bytes(dt, B, L, ctypes.byref(c))), dtype=torch.uint8, device=dev)
+    sy = torch.zeros(max(1, lib.tpl_a2b_sync_bytes(dt, B, L, ctypes.byref(c))), dtype=torch.uint8, device=dev)
+    X = torch.full((B, 9 * L), float("nan"), dtype=dtype, device=dev)
+    GA = torch.full((B, 3, L), float("nan"), dtype=dtype, device=dev)
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert lib.tpl_a2b_forward(dt, _p(A), _p(X), _p(N), _p(st), _p(sy), _p(status), B, L, ctypes.byref(c),
+                               stream) == 0
+    assert lib.tpl_a2b_backward(dt, _p(GA), _p(G), _p(A), _p(N), _p(st), _p(sy), _p(status), B, L,
+                                ctypes.byref(c), stream) == 0
+    torch.cuda.synchronize()
+    assert int(sy.count_nonzero()) == 0
+    return X.cpu().numpy(), GA.cpu().numpy(), int(status.item())
+
+
+def test_chain_and_chunk_kernels_agree():
+    """The same 64 chains (L=500, config (2,2): at most 4 chunks of 128
+    residues per chain, FP64 I/O) alone, where every chain's workgroup is
+    resident and the chain kernels run, and at the head of a batch of 1100
+    (the rest empty), where the chunk kernels and their look-back run.  Both
+    match each other to 1e-12 of the largest value (they compose the chunk
+    aggregates in different orders) and the oracle."""
+    lib = abi.load()
+    lengths = [500, 499, 384, 383, 257, 256, 255, 129, 128, 127, 1, 0] + list(range(13, 500, 9))[:52]
+    a, n, g = synthetic.custom(lengths, 500, seed=61)
+    a64, g64 = a.astype(np.float64), g.astype(np.float64)
+    print(f"chain vs chunk kernels: B={len(lengths)} and 1100, L=500, cfg (2,2), f64", flush=True)
+    x1, ga1, s1 = _abi_step(lib, a64, n, g64, (2, 2), torch.float64)
+    pad = 1100 - len(lengths)
+    x2, ga2, s2 = _abi_step(lib, np.concatenate([a64, np.zeros((pad, 3, 500))]),
+                            np.concatenate([n, np.zeros(pad, dtype=n.dtype)]),
+                            np.concatenate([g64, np.zeros((pad, 9 * 500))]), (2, 2), torch.float64)
+    assert s1 == 0 and s2 == 0
+    x2, ga2 = x2[:len(lengths)], ga2[:len(lengths)]
+    assert np.abs(x1 - x2).max() <= 1e-12 * np.abs(x1).max()
+    assert np.abs(ga1 - ga2).max() <= 1e-12 * np.abs(ga1).max()
+    xo = O.forward(a64, n)
+    go = O.backward_adjoint(a64, n, g64)
+    assert np.abs(x1 - xo).max() <= 1e-9 * np.abs(xo).max()
+    for b in range(len(lengths)):
+        if n[b] >= 2:
+            assert np.abs(ga1[b] - go[b]).max() <= 1e-9 * np.abs(go[b]).max()
+        assert np.all(x1[b, 9 * n[b]:] == 0) and np.all(ga1[b, :, n[b]:] == 0)

@@ -155,7 +155,9 @@ int tpl_a2b_backward(int dtype, void *grad_angles, const void *grad_coords, cons
  * counters, 2 * 8 * ceil(ceil(batch / 8) / 64) * 256 bytes, then the
  * records, batch * ceil(12 * nc / 32) * 256 bytes with
  * nc = ceil(angles_stride / (64 * fwd_res_per_lane)); e.g. batch 1024 and
- * angles_stride 1000 take 0.53 MB with 4 residues per lane.  Every call on
+ * angles_stride 1000 take 0.53 MB with 4 residues per lane.  (Where one
+ * workgroup takes a whole chain, as in tpl_a2b_forward, no look-back words
+ * are used.)  Every call on
  * one GPU shares that array, so calls that overlap in time on one GPU
  * (different streams, or different threads without ordering) race and must
  * not be made.  Each call first

@@ -1778,8 +1778,11 @@ KernelFn fwd_kernel<double>(int r) {
     }
 }
 // Chain kernels: W = slots per chain (1..kChainMax), dynamic LDS at most
-// kChainMaxLds (W waves' staging).
-constexpr size_t kChainMaxLds = 64u << 10;
+// kChainMaxLds (W waves' staging), so that with the static LDS (sincos table,
+// aggregates: under 1.5 KB) a workgroup stays within 64 KB.  The largest
+// chain staging in use is 54 KB (FP32 forward with 8 residues per lane and 3
+// slots, or FP64 with 4 and 3).
+constexpr size_t kChainMaxLds = 56u << 10;
 template <typename T, int R>
 KernelFn fwd_chain_w(int w) {
     switch (w) {
